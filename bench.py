@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Benchmark of the CA3DMM local compute path on one MI355X.
+
+Workload: the plain-FP64 configuration -- C = op(A) * op(B) with
+M = N = K = 8192, FP64 entries uniform[-1,1] (fixed seed), for all four
+transA/transB combinations, through the drop-in hip_dgemm (the Cannon local
+DGEMM of a 1x1x1 grid).  One step = the four products.  Inputs are resident
+in HBM before timing starts.
+
+Prints ONE JSON line (last line of stdout).  The metric counts algorithmic
+flops 2*M*N*K per product.  `roofline` is measured live with HIP events on
+the library stream; `cpu_baseline` times the reference's CPU local DGEMM
+(cblas_dgemm NN, here through numpy's OpenBLAS) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 MFMA
+METRIC = "FP64-accurate GEMM TFLOP/s: DGEMM-MFMA vs INT8-Ozaki at equal max rel err"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--cases", default="NN,NT,TN,TT")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build runs the local compute path on a "
+              "single MI355X (the multi-GPU exchange is not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from ca3dmm_amd import device as hip
+
+    n = args.n
+    cases = [(int(c[0] == "T"), int(c[1] == "T"), c) for c in args.cases.split(",")]
+    log(f"bench: generating {n}x{n} FP64 operands")
+    rnd = np.random.default_rng(20221113)
+    A = rnd.uniform(-1.0, 1.0, (n, n))
+    B = rnd.uniform(-1.0, 1.0, (n, n))
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.zeros((n, n))
+    hip.sync()
+
+    hip.set_sync(False)
+
+    def step():
+        for ta, tb, _ in cases:
+            hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
+
+    for w in range(args.warmup):
+        step()
+        hip.sync()
+        log(f"bench: warmup {w + 1}/{args.warmup} done")
+
+    # accuracy of the last product of the warm-up against the double-double oracle
+    err_stats = None
+    try:
+        from oracle import oracle as O
+        ta, tb, _ = cases[-1]
+        hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
+        hip.sync()
+        got = dC.numpy()
+        ii = rnd.integers(0, n, 512)
+        jj = rnd.integers(0, n, 512)
+        Af, Bf = np.asfortranarray(A), np.asfortranarray(B)
+        ref, ab = O.dd_sample(ta, tb, n, n, n, Af, n, Bf, n, ii, jj)
+        e = np.abs(got[ii, jj] - ref)
+        err_stats = {"max_err_over_absAabsB": float(np.max(e / ab)),
+                     "max_rel_err": float(np.max(e / np.abs(ref))),
+                     "samples": 512, "oracle": "double-double dot products"}
+        del Af, Bf
+    except Exception as ex:  # the oracle is optional for timing
+        log(f"bench: accuracy sample skipped: {ex}")
+
+    timer = hip.Timer()
+    hip.sync()
+    t0 = time.perf_counter()
+    timer.start()
+    for s in range(args.steps):
+        step()
+    gpu_ms = timer.stop()
+    hip.sync()
+    wall = time.perf_counter() - t0
+    log(f"bench: {args.steps} timed steps done in {wall:.3f} s")
+    hip.set_sync(True)
+
+    flops_per_launch = 2.0 * n * n * n
+    launches = args.steps * len(cases)
+    value = flops_per_launch * launches / wall / 1e12
+    achieved = flops_per_launch / (gpu_ms / 1e3 / launches) / 1e12
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cpu_baseline as CB
+        nb = 4096
+        log(f"bench: CPU baseline ({nb}^3 DGEMM, ~{args.cpu_seconds:.0f} s)")
+        tf, reps, el = CB.time_dgemm(nb, args.cpu_seconds)
+        cpu = {"value": round(tf, 4), "unit": "TFLOP/s", "cores": CB.threads(), "kind": "port",
+               "sample": f"{reps} x {nb}^3 NN DGEMM (reference CPU local DGEMM, cblas_dgemm "
+                         f"shape) via numpy/{CB.blas_name()}, {el:.1f} s"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"hip_dgemm FP64-MFMA {n}^3 x {{{','.join(c for _, _, c in cases)}}}, "
+                               "uniform[-1,1] FP64, 1x1x1 grid local DGEMM",
+                   "m": n, "n": n, "k": n, "method": "dgemm-mfma"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "dgemm_kernel",
+                     "avg_launch_ms": round(gpu_ms / launches, 4)},
+        "accuracy": err_stats,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,325 @@
+// HBM-bound data-movement kernels of the CA3DMM local path.
+//
+//   hip_copy_matrix_block / hip_copy_matrix_blocks
+//       mat_redist pack & unpack (src/mat_redist.c:317-348, :389-416) and the
+//       single-block copy of src/cuda_proxy.cu:130-149.  Every block of a
+//       pack or unpack goes into ONE launch: block descriptors travel as
+//       kernel arguments, each workgroup copies one tile of whole rows of one
+//       block, rows are flattened so a short row does not idle lanes, and
+//       16-byte words are used whenever every row start allows it.
+//   hip_fold_sum / hip_op_sum
+//       the reduce-scatter's element-wise sum (src/mpi_op_omp.c:6-23) applied
+//       to all kp partials in one pass in a fixed order.
+//   hip_transpose_cm / hip_dgeam
+//       utils.c:181-196 and the cublasDgeam transpose idiom
+//       (src/ca3dmm.c:1036-1059), LDS-tiled.
+//   hip_dcopy
+//       cuda_proxy.cu:176-185.
+#include <algorithm>
+#include <cstring>
+
+#include "ca3dmm_ext.h"
+#include "hip_common.h"
+
+namespace {
+
+constexpr int kCopyThreads = 256;
+constexpr int kCopyUnroll = 4;
+constexpr int kMaxDescs = 48;     // per launch; longer lists are split
+constexpr int kTileWords = kCopyThreads * kCopyUnroll * 4;
+
+struct CopyDesc {
+    const char *src;
+    char *dst;
+    int64_t lds, ldd;   // in words
+    int32_t nrow, ncol; // ncol in words
+    int32_t rows_per_tile;
+    int32_t tile_start;
+};
+
+struct CopyBatch {
+    int32_t n;
+    int32_t total_tiles;
+    CopyDesc d[kMaxDescs];
+};
+
+template <typename W>
+__global__ __launch_bounds__(kCopyThreads) void copy_blocks_kernel(const CopyBatch b)
+{
+    const int t = blockIdx.x;
+    int lo = 0, hi = b.n - 1;       // last descriptor whose tile_start <= t
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (b.d[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+    }
+    const CopyDesc &d = b.d[lo];
+    const int r0 = (t - d.tile_start) * d.rows_per_tile;
+    const int nr = min(d.rows_per_tile, d.nrow - r0);
+    const unsigned ncol = (unsigned) d.ncol;
+    const unsigned total = (unsigned) nr * ncol;
+    const W *__restrict__ s = reinterpret_cast<const W *>(d.src) + (int64_t) r0 * d.lds;
+    W *__restrict__ o = reinterpret_cast<W *>(d.dst) + (int64_t) r0 * d.ldd;
+    for (unsigned e0 = threadIdx.x; e0 < total; e0 += kCopyThreads * kCopyUnroll) {
+        W v[kCopyUnroll];
+        unsigned rr[kCopyUnroll], cc[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            unsigned e = e0 + u * kCopyThreads;
+            rr[u] = e / ncol;
+            cc[u] = e - rr[u] * ncol;
+            if (e < total) v[u] = s[(int64_t) rr[u] * d.lds + cc[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; ++u) {
+            unsigned e = e0 + u * kCopyThreads;
+            if (e < total) o[(int64_t) rr[u] * d.ldd + cc[u]] = v[u];
+        }
+    }
+}
+
+// Copy descriptors expressed in words of `w` bytes.
+int launch_copy(const hip_blk_copy_t *descs, int nblk, size_t dt, size_t w)
+{
+    CopyBatch b;
+    int i = 0;
+    while (i < nblk) {
+        memset(&b, 0, sizeof(b));
+        int tiles = 0;
+        for (; i < nblk && b.n < kMaxDescs; ++i) {
+            const hip_blk_copy_t &s = descs[i];
+            if (s.nrow <= 0 || s.ncol <= 0) continue;
+            CopyDesc &d = b.d[b.n];
+            d.src = (const char *) s.src;
+            d.dst = (char *) s.dst;
+            d.lds = (int64_t) (s.lds * dt / w);
+            d.ldd = (int64_t) (s.ldd * dt / w);
+            d.nrow = s.nrow;
+            d.ncol = (int32_t) ((size_t) s.ncol * dt / w);
+            d.rows_per_tile = std::max(1, kTileWords / d.ncol);
+            // keep the per-tile flattened index inside 32 bits
+            int64_t max_rows = (int64_t) 0x7fffffff / d.ncol;
+            if (d.rows_per_tile > max_rows) d.rows_per_tile = (int32_t) max_rows;
+            d.tile_start = tiles;
+            tiles += (d.nrow + d.rows_per_tile - 1) / d.rows_per_tile;
+            ++b.n;
+        }
+        if (b.n == 0) continue;
+        b.total_tiles = tiles;
+        hipStream_t st = ca3::stream();
+        if (w == 16) copy_blocks_kernel<uint4><<<tiles, kCopyThreads, 0, st>>>(b);
+        else if (w == 8) copy_blocks_kernel<uint2><<<tiles, kCopyThreads, 0, st>>>(b);
+        else copy_blocks_kernel<unsigned int><<<tiles, kCopyThreads, 0, st>>>(b);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return ca3::fail(e, "copy_blocks_kernel");
+    }
+    return HIP_PROXY_OK;
+}
+
+bool aligned_to(const hip_blk_copy_t &s, size_t dt, size_t w)
+{
+    return ((uintptr_t) s.src % w) == 0 && ((uintptr_t) s.dst % w) == 0 &&
+           ((size_t) s.lds * dt) % w == 0 && ((size_t) s.ldd * dt) % w == 0 &&
+           ((size_t) s.ncol * dt) % w == 0;
+}
+
+// ---------------------------------------------------------------------
+// Fold sum: dst = ((P0 + P1) + P2) + ...   (column-major blocks)
+// ---------------------------------------------------------------------
+constexpr int kMaxFold = 16;
+struct FoldArgs {
+    int32_t nsrc;
+    const double *P[kMaxFold];
+    int64_t ldp[kMaxFold];
+};
+
+template <int V>
+__global__ __launch_bounds__(256) void fold_sum_kernel(FoldArgs f, int nrow, int ncol, double *dst,
+                                                       int64_t ldd)
+{
+    // V consecutive rows per thread; nrow % V == 0 and all starts aligned.
+    const int64_t nv = (int64_t) (nrow / V) * ncol;
+    const int rowsv = nrow / V;
+    for (int64_t e = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; e < nv;
+         e += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t c = e / rowsv;
+        const int r = (int) (e - c * rowsv) * V;
+        if constexpr (V == 2) {
+            double2 s = *reinterpret_cast<const double2 *>(f.P[0] + c * f.ldp[0] + r);
+            for (int t = 1; t < f.nsrc; ++t) {
+                double2 x = *reinterpret_cast<const double2 *>(f.P[t] + c * f.ldp[t] + r);
+                s.x = __dadd_rn(s.x, x.x);
+                s.y = __dadd_rn(s.y, x.y);
+            }
+            *reinterpret_cast<double2 *>(dst + c * ldd + r) = s;
+        } else {
+            double s = f.P[0][c * f.ldp[0] + r];
+            for (int t = 1; t < f.nsrc; ++t) s = __dadd_rn(s, f.P[t][c * f.ldp[t] + r]);
+            dst[c * ldd + r] = s;
+        }
+    }
+}
+
+template <typename T>
+__global__ void op_sum_kernel(int len, const T *__restrict__ in, T *__restrict__ inout)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x)
+        inout[i] = inout[i] + in[i];
+}
+
+// ---------------------------------------------------------------------
+// C(m x n) := alpha*op(A) + beta*op(B), column-major, 64x64 LDS tiles.
+// ---------------------------------------------------------------------
+constexpr int kTT = 64;
+
+__global__ __launch_bounds__(256) void geam_kernel(int m, int n, double alpha, const double *A,
+                                                   int64_t lda, int ta, double beta,
+                                                   const double *B, int64_t ldb, int tb,
+                                                   double *Cm, int64_t ldc)
+{
+    __shared__ double ta_s[kTT][kTT + 1];
+    __shared__ double tb_s[kTT][kTT + 1];
+    const int i0 = blockIdx.x * kTT, j0 = blockIdx.y * kTT;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    // Transposed operands are read along their own columns (coalesced) into
+    // LDS and consumed along rows.
+    if (ta) {
+        for (int jj = ty; jj < kTT; jj += 4) {   // jj indexes op(A) row i0+jj
+            int gi = i0 + jj, gj = j0 + tx;
+            ta_s[jj][tx] = (gi < m && gj < n) ? A[(int64_t) gi * lda + gj] : 0.0;
+        }
+    }
+    if (beta != 0.0 && tb) {
+        for (int jj = ty; jj < kTT; jj += 4) {
+            int gi = i0 + jj, gj = j0 + tx;
+            tb_s[jj][tx] = (gi < m && gj < n) ? B[(int64_t) gi * ldb + gj] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int jj = ty; jj < kTT; jj += 4) {
+        int gi = i0 + tx, gj = j0 + jj;
+        if (gi >= m || gj >= n) continue;
+        double a = ta ? ta_s[tx][jj] : A[(int64_t) gj * lda + gi];
+        double v = __dmul_rn(alpha, a);
+        if (beta != 0.0) {
+            double b = tb ? tb_s[tx][jj] : B[(int64_t) gj * ldb + gi];
+            v = __dadd_rn(v, __dmul_rn(beta, b));
+        }
+        Cm[(int64_t) gj * ldc + gi] = v;
+    }
+}
+
+__global__ void dcopy_kernel(int n, const double *x, int incx, double *y, int incy)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        y[(int64_t) i * incy] = x[(int64_t) i * incx];
+}
+
+unsigned grid_for(int64_t work, int per_block)
+{
+    int64_t b = (work + per_block - 1) / per_block;
+    int64_t cap = (int64_t) ca3::cu_count() * 32;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (unsigned) b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hip_copy_matrix_blocks(size_t dt_size, int nblk, const hip_blk_copy_t *descs)
+{
+    if (dt_size != 4 && dt_size != 8)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: dt_size must be 4 or 8 (got %zu)", dt_size);
+    if (nblk < 0 || (nblk > 0 && !descs)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: bad list");
+    size_t w = 16;
+    for (int i = 0; i < nblk; ++i) {
+        const hip_blk_copy_t &s = descs[i];
+        if (s.nrow < 0 || s.ncol < 0 || (s.nrow > 0 && s.ncol > 0 && (!s.src || !s.dst)))
+            return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: block %d invalid", i);
+        if (s.nrow > 0 && s.ncol > 0 && (s.lds < s.ncol || s.ldd < s.ncol))
+            return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: block %d leading dim < ncol", i);
+        if (s.nrow > 0 && s.ncol > 0 && !aligned_to(s, dt_size, w)) w = dt_size;
+    }
+    int rc = launch_copy(descs, nblk, dt_size, w);
+    if (rc) return rc;
+    return ca3::finish();
+}
+
+int hip_copy_matrix_block(size_t dt_size, const int nrow, const int ncol, const void *src,
+                          const int lds, void *dst, const int ldd)
+{
+    hip_blk_copy_t d;
+    d.src = src; d.dst = dst; d.lds = lds; d.ldd = ldd; d.nrow = nrow; d.ncol = ncol;
+    return hip_copy_matrix_blocks(dt_size, 1, &d);
+}
+
+int hip_fold_sum(int nsrc, const double *const *P, const int *ldp, int nrow, int ncol,
+                 double *dst, int ldd)
+{
+    if (nsrc < 1 || nsrc > kMaxFold || !P || !ldp || !dst || nrow < 0 || ncol < 0 || ldd < nrow)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "fold_sum: bad arguments (nsrc=%d)", nsrc);
+    if (nrow == 0 || ncol == 0) return HIP_PROXY_OK;
+    FoldArgs f;
+    memset(&f, 0, sizeof(f));
+    f.nsrc = nsrc;
+    bool vec = (nrow % 2 == 0) && ((uintptr_t) dst % 16 == 0) && (ldd % 2 == 0);
+    for (int t = 0; t < nsrc; ++t) {
+        if (!P[t] || ldp[t] < nrow) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "fold_sum: source %d", t);
+        f.P[t] = P[t];
+        f.ldp[t] = ldp[t];
+        vec = vec && ((uintptr_t) P[t] % 16 == 0) && (ldp[t] % 2 == 0);
+    }
+    int64_t work = (int64_t) nrow * ncol / (vec ? 2 : 1);
+    unsigned g = grid_for(work, 256 * 4);
+    if (vec) fold_sum_kernel<2><<<g, 256, 0, ca3::stream()>>>(f, nrow, ncol, dst, ldd);
+    else fold_sum_kernel<1><<<g, 256, 0, ca3::stream()>>>(f, nrow, ncol, dst, ldd);
+    return ca3::finish();
+}
+
+int hip_op_sum(int dtype, int len, const void *invec, void *inoutvec)
+{
+    if (len < 0 || (len > 0 && (!invec || !inoutvec)))
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "op_sum: bad arguments");
+    if (len == 0) return HIP_PROXY_OK;
+    unsigned g = grid_for(len, 256 * 4);
+    hipStream_t st = ca3::stream();
+    if (dtype == 0) op_sum_kernel<double><<<g, 256, 0, st>>>(len, (const double *) invec, (double *) inoutvec);
+    else if (dtype == 1) op_sum_kernel<float><<<g, 256, 0, st>>>(len, (const float *) invec, (float *) inoutvec);
+    else if (dtype == 2) op_sum_kernel<int><<<g, 256, 0, st>>>(len, (const int *) invec, (int *) inoutvec);
+    else return ca3::fail_msg(HIP_PROXY_ERR_ARG, "op_sum: dtype %d", dtype);
+    return ca3::finish();
+}
+
+int hip_dgeam(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n,
+              const double alpha, const double *A, const int ldA, const double beta,
+              const double *B, const int ldB, double *C, const int ldC)
+{
+    int ta = (transA != HipNoTrans), tb = (transB != HipNoTrans);
+    if (m < 0 || n < 0 || !C || ldC < std::max(1, m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgeam: bad C");
+    if (m == 0 || n == 0) return HIP_PROXY_OK;
+    if (!A || ldA < std::max(1, ta ? n : m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgeam: bad A");
+    if (beta != 0.0 && (!B || ldB < std::max(1, tb ? n : m)))
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgeam: bad B");
+    dim3 grid((m + kTT - 1) / kTT, (n + kTT - 1) / kTT);
+    if (grid.y > 65535) return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "dgeam: n too large");
+    geam_kernel<<<grid, 256, 0, ca3::stream()>>>(m, n, alpha, A, ldA, ta, beta, B, ldB, tb, C, ldC);
+    return ca3::finish();
+}
+
+int hip_transpose_cm(int nrow, int ncol, const double *A, int ldA, double *AT, int ldAT)
+{
+    return hip_dgeam(HipTrans, HipNoTrans, nrow, ncol, 1.0, A, ldA, 0.0, nullptr, 1, AT, ldAT);
+}
+
+int hip_dcopy(const int n, const double *x, const int incx, double *y, const int incy)
+{
+    if (n <= 0) return HIP_PROXY_OK;
+    if (!x || !y || incx == 0 || incy == 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dcopy: bad arguments");
+    if (incx < 0 || incy < 0) return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "dcopy: negative increments");
+    if (incx == 1 && incy == 1) return hip_copy_matrix_block(8, 1, n, x, n, y, n);
+    dcopy_kernel<<<grid_for(n, 1024), 256, 0, ca3::stream()>>>(n, x, incx, y, incy);
+    return ca3::finish();
+}
+
+}  // extern "C"

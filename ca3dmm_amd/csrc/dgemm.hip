@@ -1,0 +1,128 @@
+// FP64 MFMA DGEMM for gfx950 -- the local DGEMM of CA3DMM's Cannon step.
+//
+// Replaces cublasDgemm behind cuda_cublas_dgemm (src/cuda_proxy.cu:211-230),
+// called at src/cannon.c:418,586,616,665 and src/dev_type.c:212.
+//
+// Column-major C := alpha*op(A)*op(B) + beta*C, optionally followed by the
+// k-layer partial-C fold of the reduce-scatter (src/ca3dmm.c:1311-1324):
+// C := ((P0 + P1) + ...) + (alpha*op(A)*op(B) + beta*C).
+//
+// Design (MI355X / CDNA4):
+//  * v_mfma_f64_16x16x4_f64.  A workgroup of 4 wave64s owns a 128x128 tile
+//    of C; each wave owns 64x64 = 4x4 MFMA blocks (64 accumulators per lane).
+//  * K advances in stages of 16.  A and B stage tiles are held in LDS as
+//    [k][m] and [k][n] (m / n fastest), double-buffered: the next stage's
+//    global loads are issued before the current stage's MFMAs and written to
+//    the other buffer afterwards, one barrier per stage.
+//  * Transposes are fused into the loads: an operand is either contiguous
+//    along m/n (A as stored, B^T) or along k (A^T, B as stored); both map to
+//    coalesced global reads, and the LDS image is identical, so the MFMA
+//    loop is the same for all four cases.
+//  * LDS addresses are XOR-swizzled per k row, so both the transposing
+//    stores (16 k rows, one column) and the MFMA operand reads (16
+//    consecutive m or n, two k rows per half wave) hit distinct banks.
+//  * Tiles are visited XCD-contiguously (workgroup b runs on XCD b % 8) and
+//    in groups of 8 tile-rows, so the tiles resident on one XCD share A and
+//    B panels in that XCD's L2.
+#include <algorithm>
+#include <cstring>
+
+#include "ca3dmm_ext.h"
+#include "dgemm_tile.h"
+#include "hip_common.h"
+
+namespace {
+
+using namespace ca3::gemm;
+
+template <int TA, int TB>
+__global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(GemmArgs g)
+{
+    __shared__ double As[2][kBK * kBM];
+    __shared__ double Bs[2][kBK * kBN];
+
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn);
+    const int m0 = tm * kBM, n0 = tn * kBN;
+
+    // A as stored (TA = 0) is contiguous along m; B as stored (TB = 0) is
+    // contiguous along k, B^T along n.
+    OperandLoader<!TA> la;
+    OperandLoader<TB> lb;
+    la.init(g.A, g.lda, m0, g.M);
+    lb.init(g.B, g.ldb, n0, g.N);
+
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    mainloop(la, lb, 0, g.K, As, Bs, acc);
+
+    epilogue(g, m0, n0, acc);
+}
+
+template <int TA, int TB>
+int launch(const GemmArgs &g)
+{
+    const int tiles = ((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
+    dgemm_kernel<TA, TB><<<tiles, kThreads, 0, ca3::stream()>>>(g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ca3::fail(e, "dgemm_kernel");
+    return HIP_PROXY_OK;
+}
+
+int run(hip_proxy_enum_t transA, hip_proxy_enum_t transB, int m, int n, int k, double alpha,
+        const double *A, int ldA, const double *B, int ldB, double beta, double *C, int ldC,
+        int nsrc, const double *const *P, const int *ldp)
+{
+    const int ta = (transA != HipNoTrans), tb = (transB != HipNoTrans);
+    if (m < 0 || n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: negative size");
+    if (m == 0 || n == 0) return HIP_PROXY_OK;
+    if (!C || ldC < std::max(1, m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: bad C / ldC");
+    if (k > 0) {
+        if (!A || ldA < std::max(1, ta ? k : m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: bad A / ldA");
+        if (!B || ldB < std::max(1, tb ? n : k)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: bad B / ldB");
+    }
+    if (nsrc < 0 || nsrc > kMaxFoldSrc) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: nsrc %d", nsrc);
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = m; g.N = n; g.K = k;
+    g.alpha = alpha; g.beta = beta;
+    g.A = A; g.lda = ldA; g.B = B; g.ldb = ldB; g.C = C; g.ldc = ldC;
+    g.nsrc = nsrc;
+    for (int t = 0; t < nsrc; ++t) {
+        if (!P || !ldp || !P[t] || ldp[t] < m) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: partial %d", t);
+        g.P[t] = P[t];
+        g.ldp[t] = ldp[t];
+    }
+    int rc;
+    if (!ta && !tb) rc = launch<0, 0>(g);
+    else if (!ta && tb) rc = launch<0, 1>(g);
+    else if (ta && !tb) rc = launch<1, 0>(g);
+    else rc = launch<1, 1>(g);
+    if (rc) return rc;
+    return ca3::finish();
+}
+
+}  // namespace
+
+extern "C" {
+
+int hip_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n,
+              const int k, const double alpha, const double *A, const int ldA, const double *B,
+              const int ldB, const double beta, double *C, const int ldC)
+{
+    return run(transA, transB, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, 0, nullptr, nullptr);
+}
+
+int hip_dgemm_fold(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n,
+                   const int k, const double alpha, const double *A, const int ldA,
+                   const double *B, const int ldB, const double beta, double *C, const int ldC,
+                   int nsrc, const double *const *P, const int *ldp)
+{
+    return run(transA, transB, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, nsrc, P, ldp);
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// Device building blocks of the FP64 MFMA GEMM tile, shared by the plain
+// DGEMM, the grouped (simulated-grid) GEMM and the Gram kernel.
+//
+// MFMA operand / result layout used throughout (v_mfma_f64_16x16x4_f64,
+// one wave64):  X(16x4): lane l holds X[l%16][l/16];  Y(4x16): lane l holds
+// Y[l/16][l%16];  D = X*Y (16x16): lane l, register r holds D[4*r + l/16][l%16].
+// The tile code feeds X = op(B)^T and Y = op(A)^T, so D is a block of C^T:
+// lane l, register r holds C[m = l%16][n = 4*r + l/16].  For a fixed r the 16
+// lanes of a row group then hold 16 consecutive rows of one column of C,
+// which makes the column-major stores 128-byte contiguous.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ca3 {
+namespace gemm {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBM = 128, kBN = 128, kBK = 16, kThreads = 256;
+constexpr int kMaxFoldSrc = 8;
+
+struct GemmArgs {
+    int M, N, K;
+    double alpha, beta;
+    const double *A;
+    int64_t lda;
+    const double *B;
+    int64_t ldb;
+    double *C;
+    int64_t ldc;
+    int nsrc;                       // partial-C fold in the epilogue
+    const double *P[kMaxFoldSrc];
+    int64_t ldp[kMaxFoldSrc];
+};
+
+// XOR swizzle of column x within LDS row kk (see dgemm.hip header).
+__device__ __forceinline__ int swz(int kk, int x)
+{
+    return x ^ (((kk & 1) << 4) | ((kk >> 1) & 7));
+}
+
+// Linear workgroup id -> (tile row, tile col): XCD-contiguous ranges, then
+// groups of 8 tile-rows walked column by column.
+__device__ __forceinline__ void tile_coords(int bid, int nb, int M, int N, int *tm, int *tn)
+{
+    const int tiles_m = (M + kBM - 1) / kBM, tiles_n = (N + kBN - 1) / kBN;
+    (void) tiles_n;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    constexpr int G = 8;
+    const int per = G * tiles_n;
+    const int grp = bid / per;
+    const int first = grp * G;
+    const int gsz = min(tiles_m - first, G);
+    const int r = bid - grp * per;
+    *tm = first + r % gsz;
+    *tn = r / gsz;
+}
+
+// Loads one 16 x 128 stage of an operand into registers and writes it to
+// LDS as [k][x].  XC = 1: the operand is contiguous along x (m or n);
+// XC = 0: contiguous along k.  Out-of-range elements read as 0.
+template <int XC>
+struct OperandLoader {
+    const double *base;   // element (x = this thread's first x, k = 0)
+    int64_t ld;
+    int kk0;              // XC: first k row of this thread (0/1); !XC: its k row
+    unsigned xmask;       // bit p: x of element p is in range
+    int xcol;             // XC: this thread's x
+    double r[8];
+
+    __device__ __forceinline__ void init(const double *M, int64_t ld_, int x0, int xdim)
+    {
+        const int t = threadIdx.x;
+        ld = ld_;
+        if (XC) {
+            xcol = t & 127;
+            kk0 = t >> 7;
+            xmask = (x0 + xcol < xdim) ? 0xffu : 0u;
+            base = M + (x0 + xcol);
+        } else {
+            kk0 = t & 15;
+            const int xt = t >> 4;
+            xcol = xt;
+            xmask = 0;
+#pragma unroll
+            for (int p = 0; p < 8; ++p)
+                if (x0 + xt + 16 * p < xdim) xmask |= 1u << p;
+            base = M + (int64_t) (x0 + xt) * ld;
+        }
+    }
+
+    __device__ __forceinline__ void load(int k0, int kend)
+    {
+        if (XC) {
+            const double *q = base + (int64_t) (k0 + kk0) * ld;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const bool ok = xmask && (k0 + kk0 + 2 * p < kend);
+                r[p] = ok ? q[(int64_t) (2 * p) * ld] : 0.0;
+            }
+        } else {
+            const bool kok = (k0 + kk0 < kend);
+            const double *q = base + (k0 + kk0);
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const bool ok = kok && ((xmask >> p) & 1u);
+                r[p] = ok ? q[(int64_t) (16 * p) * ld] : 0.0;
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(double *S) const
+    {
+        if (XC) {
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const int kk = kk0 + 2 * p;
+                S[kk * 128 + swz(kk, xcol)] = r[p];
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < 8; ++p) S[kk0 * 128 + swz(kk0, xcol + 16 * p)] = r[p];
+        }
+    }
+};
+
+// 16 k-steps of MFMAs on the LDS stage (4 x v_mfma_f64_16x16x4 k-slices).
+__device__ __forceinline__ void stage_mfma(const double *As, const double *Bs, f64x4 (&acc)[4][4])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15;
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ++ks) {
+        const int kk = ks * 4 + (lane >> 4);
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[kk * kBM + swz(kk, wm * 64 + i * 16 + li)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[kk * kBN + swz(kk, wn * 64 + j * 16 + li)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+    }
+}
+
+// Accumulate op(A)[m0:+128, kbeg:kend) * op(B)[kbeg:kend, n0:+128) into acc.
+template <class LA, class LB>
+__device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend,
+                                         double (*As)[kBK * kBM], double (*Bs)[kBK * kBN],
+                                         f64x4 (&acc)[4][4])
+{
+    if (kend <= kbeg) return;
+    const int nk = (kend - kbeg + kBK - 1) / kBK;
+    la.load(kbeg, kend);
+    lb.load(kbeg, kend);
+    la.store(As[0]);
+    lb.store(Bs[0]);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = (kt + 1 < nk);
+        if (more) {
+            la.load(kbeg + (kt + 1) * kBK, kend);
+            lb.load(kbeg + (kt + 1) * kBK, kend);
+        }
+        stage_mfma(As[cur], Bs[cur], acc);
+        if (more) {
+            la.store(As[cur ^ 1]);
+            lb.store(Bs[cur ^ 1]);
+        }
+        __syncthreads();
+    }
+}
+
+// C := fold(P) + (alpha*acc + beta*C) for the tile at (m0, n0).
+__device__ __forceinline__ void epilogue(const GemmArgs &g, int m0, int n0, const f64x4 (&acc)[4][4])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = m0 + wm * 64 + i * 16 + (lane & 15);
+        if (row >= g.M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = n0 + wn * 64 + j * 16 + 4 * r + (lane >> 4);
+                if (col >= g.N) continue;
+                double *c = g.C + (int64_t) col * g.ldc + row;
+                double v = __dmul_rn(g.alpha, acc[i][j][r]);
+                if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *c));
+                if (g.nsrc > 0) {
+                    double s = g.P[0][(int64_t) col * g.ldp[0] + row];
+                    for (int t = 1; t < g.nsrc; ++t) s = __dadd_rn(s, g.P[t][(int64_t) col * g.ldp[t] + row]);
+                    v = __dadd_rn(s, v);
+                }
+                *c = v;
+            }
+        }
+    }
+}
+
+}  // namespace gemm
+}  // namespace ca3

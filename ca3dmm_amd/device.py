@@ -1,0 +1,141 @@
+"""Device buffers and thin Python calls over the C-ABI.
+
+The host side of tests and benchmarks: numpy arrays in, device buffers from
+the library's own allocator (hip_malloc_dev), results copied back with
+hip_memcpy_d2h.  Matrices are column-major (Fortran order), as in the
+reference.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import BlkCopy, check, lib
+
+NOTRANS, TRANS = 0, 1
+
+
+def device_count():
+    return lib.hip_proxy_device_count()
+
+
+class DeviceArray:
+    """A device allocation holding a column-major 2-D array (or a vector)."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib.hip_malloc_dev(C.byref(p), self.nbytes), "hip_malloc_dev")
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.asarray(a)
+        out = cls(a.shape, a.dtype)
+        host = np.asfortranarray(a)
+        if out.nbytes:
+            check(lib.hip_memcpy_h2d(host.ctypes.data, out.ptr, out.nbytes), "hip_memcpy_h2d")
+        return out
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float64):
+        out = cls(shape, dtype)
+        if out.nbytes:
+            check(lib.hip_memset_dev(out.ptr, 0, out.nbytes), "hip_memset_dev")
+        return out
+
+    @classmethod
+    def filled(cls, shape, byte, dtype=np.float64):
+        out = cls(shape, dtype)
+        if out.nbytes:
+            check(lib.hip_memset_dev(out.ptr, byte, out.nbytes), "hip_memset_dev")
+        return out
+
+    def numpy(self):
+        host = np.empty(self.shape, dtype=self.dtype, order="F")
+        if self.nbytes:
+            check(lib.hip_stream_sync(None), "hip_stream_sync")
+            check(lib.hip_memcpy_d2h(self.ptr, host.ctypes.data, self.nbytes), "hip_memcpy_d2h")
+        return host
+
+    def at(self, offset_elems):
+        """Raw device address of element `offset_elems` (flat, column-major)."""
+        return self.ptr + int(offset_elems) * self.dtype.itemsize
+
+    def free(self):
+        if self.ptr:
+            lib.hip_free_dev(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def sync():
+    check(lib.hip_stream_sync(None), "hip_stream_sync")
+
+
+def set_sync(flag):
+    lib.hip_proxy_set_sync(1 if flag else 0)
+
+
+class Timer:
+    """HIP events on the library stream (where every kernel is launched)."""
+
+    def start(self):
+        check(lib.hip_timer_start(), "hip_timer_start")
+
+    def stop(self):
+        ms = C.c_double()
+        check(lib.hip_timer_stop(C.byref(ms)), "hip_timer_stop")
+        return ms.value
+
+
+def _ld(a):
+    return max(1, a.shape[0])
+
+
+def dgemm(transA, transB, m, n, k, alpha, A, B, beta, Cm, lda=None, ldb=None, ldc=None):
+    return check(lib.hip_dgemm(transA, transB, m, n, k, alpha, A.ptr, lda or _ld(A), B.ptr,
+                               ldb or _ld(B), beta, Cm.ptr, ldc or _ld(Cm)), "hip_dgemm")
+
+
+def dgemm_fold(transA, transB, m, n, k, alpha, A, B, beta, Cm, parts):
+    arr = (C.c_void_p * len(parts))(*[p.ptr for p in parts])
+    lds = (C.c_int * len(parts))(*[_ld(p) for p in parts])
+    return check(lib.hip_dgemm_fold(transA, transB, m, n, k, alpha, A.ptr, _ld(A), B.ptr, _ld(B),
+                                    beta, Cm.ptr, _ld(Cm), len(parts), arr, lds), "hip_dgemm_fold")
+
+
+def copy_blocks(dt_size, blocks):
+    """blocks: list of (src_addr, lds, dst_addr, ldd, nrow, ncol) row-major copies."""
+    arr = (BlkCopy * max(1, len(blocks)))()
+    for i, (s, lds, d, ldd, nr, nc) in enumerate(blocks):
+        arr[i] = BlkCopy(s, d, lds, ldd, nr, nc)
+    return check(lib.hip_copy_matrix_blocks(dt_size, len(blocks), arr), "hip_copy_matrix_blocks")
+
+
+def copy_matrix_block(dt_size, nrow, ncol, src_addr, lds, dst_addr, ldd):
+    return check(lib.hip_copy_matrix_block(dt_size, nrow, ncol, src_addr, lds, dst_addr, ldd),
+                 "hip_copy_matrix_block")
+
+
+def fold_sum(parts, nrow, ncol, dst, ldd=None):
+    arr = (C.c_void_p * len(parts))(*[p.ptr for p in parts])
+    lds = (C.c_int * len(parts))(*[_ld(p) for p in parts])
+    return check(lib.hip_fold_sum(len(parts), arr, lds, nrow, ncol, dst.ptr, ldd or _ld(dst)),
+                 "hip_fold_sum")
+
+
+def op_sum(invec, inoutvec):
+    code = {np.dtype(np.float64): 0, np.dtype(np.float32): 1, np.dtype(np.int32): 2}[invec.dtype]
+    n = int(np.prod(invec.shape))
+    return check(lib.hip_op_sum(code, n, invec.ptr, inoutvec.ptr), "hip_op_sum")
+
+
+def transpose_cm(nrow, ncol, A, AT):
+    return check(lib.hip_transpose_cm(nrow, ncol, A.ptr, _ld(A), AT.ptr, _ld(AT)), "hip_transpose_cm")

@@ -1,0 +1,61 @@
+/*
+ * Entry points of libca3dmm_amd with no one-to-one reference counterpart.
+ * Each either batches or fuses a reference step (cited) or adds a path the
+ * reference lacks.  Same conventions as hip_proxy.h: int status, plain
+ * device pointers, work on the library stream.
+ */
+#ifndef CA3DMM_AMD_EXT_H
+#define CA3DMM_AMD_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "hip_proxy.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One row-major block copy: nrow rows of ncol elements, src leading dim lds,
+ * dst leading dim ldd (elements). */
+typedef struct {
+    const void *src;
+    void       *dst;
+    int64_t     lds, ldd;
+    int32_t     nrow, ncol;
+} hip_blk_copy_t;
+
+/* All blocks of one mat_redist pack (mat_redist.c:317-348) or unpack
+ * (:389-416) -- or any list of block copies -- in ONE launch.  descs is a
+ * host array; dt_size 4 or 8.  Bit-exact. */
+int hip_copy_matrix_blocks(size_t dt_size, int nblk, const hip_blk_copy_t *descs);
+
+/* The reduce operator of the C reduce-scatter, mpi_op_omp.c:6-23:
+ * inoutvec[i] += invec[i]; dtype 0 = double, 1 = float, 2 = int. */
+int hip_op_sum(int dtype, int len, const void *invec, void *inoutvec);
+
+/* k-layer partial-C reduction (ca3dmm.c:1311-1324 with mpi_op_omp_sum) in
+ * one pass and a fixed order: dst = ((P[0] + P[1]) + P[2]) + ...
+ * Column-major nrow x ncol blocks; P[t] leading dim ldp[t].  dst may alias
+ * P[0].  Bit-exact against the same fold on the CPU. */
+int hip_fold_sum(int nsrc, const double *const *P, const int *ldp,
+                 int nrow, int ncol, double *dst, int ldd);
+
+/* hip_dgemm whose epilogue also absorbs k-layer partials:
+ * C := fold(P[0..nsrc-1]) + (alpha*op(A)*op(B) + beta*C), left to right.
+ * With beta == 0 and the last layer's product this equals hip_fold_sum of
+ * all kp partials bit for bit. */
+int hip_dgemm_fold(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
+                   const int m, const int n, const int k, const double alpha,
+                   const double *A, const int ldA, const double *B, const int ldB,
+                   const double beta, double *C, const int ldC,
+                   int nsrc, const double *const *P, const int *ldp);
+
+/* utils.c:181-196 (transpose_cm_mat) on the device: AT (nrow x ncol,
+ * col-major, ldAT) := transpose of A (ncol x nrow, col-major, ldA). */
+int hip_transpose_cm(int nrow, int ncol, const double *A, int ldA, double *AT, int ldAT);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,100 @@
+"""GPU: the FP64 MFMA DGEMM (cuda_cublas_dgemm replacement, src/cuda_proxy.cu:211-230)
+against the oracle.
+
+Tolerance (componentwise, for random inputs): |C - C_ref| <= k * u * (|A||B|)_ij
+with u = 2^-53, C_ref from double-double dot products; integer-valued inputs
+must match exactly (every summation order is exact on them)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+U = 2.0 ** -53
+
+
+def _mats(rnd, m, n, k, ta, tb, kind):
+    sa = (k, m) if ta else (m, k)
+    sb = (n, k) if tb else (k, n)
+    if kind == "int":
+        return (rnd.integers(-16, 17, sa).astype(np.float64), rnd.integers(-16, 17, sb).astype(np.float64))
+    return rnd.uniform(-1, 1, sa), rnd.uniform(-1, 1, sb)
+
+
+SHAPES = [(1, 1, 1), (15, 17, 3), (16, 16, 4), (128, 128, 16), (129, 130, 17), (200, 300, 500),
+          (257, 64, 1000), (64, 511, 33), (1000, 1000, 1000)]
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", SHAPES)
+def test_dgemm_integer_exact(hip, ta, tb, m, n, k):
+    rnd = np.random.default_rng(m * 31 + n * 7 + k + ta * 2 + tb)
+    A, B = _mats(rnd, m, n, k, ta, tb, "int")
+    want = (A.T if ta else A) @ (B.T if tb else B)          # exact: integers < 2^53
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.filled((m, n), 0xFF)                 # NaN garbage; beta = 0 ignores it
+    hip.dgemm(ta, tb, m, n, k, 1.0, dA, dB, 0.0, dC)
+    assert np.array_equal(dC.numpy(), want)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_dgemm_random_componentwise_bound(hip, ta, tb):
+    m, n, k = 777, 555, 1999
+    rnd = np.random.default_rng(42 + ta * 2 + tb)
+    A, B = _mats(rnd, m, n, k, ta, tb, "f")
+    Af, Bf = np.asfortranarray(A), np.asfortranarray(B)
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.zeros((m, n))
+    hip.dgemm(ta, tb, m, n, k, 1.0, dA, dB, 0.0, dC)
+    got = dC.numpy()
+    ii = rnd.integers(0, m, 3000); jj = rnd.integers(0, n, 3000)
+    ref, ab = O.dd_sample(ta, tb, m, n, k, Af, Af.shape[0], Bf, Bf.shape[0], ii, jj)
+    err = np.abs(got[ii, jj] - ref)
+    assert np.all(err <= k * U * ab), float(np.max(err / ab))
+
+
+def test_dgemm_alpha_beta_and_ld(hip):
+    rnd = np.random.default_rng(3)
+    m, n, k = 150, 140, 130
+    lda, ldb, ldc = 157, 133, 151
+    A = rnd.integers(-8, 9, (lda, k)).astype(np.float64)
+    B = rnd.integers(-8, 9, (ldb, n)).astype(np.float64)
+    Cm = rnd.integers(-8, 9, (ldc, n)).astype(np.float64)
+    want = Cm.copy()
+    want[:m] = 2.0 * (A[:m] @ B[:k]) - 3.0 * Cm[:m]
+    dA, dB, dC = (hip.DeviceArray.from_numpy(x) for x in (A, B, Cm))
+    hip.dgemm(0, 0, m, n, k, 2.0, dA, dB, -3.0, dC, lda=lda, ldb=ldb, ldc=ldc)
+    assert np.array_equal(dC.numpy(), want)          # rows m..ldc untouched
+
+
+def test_dgemm_k_zero_scales_c(hip):
+    Cm = np.arange(12.0).reshape(3, 4)
+    dC = hip.DeviceArray.from_numpy(Cm)
+    dA = hip.DeviceArray.zeros((3, 1)); dB = hip.DeviceArray.zeros((1, 4))
+    hip.dgemm(0, 0, 3, 4, 0, 1.0, dA, dB, 0.5, dC)
+    assert np.array_equal(dC.numpy(), 0.5 * Cm)
+
+
+def test_dgemm_fold_epilogue_equals_fold_sum(hip):
+    """C := (P0 + P1) + A*B equals hip_fold_sum over [P0, P1, A*B] bit for bit."""
+    rnd = np.random.default_rng(8)
+    m, n, k = 300, 260, 411
+    A, B = rnd.uniform(-1, 1, (m, k)), rnd.uniform(-1, 1, (k, n))
+    P = [rnd.uniform(-1, 1, (m, n)) * 100 for _ in range(2)]
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dP = [hip.DeviceArray.from_numpy(x) for x in P]
+    prod = hip.DeviceArray.zeros((m, n))
+    hip.dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, prod)
+    want = hip.DeviceArray.zeros((m, n))
+    hip.fold_sum(dP + [prod], m, n, want)
+    fused = hip.DeviceArray.zeros((m, n))
+    hip.dgemm_fold(0, 0, m, n, k, 1.0, dA, dB, 0.0, fused, dP)
+    assert np.array_equal(fused.numpy(), want.numpy())
+    assert np.array_equal(want.numpy(), (P[0] + P[1]) + prod.numpy())
+
+
+def test_dgemm_rejects_bad_arguments(hip):
+    from ca3dmm_amd import HipError
+    dA = hip.DeviceArray.zeros((4, 4))
+    with pytest.raises(HipError):
+        hip.dgemm(0, 0, 4, 4, 4, 1.0, dA, dA, 0.0, dA, lda=2)
