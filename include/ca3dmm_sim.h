@@ -1,0 +1,62 @@
+/*
+ * Simulated process grid: every rank of a p-rank CA3DMM grid planned and run
+ * in ONE process on ONE GPU.  No reference counterpart as an API; it stands
+ * in for `mpirun -np p` on one node (the reference's only multi-rank test
+ * method, README.md:31-33) and shows the 3D decomposition at its real
+ * per-rank shapes.
+ *
+ * Each rank's engine is built exactly as ca3dmm_engine_init(_BTB) would on
+ * that rank (same grid, blocks, redistribution lists and workspace size) and
+ * is readable through ca3dmm_sim_engine().  ca3dmm_sim_exec runs the
+ * reference's exec sequence for all ranks at once, with every exchange
+ * replaced by device-to-device copies:
+ *   redistribute A, B   pack (all ranks, one launch) -> exchange -> unpack
+ *   transpose, allgather of the replicated operand
+ *   Cannon              q rounds per rank, operands read in place
+ *   reduce-scatter      fixed-order k-layer fold (rank order)
+ *   redistribute C
+ */
+#ifndef CA3DMM_AMD_SIM_H
+#define CA3DMM_AMD_SIM_H
+
+#include "ca3dmm.h"
+
+typedef struct ca3dmm_sim *ca3dmm_sim_p;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plan all p ranks; with allocate != 0 also allocate each rank's device work
+ * buffer (allocate == 0 plans only: fields are readable, exec refuses).
+ * src_A, src_B, dst_C: 4*p ints, rank r's column-major block
+ * {srow, nrow, scol, ncol} of A, B (as stored) and C; dst_C == NULL skips
+ * the C redistribution.  For is_btb != 0, C := B^T B with B k x n (m, transA,
+ * transB and src_A are ignored).  Returns 0 or a negative status. */
+int ca3dmm_sim_init(int p, int m, int n, int k, int trans_A, int trans_B, const int *src_A,
+                    const int *src_B, const int *dst_C, const int *proc_grid, int is_btb,
+                    dev_type_t dev_type, int allocate, ca3dmm_sim_p *sim);
+
+/* Work-buffer bytes rank r's ca3dmm_engine_init would report
+ * (src/ca3dmm.c:398-423: largest child buffer + own buffers). */
+size_t ca3dmm_sim_workbuf_bytes(ca3dmm_sim_p sim, int rank);
+
+/* Rank r's engine (fields as the reference's ca3dmm_engine on that rank). */
+ca3dmm_engine_p ca3dmm_sim_engine(ca3dmm_sim_p sim, int rank);
+
+/* Run one exec of every rank.  Per-rank device pointers and leading
+ * dimensions (column-major).  For is_btb src_A / ldA are ignored. */
+int ca3dmm_sim_exec(ca3dmm_sim_p sim, const void *const *src_A, const int *ldA,
+                    const void *const *src_B, const int *ldB, void *const *dst_C, const int *ldC);
+
+/* Device address of rank r's C_out block (C_out_nrow x C_out_ncol,
+ * leading dimension C_out_nrow) after exec. */
+void *ca3dmm_sim_c_out(ca3dmm_sim_p sim, int rank);
+
+void ca3dmm_sim_free(ca3dmm_sim_p *sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

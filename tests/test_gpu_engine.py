@@ -1,0 +1,152 @@
+"""GPU: the CA3DMM engine end to end (src/ca3dmm.c exec sequence) -- single
+rank on the device, and simulated p-rank grids on one GPU -- against the
+oracle.  Integer-valued inputs make every product exact, so results must be
+bit-identical; the linear fill of examples/example_AB.c is checked with the
+reference's own criterion (relative error <= 1e-12, zero failures)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ca3dmm_amd import device as hip
+from ca3dmm_amd import engine as E
+from tests.simutil import linear_fill, run_sim, single_engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _ints(rnd, shape):
+    return np.asfortranarray(rnd.integers(-4, 5, shape).astype(np.float64))
+
+
+@pytest.mark.parametrize("force", [0, 1])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_single_rank_engine_exact(hip, monkeypatch, ta, tb, force):
+    monkeypatch.setenv("CA3DMM_FORCE_REDIST", str(force))
+    rnd = np.random.default_rng(10 + 2 * ta + tb)
+    m, n, k = 333, 257, 400
+    A = _ints(rnd, (k, m) if ta else (m, k))
+    B = _ints(rnd, (n, k) if tb else (k, n))
+    want = (A.T if ta else A) @ (B.T if tb else B)
+    eng = single_engine(m, n, k, ta, tb)
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.filled((m, n), 0xFF)
+    E.lib.ca3dmm_engine_exec(eng, dA.ptr, A.shape[0], dB.ptr, B.shape[0], dC.ptr, m)
+    assert np.array_equal(dC.numpy(), want)
+    e = eng.contents
+    assert e.n_exec == 1 and e.mp == e.np == e.kp == 1
+    E.lib.ca3dmm_engine_free(C.byref(eng))
+
+
+def test_single_rank_sub_block_output(hip):
+    """dst_C asks for rows 10..59, cols 5..24 only (redistribution of C)."""
+    rnd = np.random.default_rng(3)
+    m, n, k = 100, 40, 70
+    A, B = _ints(rnd, (m, k)), _ints(rnd, (k, n))
+    eng = single_engine(m, n, k, 0, 0, dst=(10, 50, 5, 20))
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.filled((50, 20), 0xFF)
+    E.lib.ca3dmm_engine_exec(eng, dA.ptr, m, dB.ptr, k, dC.ptr, 50)
+    assert np.array_equal(dC.numpy(), (A @ B)[10:60, 5:25])
+    E.lib.ca3dmm_engine_free(C.byref(eng))
+
+
+def test_single_rank_btb(hip):
+    rnd = np.random.default_rng(4)
+    n, k = 120, 900
+    B = _ints(rnd, (k, n))
+    eng = single_engine(n, n, k, 0, 0, btb=True)
+    dB = hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.filled((n, n), 0xFF)
+    E.lib.ca3dmm_engine_exec(eng, None, 0, dB.ptr, k, dC.ptr, n)
+    got = dC.numpy()
+    assert np.array_equal(got, B.T @ B)
+    assert np.array_equal(got, got.T)
+    E.lib.ca3dmm_engine_free(C.byref(eng))
+
+
+def test_mat_redist_single_rank(hip):
+    """A requested window of the source block (row-major view)."""
+    rnd = np.random.default_rng(5)
+    src = rnd.standard_normal((100, 50))          # row-major rows x cols
+    eng = C.POINTER(E.MatRedist)()
+    E.lib.mat_redist_engine_init(0, 0, 100, 50, 20, 10, 50, 30, E.comm_self(), 0, 8, E.DEV_TYPE_HIP,
+                                 C.byref(eng), None)
+    assert eng and eng.contents.n_proc_send == 1 and eng.contents.recv_cnt == 1500
+    ds = hip.DeviceArray.from_numpy(src.ravel())
+    dd = hip.DeviceArray.zeros((1500,))
+    E.lib.mat_redist_engine_exec(eng, ds.ptr, 50, dd.ptr, 30)
+    assert np.array_equal(dd.numpy().reshape(50, 30), src[20:70, 10:40])
+    E.lib.mat_redist_engine_free(C.byref(eng))
+
+
+def test_cannon_engine_single(hip):
+    rnd = np.random.default_rng(6)
+    m, n, k = 90, 70, 50
+    A, B, Cm = _ints(rnd, (m, k)), _ints(rnd, (k, n)), _ints(rnd, (m, n))
+    ce = C.POINTER(E.Cannon)()
+    E.lib.cannon_engine_init(m, n, k, E.comm_self(), E.DEV_TYPE_HIP, C.byref(ce), None)
+    dA, dB, dC = (hip.DeviceArray.from_numpy(x) for x in (A, B, Cm))
+    E.lib.cannon_engine_exec(ce, 2.0, 1.0, dA.ptr, dB.ptr, dC.ptr)
+    assert np.array_equal(dC.numpy(), 2.0 * (A @ B) + Cm)
+    E.lib.cannon_engine_free(C.byref(ce))
+
+
+def test_sim_example_ab_2000_p4_linear_fill(hip):
+    """BASELINE config 0 on the GPU: 4 ranks (grid 2x1x2), 2000^3 NN, the
+    example_AB linear fill, validated as example_AB.c:287-305 does."""
+    m = n = k = 2000
+    A = linear_fill(m, k, 0.19, 0.24)
+    B = linear_fill(k, n, 0.11, 0.12)
+    got, sim, w = run_sim(4, m, n, k, 0, 0, A, B, keep=True)
+    e = sim.engine(0)
+    assert (e.mp, e.np, e.kp) == (2, 1, 2)
+    ref = A @ B
+    errors = int(np.sum(np.abs((got - ref) / ref) > 1e-12))
+    assert errors == 0
+    sim.free()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_sim_24_ranks_uneven_exact(hip, ta, tb):
+    """Grid 4x2x3 (k split in 3 layers, q = 2 Cannon rounds, B replicated over
+    2 tasks) on an uneven 1001^3 problem, integer inputs: bit-exact."""
+    rnd = np.random.default_rng(20 + 2 * ta + tb)
+    m = n = k = 1001
+    A = _ints(rnd, (k, m) if ta else (m, k))
+    B = _ints(rnd, (n, k) if tb else (k, n))
+    got, sim, w = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3], keep=True)
+    assert np.array_equal(got, (A.T if ta else A) @ (B.T if tb else B))
+    sim.free()
+
+
+def test_sim_random_fp_within_bound(hip):
+    """Random FP64 inputs on a 2x2x3 grid: within the componentwise bound
+    k*u*(|A||B|) of the exact product."""
+    rnd = np.random.default_rng(7)
+    m, n, k = 240, 300, 900
+    A, B = rnd.uniform(-1, 1, (m, k)), rnd.uniform(-1, 1, (k, n))
+    got = run_sim(12, m, n, k, 0, 0, A, B, proc_grid=[2, 2, 3])
+    bound = k * 2.0 ** -53 * (np.abs(A) @ np.abs(B))
+    assert np.all(np.abs(got - A @ B) <= bound + 2.0 ** -53 * np.abs(A @ B))
+
+
+def test_sim_idle_ranks(hip):
+    """p = 7 with grid 2x1x3: rank 6 idles but still takes part in the
+    redistribution of its input columns."""
+    rnd = np.random.default_rng(8)
+    m, n, k = 130, 77, 210
+    A, B = _ints(rnd, (m, k)), _ints(rnd, (k, n))
+    got, sim, w = run_sim(7, m, n, k, 0, 0, A, B, proc_grid=[2, 1, 3], keep=True)
+    assert sim.engine(6).is_active == 0 and sim.engine(0).rp == 1
+    assert np.array_equal(got, A @ B)
+    sim.free()
+
+
+@pytest.mark.parametrize("p,pg", [(4, None), (8, [2, 2, 2]), (12, [2, 2, 3])])
+def test_sim_btb(hip, p, pg):
+    rnd = np.random.default_rng(p)
+    n, k = 150, 1200
+    B = _ints(rnd, (k, n))
+    got = run_sim(p, n, n, k, 0, 0, None, B, proc_grid=pg, is_btb=True)
+    assert np.array_equal(got, B.T @ B)
