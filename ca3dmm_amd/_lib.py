@@ -58,12 +58,18 @@ SIGNATURES = {
     "hip_dcopy": (I, [I, P, I, P, I]),
     "hip_dgemm": (I, [I, I, I, I, I, D, P, I, P, I, D, P, I]),
     "hip_dgeam": (I, [I, I, I, I, D, P, I, D, P, I, P, I]),
+    "hip_dtrsm": (I, [I, I, I, I, I, I, D, P, I, P, I]),
+    "hip_dpotrf": (I, [C.c_char, I, P, I]),
     # ca3dmm_ext.h
     "hip_copy_matrix_blocks": (I, [SZ, I, C.POINTER(BlkCopy)]),
     "hip_op_sum": (I, [I, I, P, P]),
     "hip_fold_sum": (I, [I, PP, IP, I, I, P, I]),
     "hip_dgemm_fold": (I, [I, I, I, I, I, D, P, I, P, I, D, P, I, I, PP, IP]),
     "hip_transpose_cm": (I, [I, I, P, I, P, I]),
+    "hip_ozaki_dgemm": (I, [I, I, I, I, I, D, P, I, P, I, D, P, I, I]),
+    "hip_ozaki_workspace_bytes": (SZ, [I, I, I, I]),
+    "hip_dsyrk_gram": (I, [I, I, D, P, I, D, P, I, I]),
+    "hip_dtrtri_upper": (I, [I, P, I, P, I]),
     # dev_type.h
     "is_dev_type_valid": (I, [I]),
     "dev_type_malloc": (P, [SZ, I]),

@@ -1,0 +1,151 @@
+// Upper-triangle Gram product C = A^T A on FP64 MFMA (gfx950).
+//
+// The reference computes B^T B in full through the general Cannon DGEMM
+// (src/ca3dmm.c:1172-1265) and the CholeskyQR driver uses it for
+// S = B^T B (examples/example_CholeskyQR.c:109-117, :160-168).  Here only
+// tiles (i <= j) of the upper triangle are computed: n(n+1)k flops instead
+// of 2n^2 k.  Tall-skinny inputs (k >> n, e.g. 2e6 x 512) give few tiles,
+// so k is split across workgroups; each split writes its partial tile and a
+// second kernel adds the partials in split order (deterministic) and writes
+// the upper triangle, optionally mirrored into the lower one.  Mirroring
+// copies values, so the result is exactly symmetric.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "ca3dmm_ext.h"
+#include "dgemm_tile.h"
+#include "hip_common.h"
+
+namespace {
+
+using namespace ca3::gemm;
+
+struct GramArgs {
+    int n, k, ntile, npairs, ksplit, kstep;
+    const double *A;
+    int64_t lda;
+    double *P;      // partials: [ksplit][npairs][128 * 128]
+};
+
+// pair index -> (ti, tj), ti <= tj, row-major over the upper triangle
+__device__ __forceinline__ void pair_of(int p, int nt, int *ti, int *tj)
+{
+    int i = 0, rem = p;
+    while (rem >= nt - i) { rem -= nt - i; ++i; }
+    *ti = i;
+    *tj = i + rem;
+}
+
+__global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
+{
+    __shared__ double As[2][kBK * kBM];
+    __shared__ double Bs[2][kBK * kBN];
+    const int pair = blockIdx.x % g.npairs, ks = blockIdx.x / g.npairs;
+    int ti, tj;
+    pair_of(pair, g.ntile, &ti, &tj);
+    const int kbeg = ks * g.kstep, kend = min(g.k, kbeg + g.kstep);
+    // op(A) = A^T and op(B) = A: both panels are contiguous along k
+    OperandLoader<0> la, lb;
+    la.init(g.A, g.lda, ti * kBM, g.n);
+    lb.init(g.A, g.lda, tj * kBN, g.n);
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    mainloop(la, lb, kbeg, kend, As, Bs, acc);
+    // store the full 128 x 128 partial (rows fastest)
+    double *out = g.P + ((int64_t) ks * g.npairs + pair) * (kBM * kBN);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wm * 64 + i * 16 + (lane & 15);
+                const int col = wn * 64 + j * 16 + 4 * r + (lane >> 4);
+                out[col * kBM + row] = acc[i][j][r];
+            }
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restrict__ P, int n, int ntile, int npairs,
+                                                          int ksplit, double alpha, double beta, double *C,
+                                                          int64_t ldc, int mirror)
+{
+    const int pair = blockIdx.y;
+    int ti, tj;
+    pair_of(pair, ntile, &ti, &tj);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < kBM * kBN; e += gridDim.x * blockDim.x) {
+        const int row = e % kBM, col = e / kBM;
+        const int gi = ti * kBM + row, gj = tj * kBN + col;
+        if (gi >= n || gj >= n || gi > gj) continue;
+        double s = P[(int64_t) pair * (kBM * kBN) + e];
+        for (int ks = 1; ks < ksplit; ++ks) s = __dadd_rn(s, P[((int64_t) ks * npairs + pair) * (kBM * kBN) + e]);
+        double *c = C + (int64_t) gj * ldc + gi;
+        double v = __dmul_rn(alpha, s);
+        if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *c));
+        *c = v;
+        if (mirror && gi != gj) C[(int64_t) gi * ldc + gj] = v;
+    }
+}
+
+struct Ws {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+Ws g_gram_ws[16];
+std::mutex g_gram_mu;
+
+}  // namespace
+
+extern "C" {
+
+int hip_dsyrk_gram(const int n, const int k, const double alpha, const double *A, const int ldA,
+                   const double beta, double *C, const int ldC, int mirror)
+{
+    if (n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: negative size");
+    if (n == 0) return HIP_PROXY_OK;
+    if (!C || ldC < n) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: bad C / ldC");
+    if (k > 0 && (!A || ldA < std::max(1, k))) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: bad A / ldA");
+    const int ntile = (n + kBM - 1) / kBM, npairs = ntile * (ntile + 1) / 2;
+    // enough workgroups to fill the chip twice over, stages of >= 1024 k
+    const int target = 2 * 2 * ca3::cu_count();
+    int ksplit = std::max(1, std::min(target / npairs, (k + 1023) / 1024));
+    if (k == 0) ksplit = 1;
+    // k-range per split: ceil(k / ksplit) rounded up to whole stages
+    const int kstep = std::max(kBK, ((k + ksplit - 1) / ksplit + kBK - 1) / kBK * kBK);
+    ksplit = std::max(1, (k + kstep - 1) / kstep);
+    const size_t bytes = sizeof(double) * (size_t) ksplit * npairs * kBM * kBN;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    Ws &w = g_gram_ws[dev & 15];
+    {
+        std::lock_guard<std::mutex> lk(g_gram_mu);
+        if (w.bytes < bytes) {
+            if (w.p) {
+                (void) hipStreamSynchronize(ca3::stream());
+                (void) hipFree(w.p);
+                w.p = nullptr;
+                w.bytes = 0;
+            }
+            CA3_CHECK(hipMalloc(&w.p, bytes));
+            w.bytes = bytes;
+        }
+    }
+    GramArgs g;
+    memset(&g, 0, sizeof(g));
+    g.n = n; g.k = k; g.ntile = ntile; g.npairs = npairs; g.ksplit = ksplit; g.kstep = kstep;
+    g.A = A; g.lda = ldA; g.P = (double *) w.p;
+    hipStream_t st = ca3::stream();
+    gram_partial_kernel<<<npairs * ksplit, kThreads, 0, st>>>(g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ca3::fail(e, "gram_partial_kernel");
+    gram_reduce_kernel<<<dim3(16, npairs), 256, 0, st>>>(g.P, n, ntile, npairs, ksplit, alpha, beta, C, ldC,
+                                                        mirror ? 1 : 0);
+    return ca3::finish();
+}
+
+}  // extern "C"

@@ -1,0 +1,436 @@
+// Ozaki-scheme split-precision DGEMM on INT8 MFMA (gfx950).
+//
+// No reference counterpart: an alternative to the FP64 MFMA DGEMM that
+// keeps FP64 accuracy.  Specification (and the bit-exact oracle):
+// oracle/ozaki.py, functions split8 / ozaki_dgemm8.
+//
+//   1. ozaki_exp_kernel   per line (row of op(A), column of op(B)) the
+//                          frexp exponent of the largest |x| (atomicMax);
+//   2. ozaki_slice_kernel  x' = x * 2^-(e+2) (|x'| < 1/4) and the s balanced
+//                          base-256 digits of rint(x' * 2^(8s)) as int8
+//                          planes, line-major with k contiguous, zero padded
+//                          to whole tiles; LDS-tiled so both source layouts
+//                          read coalesced;
+//   3. ozaki_diag_kernel   diagonal d = s-1 .. 0: sum_{t+u=d} QA_t QB_u on
+//                          v_mfma_i32_16x16x64_i8 with exact INT32
+//                          accumulation, added in FP64 (x 2^-8(d+2)),
+//                          smallest diagonal first; the last launch rescales
+//                          by 2^eA_i 2^eB_j and applies alpha/beta.
+// Each diagonal (and INT32-exact k-chunk) is one launch of an INT8 GEMM
+// over the concatenated slice pairs: 256 x 128 tiles, 128 x 64 per wave,
+// LDS double buffer fed from two register sets (prefetch distance two
+// stages).  The FP64 accumulation of the diagonals runs through a workspace
+// W that stays L2/HBM resident, which frees the registers the FP64
+// accumulators would need.  INT32 exactness needs
+// (d+1) * Kchunk * 128 * 128 < 2^31; for each diagonal the chunks are added
+// in k order (the oracle follows the same order).
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <mutex>
+
+#include "ca3dmm_ext.h"
+#include "hip_common.h"
+
+namespace {
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 256;     // padding of lines (a multiple of both tile sides)
+constexpr int kKb = 64;        // k bytes per pipeline stage
+constexpr int kMaxSlices = 16;
+constexpr int kExpEmpty = INT_MIN;
+
+// ---------------------------------------------------------------------
+// 1. line exponents
+// ---------------------------------------------------------------------
+// Line l of the operand: element (l, kk) at X[l*sl + kk*sk].
+__global__ __launch_bounds__(256) void ozaki_exp_kernel(const double *__restrict__ X, int lines, int K,
+                                                         int64_t sl, int64_t sk, int *__restrict__ ex)
+{
+    // block: 64 lines x (K / gridDim.y chunk); thread layout follows the
+    // contiguous direction so loads coalesce
+    const bool line_contig = (sl == 1);
+    const int t = threadIdx.x;
+    const int kchunk = (K + gridDim.y - 1) / gridDim.y;
+    const int k0 = blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+    const int l0 = blockIdx.x * 64;
+    __shared__ double red[256];
+    double mx = 0.0;
+    int myline;
+    if (line_contig) {
+        myline = l0 + (t & 63);
+        for (int kk = k0 + (t >> 6); kk < k1; kk += 4)
+            if (myline < lines) mx = fmax(mx, fabs(X[myline + (int64_t) kk * sk]));
+    } else {
+        myline = l0 + (t >> 2);
+        for (int kk = k0 + (t & 3); kk < k1; kk += 4)
+            if (myline < lines) mx = fmax(mx, fabs(X[(int64_t) myline * sl + kk]));
+    }
+    red[t] = mx;
+    __syncthreads();
+    if (t < 64) {
+        const int l = l0 + t;
+        double m = 0.0;
+        if (line_contig) {
+            for (int j = 0; j < 4; ++j) m = fmax(m, red[t + 64 * j]);
+        } else {
+            for (int j = 0; j < 4; ++j) m = fmax(m, red[4 * t + j]);
+        }
+        if (l < lines && m > 0.0) {
+            int e;
+            frexp(m, &e);
+            atomicMax(&ex[l], e);
+        }
+    }
+}
+
+__global__ void fill_int_kernel(int *p, int n, int v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------
+// 2. slices
+// ---------------------------------------------------------------------
+// Balanced base-256 digits of x * 2^-E, |.| < 1/4, as digit t (0 = most
+// significant) of rint(x * 2^(8s - E)).
+__device__ __forceinline__ void digits_of(double x, int E, int s, int (&dig)[kMaxSlices])
+{
+#pragma unroll
+    for (int t = 0; t < kMaxSlices; ++t) dig[t] = 0;
+    const double a = ldexp(x, -E);
+    if (a == 0.0) return;
+    int ex;
+    const double f = frexp(a, &ex);
+    const int64_t M = (int64_t) ldexp(f, 53);        // exact signed 53-bit significand
+    const int z = ex - 53 + 8 * s;
+    int64_t X;
+    int off;
+    if (z < 0) {
+        X = (int64_t) rint(ldexp((double) M, z));     // < 2^53 in magnitude: exact
+        off = 0;
+    } else {
+        X = M << (z & 7);
+        off = z >> 3;
+    }
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const int64_t c = ((X + 128) & 255) - 128;
+        X = (X - c) >> 8;
+        const int tpos = s - 1 - (off + i);
+#pragma unroll
+        for (int t = 0; t < kMaxSlices; ++t)
+            if (t == tpos) dig[t] = (int) c;
+    }
+}
+
+// Tile of 64 lines x 64 k.  Output plane t: P[t * plane + line * Kp + kk].
+__global__ __launch_bounds__(256) void ozaki_slice_kernel(const double *__restrict__ X, int lines, int K,
+                                                           int64_t sl, int64_t sk, const int *__restrict__ ex,
+                                                           int s, int8_t *__restrict__ P, int64_t plane, int Kp)
+{
+    __shared__ double tile[64][65];
+    const int t = threadIdx.x;
+    const int l0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    if (sl == 1) {            // lines contiguous
+        for (int p = 0; p < 16; ++p) {
+            const int li = t & 63, ki = (t >> 6) + 4 * p;
+            const int l = l0 + li, kk = k0 + ki;
+            tile[li][ki] = (l < lines && kk < K) ? X[l + (int64_t) kk * sk] : 0.0;
+        }
+    } else {                  // k contiguous
+        for (int p = 0; p < 16; ++p) {
+            const int ki = t & 63, li = (t >> 6) + 4 * p;
+            const int l = l0 + li, kk = k0 + ki;
+            tile[li][ki] = (l < lines && kk < K) ? X[(int64_t) l * sl + kk] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int li = t >> 2, kq = (t & 3) * 16;
+    const int l = l0 + li;
+    const int E = (l < lines && ex[l] != kExpEmpty) ? ex[l] + 2 : 0;
+    const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
+    uint32_t packed[kMaxSlices][4];
+#pragma unroll
+    for (int t2 = 0; t2 < kMaxSlices; ++t2)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) packed[t2][w] = 0;
+    for (int j = 0; j < 16; ++j) {
+        int dig[kMaxSlices];
+        digits_of(zero_line ? 0.0 : tile[li][kq + j], E, s, dig);
+#pragma unroll
+        for (int t2 = 0; t2 < kMaxSlices; ++t2)
+            packed[t2][j >> 2] |= ((uint32_t) (dig[t2] & 0xff)) << (8 * (j & 3));
+    }
+    int8_t *base = P + (int64_t) l * Kp + k0 + kq;
+#pragma unroll
+    for (int t2 = 0; t2 < kMaxSlices; ++t2) {
+        if (t2 < s) {
+            uint4 v = make_uint4(packed[t2][0], packed[t2][1], packed[t2][2], packed[t2][3]);
+            *reinterpret_cast<uint4 *>(base + (int64_t) t2 * plane) = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// 3. one diagonal of slice products, FP64 recombination in the epilogue
+// ---------------------------------------------------------------------
+// A launch computes, for one diagonal d and one INT32-exact k-chunk, the
+// integer matrix D = sum_{t=0..d} QA_t QB_{d-t} over that chunk (a plain
+// INT8 GEMM over the concatenated slice pairs) and folds it into the FP64
+// accumulator W (m x n, ld Mp): W = D * 2^-8(d+2) on the first launch,
+// W + D * 2^-8(d+2) afterwards; the last launch writes
+// C = alpha * 2^eA 2^eB (W + ...) + beta * C instead.
+constexpr int kBMo = 256, kBNo = 128;   // workgroup tile (4 waves of 128 x 64)
+
+struct OzArgs {
+    int M, N, Mp, Np, Kp;
+    int d, kb0, nkb;        // diagonal, first k-block of the chunk, k-blocks in the chunk
+    int first, last;
+    double alpha, beta;
+    const int8_t *PA;       // planes of Mp x Kp
+    const int8_t *PB;       // planes of Np x Kp
+    int64_t planeA, planeB;
+    const int *eA, *eB;     // line exponents (kExpEmpty for zero lines)
+    double *W;              // FP64 accumulator, ld Mp
+    double *C;
+    int64_t ldc;
+};
+
+// 16-byte chunk swizzle within a 64-byte LDS row: rows r..r+15 read by one
+// MFMA operand land on 16 distinct 16-byte bank groups.
+__device__ __forceinline__ int lds_off(int row, int chunk) { return row * 64 + ((chunk ^ ((row >> 2) & 3)) << 4); }
+
+__device__ __forceinline__ void oz_gload(const OzArgs &g, const int8_t *gA, const int8_t *gB, int64_t row64,
+                                         int z, u32x4 (&ra)[4], u32x4 (&rb)[2])
+{
+    const int t = z / g.nkb, b = z - t * g.nkb;
+    const int64_t koff = (int64_t) (g.kb0 + b) * kKb;
+    const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
+    const int8_t *pb = gB + (int64_t) (g.d - t) * g.planeB + koff;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) ra[p] = *reinterpret_cast<const u32x4 *>(pa + p * row64);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) rb[p] = *reinterpret_cast<const u32x4 *>(pb + p * row64);
+}
+
+__device__ __forceinline__ void oz_sstore(int8_t *As, int8_t *Bs, int lrow, int lchunk, const u32x4 (&ra)[4],
+                                          const u32x4 (&rb)[2])
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) *reinterpret_cast<u32x4 *>(&As[lds_off(lrow + 64 * p, lchunk)]) = ra[p];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) *reinterpret_cast<u32x4 *>(&Bs[lds_off(lrow + 64 * p, lchunk)]) = rb[p];
+}
+
+__device__ __forceinline__ void oz_compute(const int8_t *As, const int8_t *Bs, int wm, int wn, int li, int kq,
+                                           i32x4 (&acc)[8][4])
+{
+    i32x4 bf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const i32x4 *>(&Bs[lds_off(wn * 64 + j * 16 + li, kq)]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const i32x4 af = *reinterpret_cast<const i32x4 *>(&As[lds_off(wm * 128 + i * 16 + li, kq)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af, acc[i][j], 0, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
+{
+    __shared__ __attribute__((aligned(16))) int8_t As[2][kBMo * kKb];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][kBNo * kKb];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int tiles_m = g.Mp / kBMo, tiles_n = g.Np / kBNo;
+    int bid = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int G = 4, per = G * tiles_n, grp = bid / per, first = grp * G;
+    const int gsz = min(tiles_m - first, G), rr = bid - grp * per;
+    const int tm = first + rr % gsz, tn = rr / gsz;
+    const int m0 = tm * kBMo, n0 = tn * kBNo;
+
+    const int lrow = tid >> 2, lchunk = tid & 3;
+    const int8_t *gA = g.PA + (int64_t) (m0 + lrow) * g.Kp + lchunk * 16;
+    const int8_t *gB = g.PB + (int64_t) (n0 + lrow) * g.Kp + lchunk * 16;
+    const int64_t row64 = (int64_t) 64 * g.Kp;
+    const int d = g.d;
+    const int total = (d + 1) * g.nkb;        // stages: slice pair t, k-block b
+
+    i32x4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+    const int li = lane & 15, kq = lane >> 4;
+
+    // two register sets: the load of stage z+2 is in flight while stages z
+    // and z+1 compute
+    u32x4 ra0[4], rb0[2], ra1[4], rb1[2];
+    // Loads past the last stage re-read the last stage (in bounds) and the
+    // stores they feed are never consumed; this keeps the register sets
+    // free of control flow so they stay in VGPRs.
+    oz_gload(g, gA, gB, row64, 0, ra0, rb0);
+    oz_gload(g, gA, gB, row64, min(1, total - 1), ra1, rb1);
+    oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
+    __syncthreads();
+    for (int z = 0; z < total; z += 2) {
+        oz_gload(g, gA, gB, row64, min(z + 2, total - 1), ra0, rb0);
+        oz_compute(As[0], Bs[0], wm, wn, li, kq, acc);
+        oz_sstore(As[1], Bs[1], lrow, lchunk, ra1, rb1);
+        __syncthreads();
+        if (z + 1 < total) {
+            oz_gload(g, gA, gB, row64, min(z + 3, total - 1), ra1, rb1);
+            oz_compute(As[1], Bs[1], wm, wn, li, kq, acc);
+            oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
+            __syncthreads();
+        }
+    }
+
+    // epilogue: lane l, register r of block (i, j) holds D[m][n] with
+    // m = l % 16, n = 4 * (l / 16) + r (operands fed as (B, A))
+    const double w = ldexp(1.0, -8 * (d + 2));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int row = m0 + wm * 128 + i * 16 + li;
+        if (row >= g.M) continue;
+        const int ea = g.eA[row];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = n0 + wn * 64 + j * 16 + 4 * kq + r;
+                if (col >= g.N) continue;
+                double *wp = g.W + (int64_t) col * g.Mp + row;
+                double v = (double) acc[i][j][r] * w;
+                if (!g.first) v = *wp + v;
+                if (!g.last) {
+                    *wp = v;
+                    continue;
+                }
+                const int eb = g.eB[col];
+                v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea + 2), eb + 2) : 0.0;
+                double *cp = g.C + (int64_t) col * g.ldc + row;
+                v = __dmul_rn(g.alpha, v);
+                if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *cp));
+                *cp = v;
+            }
+        }
+    }
+}
+
+// Library-owned, grow-only workspace for the planes and exponents.
+struct Workspace {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int dev = -1;
+};
+Workspace g_ws[16];
+std::mutex g_ws_mu;
+
+void *workspace(size_t bytes, int *rc)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    Workspace &w = g_ws[dev & 15];
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (w.bytes < bytes) {
+        if (w.p) {
+            (void) hipStreamSynchronize(ca3::stream());
+            (void) hipFree(w.p);
+            w.p = nullptr;
+            w.bytes = 0;
+        }
+        hipError_t e = hipMalloc(&w.p, bytes);
+        if (e != hipSuccess) { *rc = ca3::fail(e, "ozaki workspace"); return nullptr; }
+        w.bytes = bytes;
+    }
+    *rc = 0;
+    return w.p;
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+extern "C" {
+
+size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || nslices < 1 || nslices > kMaxSlices) return 0;
+    const size_t Mp = round_up(m, kTile), Np = round_up(n, kTile), Kp = round_up(k, kKb);
+    return (size_t) nslices * (Mp + Np) * Kp + sizeof(int) * (Mp + Np) + sizeof(double) * Mp * Np + 512;
+}
+
+int hip_ozaki_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n,
+                    const int k, const double alpha, const double *A, const int ldA, const double *B,
+                    const int ldB, const double beta, double *C, const int ldC, int nslices)
+{
+    const int ta = (transA != HipNoTrans), tb = (transB != HipNoTrans);
+    if (m < 0 || n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: negative size");
+    if (m == 0 || n == 0) return HIP_PROXY_OK;
+    if (nslices < 1 || nslices > kMaxSlices)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: nslices %d outside 1..%d", nslices, kMaxSlices);
+    if (!C || ldC < std::max(1, m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: bad C / ldC");
+    if (k > 0) {
+        if (!A || ldA < std::max(1, ta ? k : m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: bad A / ldA");
+        if (!B || ldB < std::max(1, tb ? n : k)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: bad B / ldB");
+    }
+    const int s = nslices;
+    const int Mp = round_up(m, kTile), Np = round_up(n, kTile), Kp = round_up(std::max(k, 1), kKb);
+    int rc = 0;
+    const size_t bytes = hip_ozaki_workspace_bytes(m, n, std::max(k, 1), s);
+    char *ws = (char *) workspace(bytes, &rc);
+    if (rc) return rc;
+    int8_t *PA = (int8_t *) ws;
+    int8_t *PB = PA + (size_t) s * Mp * Kp;
+    int *eA = (int *) (PB + (size_t) s * Np * Kp);
+    int *eB = eA + Mp;
+    hipStream_t st = ca3::stream();
+    fill_int_kernel<<<(Mp + Np + 255) / 256, 256, 0, st>>>(eA, Mp + Np, kExpEmpty);
+    // op(A): line i, element (i, l): A[i + l*ldA] (no trans) or A[l + i*ldA]
+    const int64_t a_sl = ta ? ldA : 1, a_sk = ta ? 1 : ldA;
+    // op(B): line j, element (l, j): B[l + j*ldB] (no trans) or B[j + l*ldB]
+    const int64_t b_sl = tb ? 1 : ldB, b_sk = tb ? ldB : 1;
+    if (k > 0) {
+        const int ky = std::min(64, std::max(1, k / 512));
+        ozaki_exp_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA);
+        ozaki_exp_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB);
+    }
+    ozaki_slice_kernel<<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA, s, PA,
+                                                               (int64_t) Mp * Kp, Kp);
+    ozaki_slice_kernel<<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB, s, PB,
+                                                               (int64_t) Np * Kp, Kp);
+    double *W = (double *) (((uintptr_t) (eB + Np) + 255) & ~(uintptr_t) 255);
+    OzArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = m; g.N = n; g.Mp = Mp; g.Np = Np; g.Kp = Kp;
+    g.alpha = alpha; g.beta = beta;
+    g.PA = PA; g.PB = PB;
+    g.planeA = (int64_t) Mp * Kp; g.planeB = (int64_t) Np * Kp;
+    g.eA = eA; g.eB = eB;
+    g.W = W; g.C = C; g.ldc = ldC;
+    // INT32 exactness of one diagonal: (d+1) * kchunk * 128 * 128 < 2^31
+    const int kchunk = std::min(Kp, (int) (((1u << 31) - 1) / (16384u * (unsigned) s)) / kKb * kKb);
+    const int nchunks = (Kp + kchunk - 1) / kchunk;
+    const int grid = (Mp / kBMo) * (Np / kBNo);
+    for (int d = s - 1; d >= 0; --d) {
+        for (int c = 0; c < nchunks; ++c) {
+            g.d = d;
+            g.kb0 = c * (kchunk / kKb);
+            g.nkb = std::min(kchunk, Kp - c * kchunk) / kKb;
+            g.first = (d == s - 1 && c == 0);
+            g.last = (d == 0 && c == nchunks - 1);
+            ozaki_diag_kernel<<<grid, 256, 0, st>>>(g);
+        }
+    }
+    return ca3::finish();
+}
+
+}  // extern "C"

@@ -104,6 +104,12 @@ def dgemm(transA, transB, m, n, k, alpha, A, B, beta, Cm, lda=None, ldb=None, ld
                                ldb or _ld(B), beta, Cm.ptr, ldc or _ld(Cm)), "hip_dgemm")
 
 
+def ozaki_dgemm(transA, transB, m, n, k, alpha, A, B, beta, Cm, nslices, lda=None, ldb=None, ldc=None):
+    return check(lib.hip_ozaki_dgemm(transA, transB, m, n, k, alpha, A.ptr, lda or _ld(A), B.ptr,
+                                     ldb or _ld(B), beta, Cm.ptr, ldc or _ld(Cm), nslices),
+                 "hip_ozaki_dgemm")
+
+
 def dgemm_fold(transA, transB, m, n, k, alpha, A, B, beta, Cm, parts):
     arr = (C.c_void_p * len(parts))(*[p.ptr for p in parts])
     lds = (C.c_int * len(parts))(*[_ld(p) for p in parts])
@@ -139,3 +145,26 @@ def op_sum(invec, inoutvec):
 
 def transpose_cm(nrow, ncol, A, AT):
     return check(lib.hip_transpose_cm(nrow, ncol, A.ptr, _ld(A), AT.ptr, _ld(AT)), "hip_transpose_cm")
+
+
+def gram(n, k, A, Cm, alpha=1.0, beta=0.0, mirror=True, lda=None, ldc=None):
+    return check(lib.hip_dsyrk_gram(n, k, alpha, A.ptr, lda or _ld(A), beta, Cm.ptr, ldc or _ld(Cm),
+                                    1 if mirror else 0), "hip_dsyrk_gram")
+
+
+def potrf_upper(n, A, lda=None):
+    """Returns LAPACK info (0 = success)."""
+    return check(lib.hip_dpotrf(b"U", n, A.ptr, lda or _ld(A)), "hip_dpotrf")
+
+
+def trtri_upper(n, R, X):
+    return check(lib.hip_dtrtri_upper(n, R.ptr, _ld(R), X.ptr, _ld(X)), "hip_dtrtri_upper")
+
+
+HIP_RIGHT, HIP_UPPER, HIP_NONUNIT = 4, 5, 8
+
+
+def trsm_right_upper(m, n, alpha, R, B):
+    """B := alpha * B * R^{-1}."""
+    return check(lib.hip_dtrsm(HIP_RIGHT, HIP_UPPER, NOTRANS, HIP_NONUNIT, m, n, alpha, R.ptr, _ld(R),
+                               B.ptr, _ld(B)), "hip_dtrsm")

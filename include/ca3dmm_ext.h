@@ -54,6 +54,33 @@ int hip_dgemm_fold(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
  * col-major, ldAT) := transpose of A (ncol x nrow, col-major, ldA). */
 int hip_transpose_cm(int nrow, int ncol, const double *A, int ldA, double *AT, int ldAT);
 
+/* ---- Ozaki-scheme split-precision DGEMM (no reference counterpart) ----
+ * C := alpha*op(A)*op(B) + beta*C computed from nslices INT8 slices of each
+ * operand (per-row / per-column power-of-two scaling, balanced base-256
+ * digits) on the INT8 matrix cores with exact INT32 accumulation and FP64
+ * recombination.  Accuracy grows by ~8 bits per slice; nslices in 1..16.
+ * The result is reproducible bit for bit (fixed recombination order). */
+int hip_ozaki_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
+                    const int m, const int n, const int k, const double alpha,
+                    const double *A, const int ldA, const double *B, const int ldB,
+                    const double beta, double *C, const int ldC, int nslices);
+
+/* Device workspace the library keeps for hip_ozaki_dgemm at this size. */
+size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices);
+
+/* C := alpha * A^T A + beta * C for A k x n (column-major): the B^T B of
+ * ca3dmm_engine_init_BTB (src/ca3dmm.c:1172-1265) and of CholeskyQR
+ * (examples/example_CholeskyQR.c:109-117).  Only the upper triangle is
+ * computed (n(n+1)k flops); mirror != 0 also writes the lower triangle as an
+ * exact copy.  k is split across workgroups and the partials are added in a
+ * fixed order, so results are reproducible. */
+int hip_dsyrk_gram(const int n, const int k, const double alpha, const double *A, const int ldA,
+                   const double beta, double *C, const int ldC, int mirror);
+
+/* X := R^{-1} for upper-triangular R (n x n, n <= 8192); the strictly
+ * lower part of X is set to zero (examples/example_CholeskyQR.c:137). */
+int hip_dtrtri_upper(const int n, const double *R, const int ldR, double *X, const int ldX);
+
 #ifdef __cplusplus
 }
 #endif

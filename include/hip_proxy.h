@@ -120,6 +120,21 @@ int hip_dgeam(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
               const double beta, const double *B, const int ldB,
               double *C, const int ldC);
 
+/* cuda_proxy.h:109-114 (cuda_cublas_dtrsm).  Implemented for
+ * side=HipRight, uplo=HipUpper, trans=HipNoTrans, diag=HipNonUnit
+ * (B := alpha * B * inv(A), the Q = A R^-1 step of CholeskyQR); other
+ * combinations return HIP_PROXY_ERR_UNSUPP.  n <= 8192. */
+int hip_dtrsm(hip_proxy_enum_t side, hip_proxy_enum_t uplo,
+              hip_proxy_enum_t trans, hip_proxy_enum_t diag,
+              const int m, const int n, const double alpha,
+              const double *A, const int ldA, double *B, const int ldB);
+
+/* cuda_proxy.h:117 (cuda_cusolver_dpotrf): Cholesky A = R^T R in place,
+ * uplo 'U' (the strictly lower part is not referenced).  Returns LAPACK's
+ * info: 0 on success, i > 0 when the leading minor of order i is not
+ * positive definite, < 0 on an argument or runtime error.  n <= 8192. */
+int hip_dpotrf(const char uplo, const int n, double *A, const int ldA);
+
 #ifdef __cplusplus
 }
 #endif

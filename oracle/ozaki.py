@@ -1,0 +1,194 @@
+"""Ozaki-scheme split-precision DGEMM, restated in numpy (TEST INFRASTRUCTURE ONLY).
+
+No reference counterpart: CA3DMM has no split-precision path.  This module
+is the specification the HIP kernels (ca3dmm_amd/csrc/ozaki.hip) are checked
+against bit for bit, and the source of the accuracy study.
+
+Scheme (error-free splitting, Ozaki et al. 2012, with INT8 slices):
+  * row i of op(A) is scaled by sigma_i = 2^(e_i + 1), e_i = frexp exponent
+    of max_l |a_il|, so |a'_il| = |a_il / sigma_i| <= 1/2; columns of op(B)
+    likewise by tau_j.  Zero rows/columns get sigma = 0 (no contribution).
+  * slice t = 0..s-1 of a' is q_t = rint(r_t * 2^(7(t+1))) with r_0 = a' and
+    r_{t+1} = r_t - q_t * 2^(-7(t+1)); every step is exact in FP64 and
+    |q_t| <= 64, so slices are int8 and a' = sum_t q_t 2^(-7(t+1)) + r_s with
+    |r_s| <= 2^(-7s-1).
+  * diagonal d = 0..s-1: D_d = sum_{t+u=d} Q^A_t Q^B_u, an exact integer
+    matrix (INT32 while k*(d+1)*64*64 < 2^31).
+  * C_ij = sigma_i tau_j * sum_{d=s-1..0} 2^(-7(d+2)) D_d,ij, the FP64 sum
+    taken from the smallest diagonal to the largest (fixed order, so the
+    result is reproducible bit for bit); then alpha*C + beta*C_in.
+"""
+import numpy as np
+
+SLICE_BITS = 7
+QMAX = 64
+
+
+def scale_exponents(X, axis):
+    """e such that max |X| along `axis` < 2^e (frexp exponent); -inf marker
+    (None -> 0 scale) for all-zero lines.  Returns (exponent int array,
+    zero mask)."""
+    mx = np.max(np.abs(X), axis=axis)
+    _, e = np.frexp(mx)
+    return e.astype(np.int64), mx == 0
+
+
+def split(X, s, axis):
+    """Slices of X scaled along `axis` (axis=1: rows of op(A); axis=0:
+    columns of op(B)).  Returns (list of s int8 arrays shaped like X,
+    scale exponent E = e + 1 per line, zero mask, remainder r_s)."""
+    e, zero = scale_exponents(X, axis)
+    E = e + 1
+    shp = (-1, 1) if axis == 1 else (1, -1)
+    r = np.ldexp(X, -E.reshape(shp))
+    r = np.where(zero.reshape(shp), 0.0, r)
+    slices = []
+    for t in range(s):
+        w = SLICE_BITS * (t + 1)
+        q = np.rint(np.ldexp(r, w))
+        assert np.all(np.abs(q) <= QMAX)
+        slices.append(q.astype(np.int8))
+        r = r - np.ldexp(q, -w)
+    return slices, E, zero, r
+
+
+def diagonals(SA, SB, s):
+    """D_d = sum_{t+u=d} SA[t] @ SB[u] as exact integers (computed in FP64,
+    exact while k * s * 64 * 64 < 2^53)."""
+    out = []
+    k = SA[0].shape[1]
+    assert k * s * QMAX * QMAX < 2 ** 53
+    for d in range(s):
+        acc = None
+        for t in range(d + 1):
+            P = SA[t].astype(np.float64) @ SB[d - t].astype(np.float64)
+            acc = P if acc is None else acc + P
+        out.append(acc)
+    return out
+
+
+def ozaki_dgemm(opA, opB, s, alpha=1.0, beta=0.0, Cin=None):
+    """Reference result of the scheme for op(A) (m x k) and op(B) (k x n)."""
+    SA, EA, zA, _ = split(opA, s, axis=1)
+    SB, EB, zB, _ = split(opB, s, axis=0)
+    D = diagonals(SA, SB, s)
+    acc = np.zeros(D[0].shape)
+    for d in range(s - 1, -1, -1):
+        acc = acc + np.ldexp(D[d], -SLICE_BITS * (d + 2))
+    C = np.ldexp(np.ldexp(acc, EA.reshape(-1, 1)), EB.reshape(1, -1))
+    C = np.where(zA.reshape(-1, 1) | zB.reshape(1, -1), 0.0, C)
+    C = alpha * C
+    if beta != 0.0:
+        C = C + beta * Cin
+    return C
+
+
+def phi_matrix(rnd, shape, phi):
+    """Test inputs of the precision study: uniform[-0.5, 0.5] * exp(phi * N(0,1))."""
+    return rnd.uniform(-0.5, 0.5, shape) * np.exp(phi * rnd.standard_normal(shape))
+
+
+def products(s):
+    """Number of INT8 slice products of the triangular scheme."""
+    return s * (s + 1) // 2
+
+
+# ---------------------------------------------------------------------------
+# 8-bit balanced digits.  Each slice carries a full int8 digit in [-128, 127]
+# of base 256.  Top-down rounding cannot produce them (a remainder of exactly
+# half a unit would need +128), so the digits are taken bottom-up from the
+# exact integer N = rint(a' * 2^(8s)), |a'| < 1/4: N = M * 2^z with M the
+# 53-bit significand, so only a window of at most 9 digits is non-zero; the
+# window X = M << (z mod 8) (<= 61 bits) is converted to balanced digits with
+# carries, c = ((X + 128) mod 256) - 128, X = (X - c) / 256.
+# ---------------------------------------------------------------------------
+DIGIT_BITS = 8
+
+
+def split8(X, s, axis):
+    """Balanced base-256 slices of X scaled per line along `axis`.
+    Scale exponent E = e + 2 (|a'| < 1/4 so the top digit never carries out).
+    Returns (list of s int8 arrays, E, zero mask)."""
+    e, zero = scale_exponents(X, axis)
+    E = e + 2
+    shp = (-1, 1) if axis == 1 else (1, -1)
+    a = np.ldexp(X, -E.reshape(shp))
+    a = np.where(zero.reshape(shp), 0.0, a)
+    f, ex = np.frexp(a)                         # a = f * 2^ex, f in [0.5, 1)
+    M = np.ldexp(f, 53).astype(np.int64)        # exact 53-bit signed significand
+    z = ex.astype(np.int64) - 53 + 8 * s        # N = M * 2^z
+    neg = z < 0
+    # z < 0: N = rint(M * 2^z) (< 2^53, exact in FP64); window offset 0
+    Nsmall = np.rint(np.ldexp(M.astype(np.float64), np.minimum(z, 0))).astype(np.int64)
+    off = np.where(neg, 0, z // 8)
+    Xw = np.where(neg, Nsmall, M << np.where(neg, 0, z % 8))
+    Xw = np.where(a == 0, 0, Xw)
+    digits = [np.zeros(X.shape, np.int64) for _ in range(s)]
+    for i in range(10):                         # at most 9 non-zero digits + slack
+        c = ((Xw + 128) & 255) - 128
+        Xw = (Xw - c) >> 8
+        j = off + i                             # position from the least significant digit
+        t = s - 1 - j                           # slice index (0 = most significant)
+        ok = (t >= 0) & (t < s)
+        for tt in range(s):
+            sel = ok & (t == tt)
+            digits[tt] = np.where(sel, c, digits[tt])
+    assert np.all(Xw == 0), "digit window overflow"
+    out = []
+    for d in digits:
+        assert np.all((d >= -128) & (d <= 127))
+        out.append(d.astype(np.int8))
+    return out, E, zero
+
+
+def reconstruct8(slices, E, axis):
+    """sum_t q_t 2^(-8(t+1)) scaled back by 2^E (exact check helper)."""
+    s = len(slices)
+    acc = np.zeros(slices[0].shape)
+    for t in range(s - 1, -1, -1):
+        acc = acc + np.ldexp(slices[t].astype(np.float64), -DIGIT_BITS * (t + 1))
+    shp = (-1, 1) if axis == 1 else (1, -1)
+    return np.ldexp(acc, E.reshape(shp))
+
+
+def kchunk8(k, s):
+    """k-block length over which one diagonal stays exact in INT32:
+    s * kchunk * 128 * 128 < 2^31, a multiple of 64, at most k rounded up
+    to 64 (the rule of the device kernel)."""
+    kp = -(-max(k, 1) // 64) * 64
+    return min(kp, ((2 ** 31 - 1) // (16384 * s)) // 64 * 64)
+
+
+def ozaki_dgemm8(opA, opB, s, alpha=1.0, beta=0.0, Cin=None):
+    """The scheme with 8-bit balanced digits: the reference result the device
+    kernels reproduce bit for bit.  The diagonals are added from d = s-1 down
+    to 0; within a diagonal, k is processed in INT32-exact chunks in k order."""
+    SA, EA, zA = split8(opA, s, axis=1)
+    SB, EB, zB = split8(opB, s, axis=0)
+    k = opA.shape[1]
+    kc = kchunk8(k, s)
+    chunks = [slice(c0, min(k, c0 + kc)) for c0 in range(0, max(k, 1), kc)]
+    Dc = [diagonals8([x[:, sl] for x in SA], [x[sl, :] for x in SB], s) for sl in chunks]
+    acc = np.zeros((opA.shape[0], opB.shape[1]))
+    for d in range(s - 1, -1, -1):
+        for D in Dc:
+            acc = acc + np.ldexp(D[d], -DIGIT_BITS * (d + 2))
+    C = np.ldexp(np.ldexp(acc, EA.reshape(-1, 1)), EB.reshape(1, -1))
+    C = np.where(zA.reshape(-1, 1) | zB.reshape(1, -1), 0.0, C)
+    C = alpha * C
+    if beta != 0.0:
+        C = C + beta * Cin
+    return C
+
+
+def diagonals8(SA, SB, s):
+    out = []
+    k = SA[0].shape[1]
+    assert k * s * 128 * 128 < 2 ** 53
+    for d in range(s):
+        acc = None
+        for t in range(d + 1):
+            P = SA[t].astype(np.float64) @ SB[d - t].astype(np.float64)
+            acc = P if acc is None else acc + P
+        out.append(acc)
+    return out

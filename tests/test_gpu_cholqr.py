@@ -1,0 +1,97 @@
+"""GPU: the B^T B Gram kernel (upper triangle, split-K) and the CholeskyQR
+pieces (potrf, trtri, trsm) against numpy/scipy and exact arithmetic, and
+CholeskyQR2 end to end (examples/example_CholeskyQR.c, two passes)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from ca3dmm_amd import cholqr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (5, 3), (128, 128), (130, 1000), (300, 20000), (512, 70000)])
+def test_gram_integer_exact_and_symmetric(hip, n, k):
+    rnd = np.random.default_rng(n + k)
+    A = rnd.integers(-3, 4, (k, n)).astype(np.float64)
+    dA = hip.DeviceArray.from_numpy(A)
+    dC = hip.DeviceArray.filled((n, n), 0xFF)
+    hip.gram(n, k, dA, dC, mirror=True)
+    C = dC.numpy()
+    assert np.array_equal(C, A.T @ A)
+    assert np.array_equal(C, C.T)
+
+
+def test_gram_upper_only_and_beta(hip):
+    rnd = np.random.default_rng(3)
+    n, k = 200, 3000
+    A = rnd.standard_normal((k, n))
+    C0 = rnd.standard_normal((n, n))
+    dA, dC = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(C0)
+    hip.gram(n, k, dA, dC, alpha=2.0, beta=0.5, mirror=False)
+    C = dC.numpy()
+    want = 2.0 * (A.T @ A) + 0.5 * C0
+    iu = np.triu_indices(n)
+    il = np.tril_indices(n, -1)
+    assert np.allclose(C[iu], want[iu], rtol=1e-13, atol=1e-10)
+    assert np.array_equal(C[il], C0[il])            # lower triangle untouched
+
+
+def test_potrf_matches_scipy_and_reports_info(hip):
+    rnd = np.random.default_rng(4)
+    n = 300
+    X = rnd.standard_normal((2 * n, n))
+    S = X.T @ X
+    dS = hip.DeviceArray.from_numpy(S)
+    assert hip.potrf_upper(n, dS) == 0
+    R = np.triu(dS.numpy())
+    assert np.allclose(R, sla.cholesky(S, lower=False), rtol=1e-10, atol=1e-12)
+    assert np.linalg.norm(R.T @ R - S) <= 1e-13 * np.linalg.norm(S)
+    # leading minor of order 3 not positive definite -> info = 3 (LAPACK)
+    B = np.eye(6) * 4.0
+    B[2, 2] = -1.0
+    dB = hip.DeviceArray.from_numpy(B)
+    assert hip.potrf_upper(6, dB) == 3
+
+
+def test_trtri_and_trsm(hip):
+    rnd = np.random.default_rng(5)
+    n, m = 257, 5000
+    R = np.triu(rnd.standard_normal((n, n))) + 4.0 * np.eye(n)
+    dR = hip.DeviceArray.from_numpy(R)
+    dX = hip.DeviceArray.filled((n, n), 0xFF)
+    hip.trtri_upper(n, dR, dX)
+    X = dX.numpy()
+    assert np.array_equal(np.tril(X, -1), np.zeros((n, n)))
+    assert np.linalg.norm(R @ X - np.eye(n)) < 1e-12
+    B = rnd.standard_normal((m, n))
+    dB = hip.DeviceArray.from_numpy(B)
+    hip.trsm_right_upper(m, n, 2.0, dR, dB)
+    want = 2.0 * sla.solve_triangular(R, B.T, trans="T", lower=False).T
+    assert np.allclose(dB.numpy(), want, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("cond", [1e2, 1e5])
+def test_cholesky_qr2(hip, cond):
+    m, n = 40000, 128
+    dA, _ = cholqr.test_matrix(m, n, cond, seed=11)
+    dQ, dR = cholqr.cholesky_qr2(dA, m, n)
+    orth, res = cholqr.qr_errors(dA, dQ, dR, m, n)
+    assert orth < 1e-13, orth
+    assert res < 1e-14, res
+    R = dR.numpy()
+    assert np.array_equal(np.tril(R, -1), np.zeros((n, n)))
+
+
+def test_cholesky_breakdown_is_reported(hip):
+    """A zero column makes pivot 6 of the first Gram exactly zero: the
+    breakdown surfaces as an error with LAPACK's info, as the reference
+    example prints it (examples/example_CholeskyQR.c:125-129)."""
+    m, n = 1000, 16
+    rnd = np.random.default_rng(9)
+    A = rnd.standard_normal((m, n))
+    A[:, 5] = 0.0
+    dA = hip.DeviceArray.from_numpy(A)
+    with pytest.raises(cholqr.CholeskyBreakdown) as ex:
+        cholqr.cholesky_qr2(dA, m, n)
+    assert ex.value.info == 6 and ex.value.step == 1
