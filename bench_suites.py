@@ -1,0 +1,253 @@
+"""Secondary measurements of bench.py (`python bench.py --suite NAME`).
+
+Each suite prints JSON lines (one per measurement) on stdout:
+  rank-share   one rank's share of the 4x2x3 grid on 8000^3: its two Cannon
+               rounds (2000 x 4000 x 1334+1333), the kp = 3 partial-C fold
+               over 2000 x 1334, and the mat_redist pack / unpack of the
+               1D-column-partitioned 8000 x 8000 A for all 24 ranks (and rank
+               0 alone), with achieved HBM GB/s on algorithmic bytes;
+  ozaki-sweep  native FP64-MFMA DGEMM vs Ozaki INT8 at 8192^3 for
+               phi in {0.5, 2, 4} and s in 3..11, error against double-double;
+  grouped      the simulated 4x2x3 grid on 10007^3: one launch per rank and
+               round vs the persistent grouped launch (untuned / tuned / graph);
+  cholqr       CholeskyQR2 of a 2e6 x 512 matrix with condition 1e2, 1e5, 1e8;
+  pmc-calib    a plain 1 GiB copy, then one of each hot kernel, for counter runs.
+"""
+import json
+import sys
+import time
+
+import numpy as np
+
+from ca3dmm_amd import device as hip
+from ca3dmm_amd import engine as E
+
+HBM_GBS = 8000.0
+FP64_TF = 78.6
+
+
+def emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def check_blocks(blocks, bufs, dt=8):
+    """Host-side bounds check of row-major block copies against the
+    allocations they read and write: every block must lie inside one buffer."""
+    spans = sorted((b.ptr, b.ptr + b.nbytes) for b in bufs)
+    def inside(addr, nr, nc, ld):
+        end = addr + dt * ((nr - 1) * ld + nc)
+        return any(lo <= addr and end <= hi for lo, hi in spans)
+    for s, lds, d, ldd, nr, nc in blocks:
+        if nr <= 0 or nc <= 0:
+            continue
+        if not (nc <= lds and nc <= ldd and inside(s, nr, nc, lds) and inside(d, nr, nc, ldd)):
+            raise ValueError(f"block copy out of bounds: {(s, lds, d, ldd, nr, nc)}")
+
+
+def timed(f, reps=5, warm=1):
+    for _ in range(warm):
+        f()
+    hip.sync()
+    t = hip.Timer()
+    t.start()
+    for _ in range(reps):
+        f()
+    return t.stop() / reps
+
+
+# ---------------------------------------------------------------------------
+def rank_share(args):
+    from oracle import oracle as O
+    hip.set_sync(False)
+    rnd = np.random.default_rng(1)
+    # two Cannon rounds of rank 0: C(2000 x 4000) = A0 B0 + A1 B1, k 1334 + 1333
+    m, n, k0, k1 = 2000, 4000, 1334, 1333
+    A0, A1 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (m, k0))), hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (m, k1)))
+    B0, B1 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (k0, n))), hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (k1, n)))
+    Cm = hip.DeviceArray.zeros((m, n))
+
+    def rounds():
+        hip.dgemm(0, 0, m, n, k0, 1.0, A0, B0, 0.0, Cm)
+        hip.dgemm(0, 0, m, n, k1, 1.0, A1, B1, 1.0, Cm)
+    ms = timed(rounds, reps=10)
+    fl = 2.0 * m * n * (k0 + k1)
+    emit({"suite": "rank-share", "what": "rank 0 Cannon rounds 2000x4000x(1334+1333)", "ms": round(ms, 4),
+          "tflops": round(fl / ms / 1e9, 2), "frac_fp64_peak": round(fl / ms / 1e9 / FP64_TF, 4)})
+    # kp = 3 fold of the 2000 x 1334 slice
+    P = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (2000, 1334))) for _ in range(3)]
+    out = hip.DeviceArray.zeros((2000, 1334))
+    ms = timed(lambda: hip.fold_sum(P, 2000, 1334, out), reps=50)
+    by = 4 * 8 * 2000 * 1334
+    emit({"suite": "rank-share", "what": "kp=3 fold 2000x1334", "ms": round(ms, 5), "bytes": by,
+          "gbs": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / HBM_GBS, 4)})
+    # the same fold on the whole 8000 x 8000 C (all ranks' slices), one launch
+    P8 = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000))) for _ in range(3)]
+    out8 = hip.DeviceArray.zeros((8000, 8000))
+    ms = timed(lambda: hip.fold_sum(P8, 8000, 8000, out8), reps=10)
+    by = 4 * 8 * 8000 * 8000
+    emit({"suite": "rank-share", "what": "kp=3 fold 8000x8000", "ms": round(ms, 4), "bytes": by,
+          "gbs": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / HBM_GBS, 4)})
+    del P8, out8
+    # pack / unpack of A for all 24 ranks (example_AB 1D column layout -> cuboid layout)
+    w = O.ca3dmm_world(8000, 8000, 8000, 0, 0, 24)
+    red = w["redist"]["A"]
+    Afull = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
+    sends = [hip.DeviceArray((max(1, red["plans"][r][1]),)) for r in range(24)]
+    recvs = [hip.DeviceArray((max(1, red["plans"][r][3]),)) for r in range(24)]
+    dsts = [hip.DeviceArray((max(1, red["req"][r][2] * red["req"][r][3]),)) for r in range(24)]
+    pack, unpack, pack0 = [], [], []
+    for r in range(24):
+        s = red["src"][r]
+        sl, sc, rl, rc = red["plans"][r]
+        base = Afull.at(s[0] * 8000)        # rank r's column block (row-major rows = columns)
+        for e in sl:
+            d = (base + 8 * ((e[0] - s[0]) * 8000 + (e[1] - s[1])), 8000, sends[r].at(e[5]), e[3], e[2], e[3])
+            pack.append(d)
+            if r == 0:
+                pack0.append(d)
+        q = red["req"][r]
+        for e in rl:
+            unpack.append((recvs[r].at(e[5]), e[3], dsts[r].at((e[0] - q[0]) * q[3] + (e[1] - q[1])), q[3], e[2], e[3]))
+    check_blocks(pack, [Afull] + sends)
+    check_blocks(unpack, recvs + dsts)
+    for name, lst in (("pack A, 24 ranks", pack), ("unpack A, 24 ranks", unpack), ("pack A, rank 0", pack0)):
+        elems = sum(d[4] * d[5] for d in lst)
+        ms = timed(lambda: hip.copy_blocks(8, lst), reps=20)
+        by = 16 * elems
+        emit({"suite": "rank-share", "what": name, "blocks": len(lst), "elements": elems, "ms": round(ms, 5),
+              "bytes": by, "gbs": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / HBM_GBS, 4)})
+    hip.set_sync(True)
+
+
+# ---------------------------------------------------------------------------
+def ozaki_sweep(args):
+    from oracle import oracle as O
+    from oracle import ozaki as Z
+    n = args.n
+    rnd = np.random.default_rng(7)
+    flops = 2.0 * n ** 3
+    for phi in (0.5, 2.0, 4.0):
+        A, B = Z.phi_matrix(rnd, (n, n), phi), Z.phi_matrix(rnd, (n, n), phi)
+        dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+        dC = hip.DeviceArray.zeros((n, n))
+        ii, jj = rnd.integers(0, n, 256), rnd.integers(0, n, 256)
+        ref, ab = O.dd_sample(0, 0, n, n, n, np.asfortranarray(A), n, np.asfortranarray(B), n, ii, jj)
+        hip.set_sync(False)
+        ms = timed(lambda: hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC), reps=2)
+        got = dC.numpy()[ii, jj]
+        nat = float(np.max(np.abs(got - ref) / ab))
+        emit({"suite": "ozaki-sweep", "phi": phi, "method": "dgemm-mfma", "ms": round(ms, 3),
+              "tflops": round(flops / ms / 1e9, 2), "max_err_over_absAabsB": nat,
+              "max_rel_err": float(np.max(np.abs(got - ref) / np.abs(ref)))})
+        for s in range(3, 12):
+            ms = timed(lambda: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, s), reps=2)
+            got = dC.numpy()[ii, jj]
+            err = float(np.max(np.abs(got - ref) / ab))
+            emit({"suite": "ozaki-sweep", "phi": phi, "method": "ozaki-int8", "s": s, "ms": round(ms, 3),
+                  "effective_tflops": round(flops / ms / 1e9, 2),
+                  "int8_tops": round(Z.products(s) * flops / ms / 1e9, 1),
+                  "max_err_over_absAabsB": err, "max_rel_err": float(np.max(np.abs(got - ref) / np.abs(ref))),
+                  "no_worse_than_native": err <= nat})
+        hip.set_sync(True)
+        del dA, dB, dC
+
+
+# ---------------------------------------------------------------------------
+def grouped(args):
+    from oracle import oracle as O
+    m = n = k = 10007
+    p = 24
+    rnd = np.random.default_rng(3)
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=[4, 2, 3])
+    A = rnd.uniform(-1, 1, (m, k))
+    B = rnd.uniform(-1, 1, (k, n))
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    del A, B
+    dC = hip.DeviceArray.zeros((m, n))
+    Ap = [dA.at(r[2] * m) for r in w["src_A"]]
+    Bp = [dB.at(r[2] * k) for r in w["src_B"]]
+    Cp = [dC.at(r[2] * m) for r in w["dst_C"]]
+    sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=[4, 2, 3])
+    flops = 2.0 * m * n * k
+    run = lambda: sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
+    hip.set_sync(False)
+    ms = timed(run, reps=3)
+    ref = dC.numpy()
+    emit({"suite": "grouped", "mode": "per-rank, per-round launches", "ms": round(ms, 3),
+          "tflops": round(flops / ms / 1e9, 2)})
+    sim.set_mode(grouped=True, graph=False)
+    sim.set_tuning(1, 1)
+    ms = timed(run, reps=3)
+    d = float(np.max(np.abs(dC.numpy() - ref)))
+    emit({"suite": "grouped", "mode": "grouped, untuned (rows=1, 1 WG/CU)", "ms": round(ms, 3),
+          "tflops": round(flops / ms / 1e9, 2), "max_abs_diff_vs_per_launch": d})
+    g, wg = sim.autotune(2)
+    ms = timed(run, reps=3)
+    emit({"suite": "grouped", "mode": f"grouped, tuned (rows={g}, {wg} WG/CU)", "ms": round(ms, 3),
+          "tflops": round(flops / ms / 1e9, 2)})
+    sim.set_mode(grouped=True, graph=True)
+    ms = timed(run, reps=3)
+    emit({"suite": "grouped", "mode": "grouped, tuned, hipGraph replay", "ms": round(ms, 3),
+          "tflops": round(flops / ms / 1e9, 2), "max_abs_diff_vs_per_launch": float(np.max(np.abs(dC.numpy() - ref)))})
+    hip.set_sync(True)
+    sim.free()
+
+
+# ---------------------------------------------------------------------------
+def cholqr(args):
+    from ca3dmm_amd import cholqr as CQ
+    m, n = 2000000, 512
+    dU = None
+    for cond in (1e2, 1e5, 1e8):
+        t0 = time.perf_counter()
+        dA, dU = CQ.test_matrix(m, n, cond, seed=2022, dU=dU)
+        log(f"cholqr: matrix kappa={cond:g} built in {time.perf_counter() - t0:.1f} s")
+        hip.set_sync(False)
+        dG = hip.DeviceArray.zeros((n, n))
+        ms_gram = timed(lambda: hip.gram(n, m, dA, dG), reps=3)
+        hip.set_sync(True)
+        rec = {"suite": "cholqr", "m": m, "n": n, "cond": cond, "gram_ms": round(ms_gram, 3),
+               "gram_tflops": round(n * (n + 1) * m / ms_gram / 1e9, 2)}
+        try:
+            hip.sync()
+            t = hip.Timer()
+            t.start()
+            dQ, dR = CQ.cholesky_qr2(dA, m, n)
+            rec["cholqr2_ms"] = round(t.stop(), 2)
+            orth, res = CQ.qr_errors(dA, dQ, dR, m, n)
+            rec.update({"orth_QtQ_minus_I_F": orth, "residual_rel_F": res})
+            del dQ, dR
+        except CQ.CholeskyBreakdown as ex:
+            rec.update({"breakdown": str(ex), "info": ex.info, "pass": ex.step})
+        emit(rec)
+        del dA, dG
+
+
+# ---------------------------------------------------------------------------
+def pmc_calib(args):
+    """One plain copy of 1 GiB (the counter scale), then one launch of each
+    hot kernel at its bench size."""
+    nel = (1 << 30) // 8
+    src = hip.DeviceArray.zeros((nel,))
+    dst = hip.DeviceArray.zeros((nel,))
+    hip.copy_matrix_block(8, 1, nel, src.ptr, nel, dst.ptr, nel)
+    n = args.n
+    rnd = np.random.default_rng(5)
+    dA = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+    dC = hip.DeviceArray.zeros((n, n))
+    hip.dgemm(0, 0, n, n, n, 1.0, dA, dA, 0.0, dC)
+    P = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (2000, 1334))) for _ in range(3)]
+    out = hip.DeviceArray.zeros((2000, 1334))
+    hip.fold_sum(P, 2000, 1334, out)
+    hip.sync()
+    emit({"suite": "pmc-calib", "copy_bytes_read": nel * 8, "copy_bytes_written": nel * 8,
+          "dgemm_n": n, "fold": "3 x 2000x1334"})
+
+
+def run(name, args):
+    {"rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "grouped": grouped, "cholqr": cholqr,
+     "pmc-calib": pmc_calib}[name](args)

@@ -19,6 +19,7 @@
 #include "ca3dmm.h"
 #include "ca3dmm_ext.h"
 #include "ca3dmm_sim.h"
+#include "grouped.h"
 #include "hip_proxy.h"
 
 #define ENG_ERR(...) fprintf(stderr, "[ca3dmm_amd] " __VA_ARGS__)
@@ -1073,6 +1074,9 @@ void ca3dmm_engine_print_stat(ca3dmm_engine_p e)
 struct ca3dmm_sim {
     Problem P;
     std::vector<ca3dmm_engine_p> eng;
+    int grouped = 0;              // Cannon rounds + k fold as one persistent launch
+    int use_graph = 0;            // replay that launch from a captured hipGraph
+    ca3::grouped::Plan *plan = nullptr;
 };
 
 namespace {
@@ -1083,6 +1087,68 @@ int cannon_member(const Problem &P, int r, int row, int col)
     const RankPlan &pl = P.plans[r];
     const int task = pl.tmi + pl.tni;
     return pl.rank_k * P.grid.mp * P.grid.np + task * pl.group + row * pl.q + col;
+}
+
+// Problems and fold groups of the persistent grouped launch: one problem
+// per active rank (its q Cannon rounds as k-segments read in place), one
+// fold group per mn-position when kp > 1.
+int build_grouped(ca3dmm_sim *S)
+{
+    const Problem &P = S->P;
+    namespace G = ca3::grouped;
+    std::vector<G::Problem> probs;
+    std::vector<G::Fold> folds;
+    const int mn = P.grid.mp * P.grid.np, kp = P.grid.kp;
+    std::vector<int> fold_of(mn, -1);
+    if (kp > 1) {
+        if (kp > G::kMaxKp) return HIP_PROXY_ERR_UNSUPP;
+        for (int r = 0; r < mn; ++r) {
+            ca3dmm_engine_p e = S->eng[r];
+            if (!e->is_active || !e->C_2dmm_nrow || !e->C_2dmm_ncol) continue;
+            G::Fold f;
+            memset(&f, 0, sizeof(f));
+            f.kp = kp;
+            f.M = e->C_2dmm_nrow; f.N = e->C_2dmm_ncol;
+            f.ldl = e->C_2dmm_nrow;
+            f.ldo = std::max(1, e->C_out_nrow);
+            for (int l = 0; l < kp; ++l) {
+                ca3dmm_engine_p el = S->eng[l * mn + r];
+                f.Cl[l] = (double *) el->C_2dmm_d;
+                f.Cout[l] = (double *) el->C_out_d;
+                f.cs[l] = el->C_out_scol - e->C_2dmm_scol;
+            }
+            f.cs[kp] = f.N;
+            fold_of[r] = (int) folds.size();
+            folds.push_back(f);
+        }
+    }
+    for (int r = 0; r < P.p; ++r) {
+        ca3dmm_engine_p e = S->eng[r];
+        if (!e->is_active) continue;
+        const cannon_engine_p ce = e->cannon_engine;
+        const int q = ce->np_dim, row = ce->rank_row, col = ce->rank_col;
+        const int Am = ce->m_displs[row + 1] - ce->m_displs[row];
+        const int Bn = ce->n_displs[col + 1] - ce->n_displs[col];
+        if (!Am || !Bn) continue;
+        if (q > G::kMaxSeg) return HIP_PROXY_ERR_UNSUPP;
+        G::Problem pr;
+        memset(&pr, 0, sizeof(pr));
+        pr.M = Am; pr.N = Bn;
+        for (int s2 = 0; s2 < q; ++s2) {
+            const int off = (row + col + s2) % q;
+            const int lk = ce->k_displs[off + 1] - ce->k_displs[off];
+            if (!lk) continue;
+            G::Seg &sg = pr.seg[pr.nseg++];
+            sg.A = (const double *) S->eng[cannon_member(P, r, row, off)]->A_2dmm_d;
+            sg.B = (const double *) S->eng[cannon_member(P, r, off, col)]->B_2dmm_d;
+            sg.lda = Am; sg.ldb = lk; sg.k = lk;
+        }
+        pr.C = (double *) (kp == 1 ? e->C_out_d : e->C_2dmm_d);
+        pr.ldc = Am;
+        pr.fold = (kp > 1) ? fold_of[r % mn] : -1;
+        probs.push_back(pr);
+    }
+    return G::build(probs, folds, &S->plan);
 }
 
 int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
@@ -1165,10 +1231,17 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
         if (rc) return rc;
     }
 
+    // 3.+4. as one persistent grouped launch (rounds accumulated in
+    //    registers, fold done by the last layer of each tile)
+    if (S->grouped) {
+        if (!S->plan && (rc = build_grouped(S))) return rc;
+        if ((rc = ca3::grouped::run(S->plan, S->use_graph != 0))) return rc;
+    }
+
     // 3. Cannon rounds: round s of rank (row, col) multiplies the A block of
     //    (row, (row+col+s) % q) by the B block of ((row+col+s) % q, col),
     //    beta = 0 in round 0 and 1 afterwards (cannon.c:319-433).
-    for (int r = 0; r < p && !rc; ++r) {
+    for (int r = 0; r < p && !rc && !S->grouped; ++r) {
         ca3dmm_engine_p e = S->eng[r];
         if (!e->is_active) continue;
         const cannon_engine_p ce = e->cannon_engine;
@@ -1190,7 +1263,7 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     if (rc) return rc;
 
     // 4. k-layer reduction in rank (layer) order (ca3dmm.c:1287-1335)
-    for (int r = 0; r < p && !rc; ++r) {
+    for (int r = 0; r < p && !rc && !S->grouped; ++r) {
         ca3dmm_engine_p e = S->eng[r];
         if (!e->is_active || e->task_k_num == 1 || !e->C_out_nrow || !e->C_out_ncol) continue;
         const int kp = e->task_k_num, mn = P.grid.mp * P.grid.np;
@@ -1292,9 +1365,44 @@ int ca3dmm_sim_exec(ca3dmm_sim_p sim, const void *const *src_A, const int *ldA, 
     return rc;
 }
 
+int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph)
+{
+    if (!sim) return HIP_PROXY_ERR_ARG;
+    sim->grouped = grouped ? 1 : 0;
+    sim->use_graph = use_graph ? 1 : 0;
+    return HIP_PROXY_OK;
+}
+
+int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu)
+{
+    if (!sim) return HIP_PROXY_ERR_ARG;
+    for (auto e : sim->eng)
+        if (!e->workbuf_d && total_bytes(e)) return HIP_PROXY_ERR_ARG;
+    int rc = 0;
+    if (!sim->plan && (rc = build_grouped(sim))) return rc;
+    if ((rc = ca3::grouped::autotune(sim->plan, std::max(1, reps)))) return rc;
+    const ca3::grouped::Tuning &t = ca3::grouped::tuning(sim->plan);
+    if (group_rows) *group_rows = t.group_rows;
+    if (wg_per_cu) *wg_per_cu = t.wg_per_cu;
+    return hip_stream_sync(nullptr);
+}
+
+int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu)
+{
+    if (!sim || group_rows < 1 || wg_per_cu < 1 || wg_per_cu > 2) return HIP_PROXY_ERR_ARG;
+    int rc = 0;
+    if (!sim->plan && (rc = build_grouped(sim))) return rc;
+    ca3::grouped::Tuning t;
+    t.group_rows = group_rows;
+    t.wg_per_cu = wg_per_cu;
+    ca3::grouped::set_tuning(sim->plan, t);
+    return HIP_PROXY_OK;
+}
+
 void ca3dmm_sim_free(ca3dmm_sim_p *sim)
 {
     if (!sim || !*sim) return;
+    if ((*sim)->plan) ca3::grouped::destroy((*sim)->plan);
     for (auto &e : (*sim)->eng) ca3dmm_engine_free(&e);
     delete *sim;
     *sim = nullptr;

@@ -1,0 +1,58 @@
+// Internal interface of the persistent grouped GEMM (grouped.hip), used by
+// the simulated-grid engine (engine.cpp).  No HIP types appear here.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace ca3 {
+namespace grouped {
+
+constexpr int kMaxSeg = 8;    // Cannon rounds per problem
+constexpr int kMaxKp = 8;     // k layers per fold group
+
+struct Seg {                  // one k-segment (Cannon round) of a problem
+    const double *A;          // m x k, column-major, leading dimension lda
+    const double *B;          // k x n, column-major, leading dimension ldb
+    int64_t lda, ldb;
+    int k;
+};
+
+struct Problem {              // C (M x N, ldc) := sum over segments of A_s B_s
+    int M, N, nseg;
+    Seg seg[kMaxSeg];
+    double *C;
+    int64_t ldc;
+    int fold;                 // fold group index or -1
+};
+
+struct Fold {                 // k-layer reduction of the C blocks of kp problems
+    int kp;
+    double *Cl[kMaxKp];       // the layers' C blocks (same shape and ld)
+    int64_t ldl;
+    int M, N;
+    int cs[kMaxKp + 1];       // column slice t = [cs[t], cs[t+1]) goes to Cout[t]
+    double *Cout[kMaxKp];
+    int64_t ldo;
+};
+
+struct Tuning {
+    int group_rows = 8;       // tile rows walked together in the queue order
+    int wg_per_cu = 2;        // persistent workgroups per CU
+};
+
+struct Plan;
+
+// Build device tables for the problems; 0 on success.
+int build(const std::vector<Problem> &probs, const std::vector<Fold> &folds, Plan **out);
+// Run once (counters reset inside the same stream order); use_graph replays
+// a captured hipGraph.
+int run(Plan *p, bool use_graph);
+// Time the candidate tunings and keep the fastest; returns 0 on success.
+int autotune(Plan *p, int reps);
+const Tuning &tuning(const Plan *p);
+void set_tuning(Plan *p, const Tuning &t);
+void destroy(Plan *p);
+
+}  // namespace grouped
+}  // namespace ca3

@@ -113,6 +113,9 @@ _SIGS = {
     "ca3dmm_sim_exec": (I, [P, C.POINTER(P), IP, C.POINTER(P), IP, C.POINTER(P), IP]),
     "ca3dmm_sim_c_out": (P, [P, I]),
     "ca3dmm_sim_free": (None, [C.POINTER(P)]),
+    "ca3dmm_sim_set_mode": (I, [P, I, I]),
+    "ca3dmm_sim_autotune": (I, [P, I, IP, IP]),
+    "ca3dmm_sim_set_tuning": (I, [P, I, I]),
 }
 for _n, (_r, _a) in _SIGS.items():
     _f = getattr(lib, _n)
@@ -156,6 +159,18 @@ class SimGrid:
         return check(lib.ca3dmm_sim_exec(self.h, arr(A_ptrs), _ints(ldA) if ldA is not None else None,
                                          arr(B_ptrs), _ints(ldB), arr(C_ptrs),
                                          _ints(ldC) if ldC is not None else None), "ca3dmm_sim_exec")
+
+    def set_mode(self, grouped=False, graph=False):
+        return check(lib.ca3dmm_sim_set_mode(self.h, 1 if grouped else 0, 1 if graph else 0),
+                     "ca3dmm_sim_set_mode")
+
+    def autotune(self, reps=3):
+        g, w = C.c_int(), C.c_int()
+        check(lib.ca3dmm_sim_autotune(self.h, reps, C.byref(g), C.byref(w)), "ca3dmm_sim_autotune")
+        return g.value, w.value
+
+    def set_tuning(self, group_rows, wg_per_cu):
+        return check(lib.ca3dmm_sim_set_tuning(self.h, group_rows, wg_per_cu), "ca3dmm_sim_set_tuning")
 
     def c_out(self, r):
         return lib.ca3dmm_sim_c_out(self.h, r)

@@ -53,6 +53,21 @@ int ca3dmm_sim_exec(ca3dmm_sim_p sim, const void *const *src_A, const int *ldA,
  * leading dimension C_out_nrow) after exec. */
 void *ca3dmm_sim_c_out(ca3dmm_sim_p sim, int rank);
 
+/* grouped != 0: run the Cannon rounds of all ranks and the k-layer fold as
+ * ONE persistent launch pulling 128 x 128 tiles from a device work queue
+ * (rounds accumulated in registers; the last layer to finish a tile adds the
+ * kp layer tiles in layer order).  use_graph != 0: replay that launch from a
+ * captured hipGraph.  Default: one DGEMM launch per rank and round. */
+int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph);
+
+/* Time the grouped launch over its tunings (tile rows walked together in
+ * the queue order: 1, 4, 8, 16; persistent workgroups per CU: 1, 2), keep
+ * the fastest for this grid's shape signature and report it. */
+int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu);
+
+/* Fix the grouped launch's tuning by hand (e.g. the untuned baseline). */
+int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu);
+
 void ca3dmm_sim_free(ca3dmm_sim_p *sim);
 
 #ifdef __cplusplus

@@ -15,7 +15,8 @@ def linear_fill(nrow, ncol, a, b):
     return np.asfortranarray(a * i + b * j)
 
 
-def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False):
+def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False, grouped=False,
+            graph=False, autotune=False):
     """Run the simulated p-rank grid with the example_AB 1D column layout.
     A, B: full matrices as stored (numpy, any order).  Returns the full C
     gathered from the ranks' dst_C blocks (and the SimGrid when keep)."""
@@ -32,10 +33,14 @@ def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False):
     ldB = [B.shape[0]] * p
     Cp = [dC.at(r_[2] * mm) for r_ in w["dst_C"]]
     ldC = [mm] * p
+    if grouped:
+        sim.set_mode(grouped=True, graph=graph)
+        if autotune:
+            sim.autotune(1)
     sim.exec(Ap, ldA, Bp, ldB, Cp, ldC)
     out = dC.numpy()
     if keep:
-        return out, sim, w
+        return out, sim, w, (Ap, ldA, Bp, ldB, Cp, ldC, dC)
     sim.free()
     return out
 

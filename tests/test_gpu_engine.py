@@ -98,7 +98,7 @@ def test_sim_example_ab_2000_p4_linear_fill(hip):
     m = n = k = 2000
     A = linear_fill(m, k, 0.19, 0.24)
     B = linear_fill(k, n, 0.11, 0.12)
-    got, sim, w = run_sim(4, m, n, k, 0, 0, A, B, keep=True)
+    got, sim, w, _ = run_sim(4, m, n, k, 0, 0, A, B, keep=True)
     e = sim.engine(0)
     assert (e.mp, e.np, e.kp) == (2, 1, 2)
     ref = A @ B
@@ -115,7 +115,7 @@ def test_sim_24_ranks_uneven_exact(hip, ta, tb):
     m = n = k = 1001
     A = _ints(rnd, (k, m) if ta else (m, k))
     B = _ints(rnd, (n, k) if tb else (k, n))
-    got, sim, w = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3], keep=True)
+    got, sim, w, _ = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3], keep=True)
     assert np.array_equal(got, (A.T if ta else A) @ (B.T if tb else B))
     sim.free()
 
@@ -137,7 +137,7 @@ def test_sim_idle_ranks(hip):
     rnd = np.random.default_rng(8)
     m, n, k = 130, 77, 210
     A, B = _ints(rnd, (m, k)), _ints(rnd, (k, n))
-    got, sim, w = run_sim(7, m, n, k, 0, 0, A, B, proc_grid=[2, 1, 3], keep=True)
+    got, sim, w, _ = run_sim(7, m, n, k, 0, 0, A, B, proc_grid=[2, 1, 3], keep=True)
     assert sim.engine(6).is_active == 0 and sim.engine(0).rp == 1
     assert np.array_equal(got, A @ B)
     sim.free()
