@@ -87,7 +87,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--suite", default=None,
-                    choices=["rank-share", "ozaki-sweep", "grouped", "cholqr", "pmc-calib"],
+                    choices=["rank-share", "ozaki-sweep", "grouped", "cholqr", "pmc-calib", "yardstick"],
                     help="run one of the secondary measurements instead (JSON lines)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -11,7 +11,8 @@ Each suite prints JSON lines (one per measurement) on stdout:
   grouped      the simulated 4x2x3 grid on 10007^3: one launch per rank and
                round vs the persistent grouped launch (untuned / tuned / graph);
   cholqr       CholeskyQR2 of a 2e6 x 512 matrix with condition 1e2, 1e5, 1e8;
-  pmc-calib    a plain 1 GiB copy, then one of each hot kernel, for counter runs.
+  pmc-calib    a plain 1 GiB copy, then one of each hot kernel, for counter runs;
+  yardstick    rocBLAS dgemm and INT8 gemm_ex at the same size (never on the path).
 """
 import json
 import sys
@@ -125,35 +126,40 @@ def rank_share(args):
 
 # ---------------------------------------------------------------------------
 def ozaki_sweep(args):
+    """Error on 4096 sampled entries against double-double, per (shape, phi, s)."""
     from oracle import oracle as O
     from oracle import ozaki as Z
-    n = args.n
     rnd = np.random.default_rng(7)
-    flops = 2.0 * n ** 3
-    for phi in (0.5, 2.0, 4.0):
-        A, B = Z.phi_matrix(rnd, (n, n), phi), Z.phi_matrix(rnd, (n, n), phi)
-        dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
-        dC = hip.DeviceArray.zeros((n, n))
-        ii, jj = rnd.integers(0, n, 256), rnd.integers(0, n, 256)
-        ref, ab = O.dd_sample(0, 0, n, n, n, np.asfortranarray(A), n, np.asfortranarray(B), n, ii, jj)
-        hip.set_sync(False)
-        ms = timed(lambda: hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC), reps=2)
-        got = dC.numpy()[ii, jj]
-        nat = float(np.max(np.abs(got - ref) / ab))
-        emit({"suite": "ozaki-sweep", "phi": phi, "method": "dgemm-mfma", "ms": round(ms, 3),
-              "tflops": round(flops / ms / 1e9, 2), "max_err_over_absAabsB": nat,
-              "max_rel_err": float(np.max(np.abs(got - ref) / np.abs(ref)))})
-        for s in range(3, 12):
-            ms = timed(lambda: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, s), reps=2)
-            got = dC.numpy()[ii, jj]
-            err = float(np.max(np.abs(got - ref) / ab))
-            emit({"suite": "ozaki-sweep", "phi": phi, "method": "ozaki-int8", "s": s, "ms": round(ms, 3),
-                  "effective_tflops": round(flops / ms / 1e9, 2),
-                  "int8_tops": round(Z.products(s) * flops / ms / 1e9, 1),
-                  "max_err_over_absAabsB": err, "max_rel_err": float(np.max(np.abs(got - ref) / np.abs(ref))),
-                  "no_worse_than_native": err <= nat})
-        hip.set_sync(True)
-        del dA, dB, dC
+    shapes = [(args.n, args.n, args.n), (65536, 65536, 1024)]
+    for m, n, k in shapes:
+        flops = 2.0 * m * n * k
+        for phi in (0.5, 2.0, 4.0):
+            A, B = Z.phi_matrix(rnd, (m, k), phi), Z.phi_matrix(rnd, (k, n), phi)
+            dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+            dC = hip.DeviceArray.zeros((m, n))
+            ii, jj = rnd.integers(0, m, 4096), rnd.integers(0, n, 4096)
+            ref, ab = O.dd_sample(0, 0, m, n, k, np.asfortranarray(A), m, np.asfortranarray(B), k, ii, jj)
+            del A, B
+            hip.set_sync(False)
+            nat_ms = ms = timed(lambda: hip.dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC), reps=2)
+            got = dC.sample(ii, jj)
+            nat = float(np.max(np.abs(got - ref) / ab))
+            base = {"suite": "ozaki-sweep", "m": m, "n": n, "k": k, "phi": phi, "samples": 4096}
+            emit(dict(base, method="dgemm-mfma", ms=round(ms, 3), tflops=round(flops / ms / 1e9, 2),
+                      max_err_over_absAabsB=nat,
+                      max_rel_err=float(np.max(np.abs(got - ref) / np.abs(ref)))))
+            for s in range(3, 12):
+                ms = timed(lambda: hip.ozaki_dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, s), reps=2)
+                got = dC.sample(ii, jj)
+                err = float(np.max(np.abs(got - ref) / ab))
+                emit(dict(base, method="ozaki-int8", s=s, ms=round(ms, 3),
+                          effective_tflops=round(flops / ms / 1e9, 2),
+                          int8_tops=round(Z.products(s) * flops / ms / 1e9, 1),
+                          max_err_over_absAabsB=err,
+                          max_rel_err=float(np.max(np.abs(got - ref) / np.abs(ref))),
+                          no_worse_than_native=err <= nat, speedup_vs_native=round(nat_ms / ms, 3)))
+            hip.set_sync(True)
+            del dA, dB, dC
 
 
 # ---------------------------------------------------------------------------
@@ -199,6 +205,8 @@ def grouped(args):
 
 # ---------------------------------------------------------------------------
 def cholqr(args):
+    """CholeskyQR2 of 2e6 x 512 at three condition numbers, with the Gram
+    through FP64 MFMA and through Ozaki INT8 (s = 7, 8)."""
     from ca3dmm_amd import cholqr as CQ
     m, n = 2000000, 512
     dU = None
@@ -206,35 +214,45 @@ def cholqr(args):
         t0 = time.perf_counter()
         dA, dU = CQ.test_matrix(m, n, cond, seed=2022, dU=dU)
         log(f"cholqr: matrix kappa={cond:g} built in {time.perf_counter() - t0:.1f} s")
-        hip.set_sync(False)
         dG = hip.DeviceArray.zeros((n, n))
-        ms_gram = timed(lambda: hip.gram(n, m, dA, dG), reps=3)
-        hip.set_sync(True)
-        rec = {"suite": "cholqr", "m": m, "n": n, "cond": cond, "gram_ms": round(ms_gram, 3),
-               "gram_tflops": round(n * (n + 1) * m / ms_gram / 1e9, 2)}
-        try:
-            hip.sync()
-            t = hip.Timer()
-            t.start()
-            dQ, dR = CQ.cholesky_qr2(dA, m, n)
-            rec["cholqr2_ms"] = round(t.stop(), 2)
-            orth, res = CQ.qr_errors(dA, dQ, dR, m, n)
-            rec.update({"orth_QtQ_minus_I_F": orth, "residual_rel_F": res})
-            del dQ, dR
-        except CQ.CholeskyBreakdown as ex:
-            rec.update({"breakdown": str(ex), "info": ex.info, "pass": ex.step})
-        emit(rec)
+        for gs in (None, 7, 8):
+            hip.set_sync(False)
+            if gs:
+                ms_gram = timed(lambda: hip.ozaki_gram(n, m, dA, dG, gs), reps=3)
+            else:
+                ms_gram = timed(lambda: hip.gram(n, m, dA, dG), reps=3)
+            hip.set_sync(True)
+            rec = {"suite": "cholqr", "m": m, "n": n, "cond": cond,
+                   "gram": f"ozaki-int8 s={gs}" if gs else "fp64-mfma", "gram_ms": round(ms_gram, 3),
+                   "gram_tflops": round(n * (n + 1) * m / ms_gram / 1e9, 2)}
+            try:
+                hip.sync()
+                t = hip.Timer()
+                t.start()
+                dQ, dR = CQ.cholesky_qr2(dA, m, n, gram_slices=gs)
+                rec["cholqr2_ms"] = round(t.stop(), 2)
+                orth, res = CQ.qr_errors(dA, dQ, dR, m, n)
+                rec.update({"orth_QtQ_minus_I_F": orth, "residual_rel_F": res})
+                del dQ, dR
+            except CQ.CholeskyBreakdown as ex:
+                rec.update({"breakdown": str(ex), "info": ex.info, "pass": ex.step})
+            emit(rec)
         del dA, dG
 
 
 # ---------------------------------------------------------------------------
 def pmc_calib(args):
-    """One plain copy of 1 GiB (the counter scale), then one launch of each
-    hot kernel at its bench size."""
+    """Counter-run program: a plain copy of 1 GiB (the scale of FETCH_SIZE /
+    WRITE_SIZE), then one launch of each hot kernel at its bench size: NN
+    DGEMM n^3, the kp = 3 fold over 2000 x 1334, the pack of A for the 24
+    ranks of the 4x2x3 grid on 8000^3, and an 8-slice Ozaki product n^3."""
+    from oracle import oracle as O
     nel = (1 << 30) // 8
     src = hip.DeviceArray.zeros((nel,))
     dst = hip.DeviceArray.zeros((nel,))
     hip.copy_matrix_block(8, 1, nel, src.ptr, nel, dst.ptr, nel)
+    hip.sync()
+    del src, dst
     n = args.n
     rnd = np.random.default_rng(5)
     dA = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
@@ -243,11 +261,73 @@ def pmc_calib(args):
     P = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (2000, 1334))) for _ in range(3)]
     out = hip.DeviceArray.zeros((2000, 1334))
     hip.fold_sum(P, 2000, 1334, out)
+    w = O.ca3dmm_world(8000, 8000, 8000, 0, 0, 24)
+    red = w["redist"]["A"]
+    Afull = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
+    sends = [hip.DeviceArray((max(1, red["plans"][r][1]),)) for r in range(24)]
+    pack = []
+    for r in range(24):
+        s0 = red["src"][r]
+        base = Afull.at(s0[0] * 8000)
+        for e in red["plans"][r][0]:
+            pack.append((base + 8 * ((e[0] - s0[0]) * 8000 + (e[1] - s0[1])), 8000, sends[r].at(e[5]),
+                         e[3], e[2], e[3]))
+    check_blocks(pack, [Afull] + sends)
+    hip.copy_blocks(8, pack)
+    hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dA, 0.0, dC, 8)
     hip.sync()
     emit({"suite": "pmc-calib", "copy_bytes_read": nel * 8, "copy_bytes_written": nel * 8,
-          "dgemm_n": n, "fold": "3 x 2000x1334"})
+          "dgemm_n": n, "fold": "3 x 2000x1334", "pack_elements": sum(d[4] * d[5] for d in pack),
+          "ozaki": f"{n}^3 s=8"})
+
+
+def yardstick(args):
+    """rocBLAS dgemm and INT8 gemm_ex at n^3, timed as a yardstick for the
+    hand-written kernels (the product path never calls rocBLAS)."""
+    import ctypes as C
+    rb = C.CDLL("librocblas.so")
+    h = C.c_void_p()
+    assert rb.rocblas_create_handle(C.byref(h)) == 0
+    rb.rocblas_dgemm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(C.c_double), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                 C.POINTER(C.c_double), C.c_void_p, C.c_int]
+    rb.rocblas_gemm_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_int32, C.c_uint32]
+    hiprt = C.CDLL("libamdhip64.so")
+    dsync = lambda: hiprt.hipDeviceSynchronize()
+    NONE, TRANS, F64, I8, I32 = 111, 112, 152, 160, 162
+    n = args.n
+    rnd = np.random.default_rng(9)
+    dA = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+    dC = hip.DeviceArray.zeros((n, n))
+    one, zero = C.c_double(1.0), C.c_double(0.0)
+
+    def wall(f, reps):
+        f()
+        dsync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        dsync()
+        return (time.perf_counter() - t0) * 1e3 / reps
+    for ta, tb, name in ((NONE, NONE, "NN"), (TRANS, NONE, "TN")):
+        ms = wall(lambda: rb.rocblas_dgemm(h, ta, tb, n, n, n, C.byref(one), dA.ptr, n, dA.ptr, n,
+                                           C.byref(zero), dC.ptr, n), 5)
+        emit({"suite": "yardstick", "lib": "rocblas_dgemm", "case": name, "n": n, "ms": round(ms, 3),
+              "tflops": round(2.0 * n ** 3 / ms / 1e9, 2)})
+    del dA, dC
+    a8 = hip.DeviceArray.filled((n, n), 1, dtype=np.int8)
+    c32 = hip.DeviceArray.zeros((n, n), dtype=np.int32)
+    ia, ib = C.c_int32(1), C.c_int32(0)
+    ms = wall(lambda: rb.rocblas_gemm_ex(h, TRANS, NONE, n, n, n, C.byref(ia), a8.ptr, I8, n, a8.ptr, I8, n,
+                                         C.byref(ib), c32.ptr, I32, n, c32.ptr, I32, n, I32, 0, 0, 0), 5)
+    emit({"suite": "yardstick", "lib": "rocblas_gemm_ex i8->i32", "case": "TN", "n": n, "ms": round(ms, 3),
+          "tops": round(2.0 * n ** 3 / ms / 1e9, 1)})
+    rb.rocblas_destroy_handle(h)
 
 
 def run(name, args):
-    {"rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "grouped": grouped, "cholqr": cholqr,
+    {"yardstick": yardstick, "rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "grouped": grouped, "cholqr": cholqr,
      "pmc-calib": pmc_calib}[name](args)

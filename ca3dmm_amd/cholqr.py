@@ -25,15 +25,20 @@ def _zero_lower(R):
     return hip.DeviceArray.from_numpy(h)
 
 
-def cholesky_qr2(dA, m, n, passes=2):
-    """dA: device m x n (column-major).  Returns (dQ, dR, stats) with Q m x n
-    orthonormal and R n x n upper triangular, A = Q R."""
+def cholesky_qr2(dA, m, n, passes=2, gram_slices=None):
+    """dA: device m x n (column-major).  Returns (dQ, dR) with Q m x n
+    orthonormal and R n x n upper triangular, A = Q R.  gram_slices: compute
+    each Gram through the Ozaki scheme with that many INT8 slices
+    (hip_ozaki_gram) instead of the FP64 MFMA kernel."""
     dQ = hip.DeviceArray((m, n))
     hip.copy_matrix_block(8, n, m, dA.ptr, m, dQ.ptr, m)
     dR = None
     for step in range(1, passes + 1):
         dG = hip.DeviceArray.zeros((n, n))
-        hip.gram(n, m, dQ, dG)
+        if gram_slices:
+            hip.ozaki_gram(n, m, dQ, dG, gram_slices)
+        else:
+            hip.gram(n, m, dQ, dG)
         info = hip.potrf_upper(n, dG)
         if info != 0:
             raise CholeskyBreakdown(info, step)

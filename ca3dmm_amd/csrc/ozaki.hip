@@ -26,6 +26,7 @@
 // in k order (the oracle follows the same order).
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -95,36 +96,35 @@ __global__ void fill_int_kernel(int *p, int n, int v)
 // ---------------------------------------------------------------------
 // 2. slices
 // ---------------------------------------------------------------------
-// Balanced base-256 digits of x * 2^-E, |.| < 1/4, as digit t (0 = most
-// significant) of rint(x * 2^(8s - E)).
-__device__ __forceinline__ void digits_of(double x, int E, int s, int (&dig)[kMaxSlices])
+// Balanced base-256 digits of x * 2^-E, |.| < 1/4, as digits of the integer
+// N = rint(x * 2^(8s - E)) (digit s-1-t goes to slice t, t = 0 the most
+// significant).  N = X * 256^off with X = M << (z mod 8) (M the 53-bit
+// significand, |X| < 2^61) or, when the scale drops bits, X = rint(M 2^z),
+// off = 0.  Adding the bias B = 0x8080...80 maps the balanced digits
+// c_i in [-128, 127] of X to the plain bytes c_i + 128 of U = X + B (mod
+// 2^64), so digit i is byte i of U xor 0x80 read as int8; eight bytes cover
+// |X| < 2^61 and the digits outside the window are zero.
+__device__ __forceinline__ void window_of(double x, int E, int s, uint64_t &U, int &off)
 {
-#pragma unroll
-    for (int t = 0; t < kMaxSlices; ++t) dig[t] = 0;
+    const uint64_t kBias = 0x8080808080808080ull;
     const double a = ldexp(x, -E);
-    if (a == 0.0) return;
+    off = 0;
+    if (a == 0.0) {           // every byte 0x80: all digits zero
+        U = kBias;
+        return;
+    }
     int ex;
     const double f = frexp(a, &ex);
     const int64_t M = (int64_t) ldexp(f, 53);        // exact signed 53-bit significand
     const int z = ex - 53 + 8 * s;
     int64_t X;
-    int off;
     if (z < 0) {
         X = (int64_t) rint(ldexp((double) M, z));     // < 2^53 in magnitude: exact
-        off = 0;
     } else {
         X = M << (z & 7);
         off = z >> 3;
     }
-#pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        const int64_t c = ((X + 128) & 255) - 128;
-        X = (X - c) >> 8;
-        const int tpos = s - 1 - (off + i);
-#pragma unroll
-        for (int t = 0; t < kMaxSlices; ++t)
-            if (t == tpos) dig[t] = (int) c;
-    }
+    U = (uint64_t) X + kBias;
 }
 
 // Tile of 64 lines x 64 k.  Output plane t: P[t * plane + line * Kp + kk].
@@ -153,24 +153,28 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(const double *__restri
     const int l = l0 + li;
     const int E = (l < lines && ex[l] != kExpEmpty) ? ex[l] + 2 : 0;
     const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
-    uint32_t packed[kMaxSlices][4];
+    uint64_t U[16];
+    int off[16];
 #pragma unroll
-    for (int t2 = 0; t2 < kMaxSlices; ++t2)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) packed[t2][w] = 0;
-    for (int j = 0; j < 16; ++j) {
-        int dig[kMaxSlices];
-        digits_of(zero_line ? 0.0 : tile[li][kq + j], E, s, dig);
-#pragma unroll
-        for (int t2 = 0; t2 < kMaxSlices; ++t2)
-            packed[t2][j >> 2] |= ((uint32_t) (dig[t2] & 0xff)) << (8 * (j & 3));
-    }
+    for (int j = 0; j < 16; ++j) window_of(zero_line ? 0.0 : tile[li][kq + j], E, s, U[j], off[j]);
     int8_t *base = P + (int64_t) l * Kp + k0 + kq;
 #pragma unroll
     for (int t2 = 0; t2 < kMaxSlices; ++t2) {
         if (t2 < s) {
-            uint4 v = make_uint4(packed[t2][0], packed[t2][1], packed[t2][2], packed[t2][3]);
-            *reinterpret_cast<uint4 *>(base + (int64_t) t2 * plane) = v;
+            u32x4 v;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int j = 4 * q + b;
+                    const int i = s - 1 - t2 - off[j];       // byte of U holding digit s-1-t2
+                    const uint32_t byte = (i >= 0 && i < 8) ? (((uint32_t) (U[j] >> (8 * i)) & 255u) ^ 0x80u) : 0u;
+                    w |= byte << (8 * b);
+                }
+                v[q] = w;
+            }
+            *reinterpret_cast<u32x4 *>(base + (int64_t) t2 * plane) = v;
         }
     }
 }
@@ -178,17 +182,22 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(const double *__restri
 // ---------------------------------------------------------------------
 // 3. one diagonal of slice products, FP64 recombination in the epilogue
 // ---------------------------------------------------------------------
-// A launch computes, for one diagonal d and one INT32-exact k-chunk, the
-// integer matrix D = sum_{t=0..d} QA_t QB_{d-t} over that chunk (a plain
-// INT8 GEMM over the concatenated slice pairs) and folds it into the FP64
-// accumulator W (m x n, ld Mp): W = D * 2^-8(d+2) on the first launch,
-// W + D * 2^-8(d+2) afterwards; the last launch writes
-// C = alpha * 2^eA 2^eB (W + ...) + beta * C instead.
+// A work item computes, for one diagonal d and one INT32-exact k-chunk c,
+// the integer tile D = sum_{t=0..d} QA_t QB_{d-t} over that chunk (a plain
+// INT8 GEMM over the concatenated slice pairs).  Two ways to add the items:
+//  * W mode (one launch per (d, c), many output tiles): the epilogue folds D
+//    into the FP64 accumulator W (m x n, ld Mp): W = D * 2^-8(d+2) on the
+//    first launch, W + D * 2^-8(d+2) afterwards; the last launch writes
+//    C = alpha * 2^eA 2^eB (W + ...) + beta * C instead;
+//  * split mode (few output tiles, long k): one launch runs every (tile, d, c)
+//    item and stores D as INT32 partials P32[d][c]; ozaki_fold_kernel then
+//    adds them per element in the same order (d = s-1 .. 0, c ascending).
+// Both give the same bits.
 constexpr int kBMo = 256, kBNo = 128;   // workgroup tile (4 waves of 128 x 64)
 
 struct OzArgs {
     int M, N, Mp, Np, Kp;
-    int d, kb0, nkb;        // diagonal, first k-block of the chunk, k-blocks in the chunk
+    int d, kb0, nkb;        // W mode: diagonal, first k-block of the chunk, k-blocks in the chunk
     int first, last;
     double alpha, beta;
     const int8_t *PA;       // planes of Mp x Kp
@@ -198,6 +207,12 @@ struct OzArgs {
     double *W;              // FP64 accumulator, ld Mp
     double *C;
     int64_t ldc;
+    // split mode
+    int s, nchunks, kcb;    // slices, chunks per diagonal, k-blocks per chunk
+    int ntiles, upper;      // output tiles per (d, c) item group; upper-triangle tiles only
+    int ngroups;            // s * nchunks
+    int *P32;               // INT32 partials, one Mp x Np plane per (d, c)
+    int64_t p32plane;
 };
 
 // 16-byte chunk swizzle within a 64-byte LDS row: rows r..r+15 read by one
@@ -205,12 +220,12 @@ struct OzArgs {
 __device__ __forceinline__ int lds_off(int row, int chunk) { return row * 64 + ((chunk ^ ((row >> 2) & 3)) << 4); }
 
 __device__ __forceinline__ void oz_gload(const OzArgs &g, const int8_t *gA, const int8_t *gB, int64_t row64,
-                                         int z, u32x4 (&ra)[4], u32x4 (&rb)[2])
+                                         int d, int kb0, int nkb, int z, u32x4 (&ra)[4], u32x4 (&rb)[2])
 {
-    const int t = z / g.nkb, b = z - t * g.nkb;
-    const int64_t koff = (int64_t) (g.kb0 + b) * kKb;
+    const int t = z / nkb, b = z - t * nkb;
+    const int64_t koff = (int64_t) (kb0 + b) * kKb;
     const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
-    const int8_t *pb = gB + (int64_t) (g.d - t) * g.planeB + koff;
+    const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
 #pragma unroll
     for (int p = 0; p < 4; ++p) ra[p] = *reinterpret_cast<const u32x4 *>(pa + p * row64);
 #pragma unroll
@@ -240,6 +255,7 @@ __device__ __forceinline__ void oz_compute(const int8_t *As, const int8_t *Bs, i
     }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[2][kBMo * kKb];
@@ -248,20 +264,53 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int tiles_m = g.Mp / kBMo, tiles_n = g.Np / kBNo;
-    int bid = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    const int G = 4, per = G * tiles_n, grp = bid / per, first = grp * G;
-    const int gsz = min(tiles_m - first, G), rr = bid - grp * per;
-    const int tm = first + rr % gsz, tn = rr / gsz;
+    int tm, tn, d, kb0, nkb;
+    int64_t part_off = 0;
+    if (!SPLIT) {
+        int bid = blockIdx.x;
+        const int nb = gridDim.x;
+        if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+        const int G = 4, per = G * tiles_n, grp = bid / per, first = grp * G;
+        const int gsz = min(tiles_m - first, G), rr = bid - grp * per;
+        tm = first + rr % gsz;
+        tn = rr / gsz;
+        d = g.d;
+        kb0 = g.kb0;
+        nkb = g.nkb;
+    } else {
+        // Workgroups are dealt to the 8 XCDs round robin: slot j of XCD x
+        // takes item group (j / ntiles) * 8 + x and output tile j % ntiles,
+        // so the tiles of one (d, c) group share that XCD's L2 and the
+        // groups (heaviest diagonal first) spread evenly over the XCDs.
+        const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+        const int jq = j / g.ntiles;
+        const int grp = jq * 8 + x;
+        if (grp >= g.ngroups) return;
+        int tile = j - jq * g.ntiles;
+        d = g.s - 1 - grp / g.nchunks;
+        const int c = grp - (grp / g.nchunks) * g.nchunks;
+        kb0 = c * g.kcb;
+        nkb = min(g.kcb, g.Kp / kKb - kb0);
+        part_off = (int64_t) (d * g.nchunks + c) * g.p32plane;
+        if (g.upper) {        // row tm of tiles holds tn = 2 tm .. tiles_n - 1 (tiles touching j >= i)
+            tm = 0;
+            while (tm < tiles_m - 1 && tile >= tiles_n - 2 * tm) {
+                tile -= tiles_n - 2 * tm;
+                ++tm;
+            }
+            tn = 2 * tm + tile;
+        } else {
+            tm = tile / tiles_n;
+            tn = tile - tm * tiles_n;
+        }
+    }
     const int m0 = tm * kBMo, n0 = tn * kBNo;
 
     const int lrow = tid >> 2, lchunk = tid & 3;
     const int8_t *gA = g.PA + (int64_t) (m0 + lrow) * g.Kp + lchunk * 16;
     const int8_t *gB = g.PB + (int64_t) (n0 + lrow) * g.Kp + lchunk * 16;
     const int64_t row64 = (int64_t) 64 * g.Kp;
-    const int d = g.d;
-    const int total = (d + 1) * g.nkb;        // stages: slice pair t, k-block b
+    const int total = (d + 1) * nkb;          // stages: slice pair t, k-block b
 
     i32x4 acc[8][4];
 #pragma unroll
@@ -276,17 +325,17 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
     // Loads past the last stage re-read the last stage (in bounds) and the
     // stores they feed are never consumed; this keeps the register sets
     // free of control flow so they stay in VGPRs.
-    oz_gload(g, gA, gB, row64, 0, ra0, rb0);
-    oz_gload(g, gA, gB, row64, min(1, total - 1), ra1, rb1);
+    oz_gload(g, gA, gB, row64, d, kb0, nkb, 0, ra0, rb0);
+    oz_gload(g, gA, gB, row64, d, kb0, nkb, min(1, total - 1), ra1, rb1);
     oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
     __syncthreads();
     for (int z = 0; z < total; z += 2) {
-        oz_gload(g, gA, gB, row64, min(z + 2, total - 1), ra0, rb0);
+        oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 2, total - 1), ra0, rb0);
         oz_compute(As[0], Bs[0], wm, wn, li, kq, acc);
         oz_sstore(As[1], Bs[1], lrow, lchunk, ra1, rb1);
         __syncthreads();
         if (z + 1 < total) {
-            oz_gload(g, gA, gB, row64, min(z + 3, total - 1), ra1, rb1);
+            oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 3, total - 1), ra1, rb1);
             oz_compute(As[1], Bs[1], wm, wn, li, kq, acc);
             oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
             __syncthreads();
@@ -295,6 +344,21 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
 
     // epilogue: lane l, register r of block (i, j) holds D[m][n] with
     // m = l % 16, n = 4 * (l / 16) + r (operands fed as (B, A))
+    if (SPLIT) {              // the padded Mp x Np plane is written whole
+        int *P = g.P32 + part_off;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = m0 + wm * 128 + i * 16 + li;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = n0 + wn * 64 + j * 16 + 4 * kq + r;
+                    P[(int64_t) col * g.Mp + row] = acc[i][j][r];
+                }
+        }
+        return;
+    }
     const double w = ldexp(1.0, -8 * (d + 2));
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -323,6 +387,32 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
             }
         }
     }
+}
+
+// Split mode, step 2: per element, the INT32 partials added in FP64 in the
+// order d = s-1 .. 0, chunks ascending, then scaled and written like the W
+// mode's last launch.  upper: only j >= i; mirror: C(j, i) := C(i, j).
+__global__ __launch_bounds__(256) void ozaki_fold_kernel(const int *__restrict__ P32, int64_t plane, int s,
+                                                          int nchunks, int Mp, int M, int N,
+                                                          const int *__restrict__ eA, const int *__restrict__ eB,
+                                                          double alpha, double beta, double *C, int64_t ldc,
+                                                          int upper, int mirror)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= M || j >= N || (upper && i > j)) return;
+    const int *p = P32 + (int64_t) j * Mp + i;
+    double acc = 0.0;
+    for (int d = s - 1; d >= 0; --d) {
+        const double w = ldexp(1.0, -8 * (d + 2));
+        for (int c = 0; c < nchunks; ++c) acc = __dadd_rn(acc, (double) p[(int64_t) (d * nchunks + c) * plane] * w);
+    }
+    const int ea = eA[i], eb = eB[j];
+    double v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(acc, ea + 2), eb + 2) : 0.0;
+    double *cp = C + (int64_t) j * ldc + i;
+    v = __dmul_rn(alpha, v);
+    if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *cp));
+    *cp = v;
+    if (mirror && i != j) C[(int64_t) i * ldc + j] = v;
 }
 
 // Library-owned, grow-only workspace for the planes and exponents.
@@ -355,7 +445,115 @@ void *workspace(size_t bytes, int *rc)
     return w.p;
 }
 
-int round_up(int x, int m) { return (x + m - 1) / m * m; }
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Sizes and mode of one product.  gram: op(B) = op(A)^T shares A's planes.
+struct OzPlan {
+    int Mp, Np, Kp, kchunk, nchunks, ntiles;
+    bool split, upper;
+    size_t planesA, planesB, exps, acc, total;
+};
+
+OzPlan oz_plan(int m, int n, int k, int s, bool gram)
+{
+    OzPlan p;
+    p.Mp = (int) round_up(m, kTile);
+    p.Np = gram ? p.Mp : (int) round_up(n, kTile);
+    p.Kp = (int) round_up(std::max(k, 1), kKb);
+    // INT32 exactness of one diagonal: s * kchunk * 128 * 128 < 2^31
+    p.kchunk = std::min(p.Kp, (int) (((1u << 31) - 1) / (16384u * (unsigned) s)) / kKb * kKb);
+    p.nchunks = (p.Kp + p.kchunk - 1) / p.kchunk;
+    p.upper = gram;
+    const int tiles_m = p.Mp / kBMo, tiles_n = p.Np / kBNo;
+    p.ntiles = tiles_m * tiles_n;
+    if (gram) {
+        p.ntiles = 0;
+        for (int tm = 0; tm < tiles_m; ++tm) p.ntiles += std::max(0, tiles_n - 2 * tm);
+    }
+    // split-K when the output tiles alone cannot fill the chip;
+    // CA3DMM_OZAKI_SPLIT=0/1 forces the mode of a non-Gram product
+    p.split = gram || (p.nchunks > 1 && p.ntiles < ca3::cu_count());
+    if (!gram) {
+        const char *f = getenv("CA3DMM_OZAKI_SPLIT");
+        if (f && (f[0] == '0' || f[0] == '1')) p.split = (f[0] == '1');
+    }
+    p.planesA = (size_t) s * p.Mp * p.Kp;
+    p.planesB = gram ? 0 : (size_t) s * p.Np * p.Kp;
+    p.exps = sizeof(int) * (size_t) (p.Mp + p.Np);
+    p.acc = p.split ? sizeof(int) * (size_t) s * p.nchunks * p.Mp * p.Np : sizeof(double) * (size_t) p.Mp * p.Np;
+    p.total = p.planesA + p.planesB + p.exps + 256 + p.acc;
+    return p;
+}
+
+// C := alpha op(A) op(B) + beta C; gram: op(A) = A^T (A is k x m), op(B) = A,
+// only j >= i computed (mirror copies it to j < i).
+int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *A, int ldA, const double *B,
+               int ldB, double beta, double *C, int ldC, int s, bool gram, bool mirror)
+{
+    const OzPlan pl = oz_plan(m, n, k, s, gram);
+    const int Mp = pl.Mp, Np = pl.Np, Kp = pl.Kp;
+    int rc = 0;
+    char *ws = (char *) workspace(pl.total, &rc);
+    if (rc) return rc;
+    int8_t *PA = (int8_t *) ws;
+    int8_t *PB = gram ? PA : PA + pl.planesA;
+    int *eA = (int *) (ws + pl.planesA + pl.planesB);
+    int *eB = gram ? eA : eA + Mp;
+    void *accp = (void *) (((uintptr_t) (ws + pl.planesA + pl.planesB + pl.exps) + 255) & ~(uintptr_t) 255);
+    hipStream_t st = ca3::stream();
+    fill_int_kernel<<<(Mp + Np + 255) / 256, 256, 0, st>>>(eA, gram ? Mp : Mp + Np, kExpEmpty);
+    // op(A): line i, element (i, l): A[i + l*ldA] (no trans) or A[l + i*ldA]
+    const int64_t a_sl = ta ? ldA : 1, a_sk = ta ? 1 : ldA;
+    // op(B): line j, element (l, j): B[l + j*ldB] (no trans) or B[j + l*ldB]
+    const int64_t b_sl = tb ? 1 : ldB, b_sk = tb ? ldB : 1;
+    const int ky = std::min(64, std::max(1, k / 512));
+    if (k > 0) {
+        ozaki_exp_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA);
+        if (!gram) ozaki_exp_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB);
+    }
+    ozaki_slice_kernel<<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA, s, PA,
+                                                               (int64_t) Mp * Kp, Kp);
+    if (!gram)
+        ozaki_slice_kernel<<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB, s, PB,
+                                                                   (int64_t) Np * Kp, Kp);
+    OzArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = m; g.N = n; g.Mp = Mp; g.Np = Np; g.Kp = Kp;
+    g.alpha = alpha; g.beta = beta;
+    g.PA = PA; g.PB = PB;
+    g.planeA = (int64_t) Mp * Kp; g.planeB = (int64_t) Np * Kp;
+    g.eA = eA; g.eB = eB;
+    g.C = C; g.ldc = ldC;
+    if (pl.split) {
+        g.s = s;
+        g.nchunks = pl.nchunks;
+        g.kcb = pl.kchunk / kKb;
+        g.ntiles = pl.ntiles;
+        g.upper = pl.upper ? 1 : 0;
+        g.ngroups = s * pl.nchunks;
+        g.P32 = (int *) accp;
+        g.p32plane = (int64_t) Mp * Np;
+        const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
+        ozaki_diag_kernel<true><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
+        ozaki_fold_kernel<<<dim3((m + 255) / 256, n), 256, 0, st>>>(g.P32, g.p32plane, s, pl.nchunks, Mp, m, n,
+                                                                     eA, eB, alpha, beta, C, ldC,
+                                                                     pl.upper ? 1 : 0, mirror ? 1 : 0);
+        return ca3::finish();
+    }
+    g.W = (double *) accp;
+    const int grid = (Mp / kBMo) * (Np / kBNo);
+    for (int d = s - 1; d >= 0; --d) {
+        for (int c = 0; c < pl.nchunks; ++c) {
+            g.d = d;
+            g.kb0 = c * (pl.kchunk / kKb);
+            g.nkb = std::min(pl.kchunk, Kp - c * pl.kchunk) / kKb;
+            g.first = (d == s - 1 && c == 0);
+            g.last = (d == 0 && c == pl.nchunks - 1);
+            ozaki_diag_kernel<false><<<grid, 256, 0, st>>>(g);
+        }
+    }
+    return ca3::finish();
+}
 
 }  // namespace
 
@@ -364,8 +562,7 @@ extern "C" {
 size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices)
 {
     if (m <= 0 || n <= 0 || k <= 0 || nslices < 1 || nslices > kMaxSlices) return 0;
-    const size_t Mp = round_up(m, kTile), Np = round_up(n, kTile), Kp = round_up(k, kKb);
-    return (size_t) nslices * (Mp + Np) * Kp + sizeof(int) * (Mp + Np) + sizeof(double) * Mp * Np + 512;
+    return oz_plan(m, n, k, nslices, false).total;
 }
 
 int hip_ozaki_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n,
@@ -382,55 +579,19 @@ int hip_ozaki_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int 
         if (!A || ldA < std::max(1, ta ? k : m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: bad A / ldA");
         if (!B || ldB < std::max(1, tb ? n : k)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki: bad B / ldB");
     }
-    const int s = nslices;
-    const int Mp = round_up(m, kTile), Np = round_up(n, kTile), Kp = round_up(std::max(k, 1), kKb);
-    int rc = 0;
-    const size_t bytes = hip_ozaki_workspace_bytes(m, n, std::max(k, 1), s);
-    char *ws = (char *) workspace(bytes, &rc);
-    if (rc) return rc;
-    int8_t *PA = (int8_t *) ws;
-    int8_t *PB = PA + (size_t) s * Mp * Kp;
-    int *eA = (int *) (PB + (size_t) s * Np * Kp);
-    int *eB = eA + Mp;
-    hipStream_t st = ca3::stream();
-    fill_int_kernel<<<(Mp + Np + 255) / 256, 256, 0, st>>>(eA, Mp + Np, kExpEmpty);
-    // op(A): line i, element (i, l): A[i + l*ldA] (no trans) or A[l + i*ldA]
-    const int64_t a_sl = ta ? ldA : 1, a_sk = ta ? 1 : ldA;
-    // op(B): line j, element (l, j): B[l + j*ldB] (no trans) or B[j + l*ldB]
-    const int64_t b_sl = tb ? 1 : ldB, b_sk = tb ? ldB : 1;
-    if (k > 0) {
-        const int ky = std::min(64, std::max(1, k / 512));
-        ozaki_exp_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA);
-        ozaki_exp_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB);
-    }
-    ozaki_slice_kernel<<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA, s, PA,
-                                                               (int64_t) Mp * Kp, Kp);
-    ozaki_slice_kernel<<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB, s, PB,
-                                                               (int64_t) Np * Kp, Kp);
-    double *W = (double *) (((uintptr_t) (eB + Np) + 255) & ~(uintptr_t) 255);
-    OzArgs g;
-    memset(&g, 0, sizeof(g));
-    g.M = m; g.N = n; g.Mp = Mp; g.Np = Np; g.Kp = Kp;
-    g.alpha = alpha; g.beta = beta;
-    g.PA = PA; g.PB = PB;
-    g.planeA = (int64_t) Mp * Kp; g.planeB = (int64_t) Np * Kp;
-    g.eA = eA; g.eB = eB;
-    g.W = W; g.C = C; g.ldc = ldC;
-    // INT32 exactness of one diagonal: (d+1) * kchunk * 128 * 128 < 2^31
-    const int kchunk = std::min(Kp, (int) (((1u << 31) - 1) / (16384u * (unsigned) s)) / kKb * kKb);
-    const int nchunks = (Kp + kchunk - 1) / kchunk;
-    const int grid = (Mp / kBMo) * (Np / kBNo);
-    for (int d = s - 1; d >= 0; --d) {
-        for (int c = 0; c < nchunks; ++c) {
-            g.d = d;
-            g.kb0 = c * (kchunk / kKb);
-            g.nkb = std::min(kchunk, Kp - c * kchunk) / kKb;
-            g.first = (d == s - 1 && c == 0);
-            g.last = (d == 0 && c == nchunks - 1);
-            ozaki_diag_kernel<<<grid, 256, 0, st>>>(g);
-        }
-    }
-    return ca3::finish();
+    return ozaki_core(ta, tb, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, nslices, false, false);
+}
+
+int hip_ozaki_gram(const int n, const int k, const double alpha, const double *A, const int ldA,
+                   const double beta, double *C, const int ldC, int mirror, int nslices)
+{
+    if (n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki gram: negative size");
+    if (n == 0) return HIP_PROXY_OK;
+    if (nslices < 1 || nslices > kMaxSlices)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki gram: nslices %d outside 1..%d", nslices, kMaxSlices);
+    if (!C || ldC < std::max(1, n)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki gram: bad C / ldC");
+    if (k > 0 && (!A || ldA < std::max(1, k))) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki gram: bad A / ldA");
+    return ozaki_core(1, 0, n, n, k, alpha, A, ldA, A, ldA, beta, C, ldC, nslices, true, mirror != 0);
 }
 
 }  // extern "C"

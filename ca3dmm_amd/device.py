@@ -59,6 +59,20 @@ class DeviceArray:
             check(lib.hip_memcpy_d2h(self.ptr, host.ctypes.data, self.nbytes), "hip_memcpy_d2h")
         return host
 
+    def sample(self, ii, jj):
+        """Entries (ii[t], jj[t]) of a 2-D array, read one by one (for arrays
+        too large to copy back whole)."""
+        assert len(self.shape) == 2
+        ld = self.shape[0]
+        out = np.empty(len(ii), dtype=self.dtype)
+        check(lib.hip_stream_sync(None), "hip_stream_sync")
+        for t, (i, j) in enumerate(zip(ii, jj)):
+            if not (0 <= i < self.shape[0] and 0 <= j < self.shape[1]):
+                raise IndexError((i, j))
+            check(lib.hip_memcpy_d2h(self.at(int(i) + int(j) * ld), out[t:].ctypes.data,
+                                     self.dtype.itemsize), "hip_memcpy_d2h")
+        return out
+
     def at(self, offset_elems):
         """Raw device address of element `offset_elems` (flat, column-major)."""
         return self.ptr + int(offset_elems) * self.dtype.itemsize
@@ -150,6 +164,11 @@ def transpose_cm(nrow, ncol, A, AT):
 def gram(n, k, A, Cm, alpha=1.0, beta=0.0, mirror=True, lda=None, ldc=None):
     return check(lib.hip_dsyrk_gram(n, k, alpha, A.ptr, lda or _ld(A), beta, Cm.ptr, ldc or _ld(Cm),
                                     1 if mirror else 0), "hip_dsyrk_gram")
+
+
+def ozaki_gram(n, k, A, Cm, nslices, alpha=1.0, beta=0.0, mirror=True, lda=None, ldc=None):
+    return check(lib.hip_ozaki_gram(n, k, alpha, A.ptr, lda or _ld(A), beta, Cm.ptr, ldc or _ld(Cm),
+                                    1 if mirror else 0, nslices), "hip_ozaki_gram")
 
 
 def potrf_upper(n, A, lda=None):

@@ -68,6 +68,16 @@ int hip_ozaki_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
 /* Device workspace the library keeps for hip_ozaki_dgemm at this size. */
 size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices);
 
+/* hip_dsyrk_gram (below) through the Ozaki scheme: C := alpha * A^T A +
+ * beta * C for A k x n (column-major), upper triangle only, mirror != 0
+ * copies it to the lower triangle.  A is sliced once and serves as both
+ * operands; k is split across workgroups and the INT32 partials are added
+ * in FP64 in the fixed order of hip_ozaki_dgemm, so the result equals
+ * hip_ozaki_dgemm(T, N, n, n, k, ...) on (A, A) bit for bit on j >= i.
+ * No reference counterpart (the Gram of examples/example_CholeskyQR.c:109-117). */
+int hip_ozaki_gram(const int n, const int k, const double alpha, const double *A, const int ldA,
+                   const double beta, double *C, const int ldC, int mirror, int nslices);
+
 /* C := alpha * A^T A + beta * C for A k x n (column-major): the B^T B of
  * ca3dmm_engine_init_BTB (src/ca3dmm.c:1172-1265) and of CholeskyQR
  * (examples/example_CholeskyQR.c:109-117).  Only the upper triangle is

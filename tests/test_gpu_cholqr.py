@@ -71,11 +71,12 @@ def test_trtri_and_trsm(hip):
     assert np.allclose(dB.numpy(), want, rtol=1e-11, atol=1e-12)
 
 
+@pytest.mark.parametrize("gram_slices", [None, 8])
 @pytest.mark.parametrize("cond", [1e2, 1e5])
-def test_cholesky_qr2(hip, cond):
+def test_cholesky_qr2(hip, cond, gram_slices):
     m, n = 40000, 128
     dA, _ = cholqr.test_matrix(m, n, cond, seed=11)
-    dQ, dR = cholqr.cholesky_qr2(dA, m, n)
+    dQ, dR = cholqr.cholesky_qr2(dA, m, n, gram_slices=gram_slices)
     orth, res = cholqr.qr_errors(dA, dQ, dR, m, n)
     assert orth < 1e-13, orth
     assert res < 1e-14, res

@@ -40,8 +40,12 @@ def test_ozaki_alpha_beta(hip):
     assert np.array_equal(dC.numpy(), want)
 
 
-def test_ozaki_k_chunks(hip):
-    """s = 16 limits an INT32-exact k-chunk to 8128: k = 20000 runs 3 chunks."""
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_ozaki_k_chunks(hip, monkeypatch, split):
+    """s = 16 limits an INT32-exact k-chunk to 8128: k = 20000 runs 3 chunks,
+    through the per-diagonal launches (split = 0) and the split-K launch with
+    INT32 partials and a separate fold (split = 1)."""
+    monkeypatch.setenv("CA3DMM_OZAKI_SPLIT", split)
     rnd = np.random.default_rng(6)
     m, n, k = 40, 50, 20000
     A, B = Z.phi_matrix(rnd, (m, k), 1.0), Z.phi_matrix(rnd, (k, n), 1.0)
@@ -50,6 +54,55 @@ def test_ozaki_k_chunks(hip):
     dC = hip.DeviceArray.zeros((m, n))
     hip.ozaki_dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, 16)
     assert np.array_equal(dC.numpy(), want)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k,s", [(1, 1, 1, 1), (200, 130, 300, 8), (300, 70, 12000, 12)])
+def test_ozaki_split_mode_bit_exact(hip, monkeypatch, ta, tb, m, n, k, s):
+    monkeypatch.setenv("CA3DMM_OZAKI_SPLIT", "1")
+    rnd = np.random.default_rng(m + n + k + s + ta)
+    opA, opB = Z.phi_matrix(rnd, (m, k), 2.0), Z.phi_matrix(rnd, (k, n), 2.0)
+    Cin = rnd.standard_normal((m, n))
+    want = Z.ozaki_dgemm8(opA, opB, s, alpha=1.5, beta=-0.25, Cin=Cin)
+    dA = hip.DeviceArray.from_numpy(opA.T if ta else opA)
+    dB = hip.DeviceArray.from_numpy(opB.T if tb else opB)
+    dC = hip.DeviceArray.from_numpy(Cin)
+    hip.ozaki_dgemm(ta, tb, m, n, k, 1.5, dA, dB, -0.25, dC, s)
+    assert np.array_equal(dC.numpy(), want)
+
+
+@pytest.mark.parametrize("n,k,s,mirror", [(1, 5, 3, True), (100, 20000, 12, True), (300, 1000, 8, False),
+                                          (513, 700, 8, True)])
+def test_ozaki_gram_bit_exact(hip, n, k, s, mirror):
+    """hip_ozaki_gram equals ozaki_dgemm8(A^T, A) on the upper triangle; the
+    lower triangle is its mirror, or C untouched when mirror = 0."""
+    rnd = np.random.default_rng(n + k + s)
+    A = Z.phi_matrix(rnd, (k, n), 1.0)
+    if n > 7:
+        A[:, 7] = 0.0
+    Cin = rnd.standard_normal((n, n))
+    want = Z.ozaki_dgemm8(np.ascontiguousarray(A.T), A, s, alpha=0.5, beta=2.0, Cin=Cin)
+    dA, dC = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(Cin)
+    hip.ozaki_gram(n, k, dA, dC, s, alpha=0.5, beta=2.0, mirror=mirror)
+    got = dC.numpy()
+    iu = np.triu_indices(n)
+    assert np.array_equal(got[iu], want[iu])
+    il = np.tril_indices(n, -1)
+    assert np.array_equal(got[il], (want.T if mirror else Cin)[il])
+
+
+def test_ozaki_gram_accuracy(hip):
+    """phi = 0.5, 256 columns, k = 100000, s = 8: max |G - G_dd| / (|A|^T |A|) below 1e-16."""
+    rnd = np.random.default_rng(8)
+    n, k = 256, 100000
+    A = Z.phi_matrix(rnd, (k, n), 0.5)
+    dA, dG = hip.DeviceArray.from_numpy(A), hip.DeviceArray.zeros((n, n))
+    hip.ozaki_gram(n, k, dA, dG, 8)
+    got = dG.numpy()
+    ii, jj = rnd.integers(0, n, 500), rnd.integers(0, n, 500)
+    Af = np.asfortranarray(A)
+    ref, ab = O.dd_sample(1, 0, n, n, k, Af, k, Af, k, ii, jj)
+    assert np.max(np.abs(got[ii, jj] - ref) / ab) < 1e-16
 
 
 def test_ozaki_accuracy_matches_fp64(hip):

@@ -78,3 +78,38 @@ def test_k_chunking_rule():
     assert Z.kchunk8(1, 3) == 64
     for s in range(1, 17):
         assert s * Z.kchunk8(10 ** 7, s) * 128 * 128 < 2 ** 31
+
+
+def _bias_window_digits(v, E, s):
+    """The device's digit rule (ozaki_slice_kernel): digit i of the balanced
+    expansion of X is byte i of U = X + 0x8080...80 (mod 2^64) xor 0x80.
+    Restated here with Python integers, independently of split8's carries."""
+    a = float(np.ldexp(v, -int(E)))
+    if a == 0.0:
+        return [0] * s
+    f, ex = np.frexp(a)
+    M = int(np.ldexp(f, 53))
+    z = int(ex) - 53 + 8 * s
+    if z < 0:
+        X, off = int(np.rint(np.ldexp(float(M), z))), 0
+    else:
+        X, off = M << (z % 8), z // 8
+    U = (X + 0x8080808080808080) % 2 ** 64
+    out = []
+    for t in range(s):
+        i = s - 1 - t - off
+        b = ((U >> (8 * i)) & 255) ^ 0x80 if 0 <= i < 8 else 0
+        out.append(b - 256 if b >= 128 else b)
+    return out
+
+
+@pytest.mark.parametrize("s", [1, 3, 8, 11, 16])
+def test_device_bias_window_rule_equals_split8(s):
+    rnd = np.random.default_rng(100 + s)
+    X = Z.phi_matrix(rnd, (10, 30), 4.0)
+    X[1, :8] = [0.24999999999999997, -0.2, 1e-300, 5e-324, -7.0, 3.0, -0.0, 2.0 ** -40]
+    S, E, zero = Z.split8(X, s, axis=1)
+    for (i, j), v in np.ndenumerate(X):
+        want = [int(S[t][i, j]) for t in range(s)]
+        got = [0] * s if zero[i] else _bias_window_digits(v, E[i], s)
+        assert got == want, (i, j, v)
