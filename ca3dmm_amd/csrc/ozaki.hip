@@ -4,9 +4,11 @@
 // keeps FP64 accuracy.  Specification (and the bit-exact oracle):
 // oracle/ozaki.py, functions split8 / ozaki_dgemm8.
 //
-//   1. ozaki_exp_kernel   per line (row of op(A), column of op(B)) the
-//                          frexp exponent of the largest |x| (atomicMax);
-//   2. ozaki_slice_kernel  x' = x * 2^-(e+2) (|x'| < 1/4) and the s balanced
+//   1. ozaki_max_kernel   per line (row of op(A), column of op(B)) the
+//                          largest |x| (atomicMax on its bits), then
+//      ozaki_scale_kernel  the line's scale exponent E: with f, e = frexp
+//                          of that max, E = e + 1 if f <= 253/256, else e + 2;
+//   2. ozaki_slice_kernel  x' = x * 2^-E (|x'| <= 253/512) and the s balanced
 //                          base-256 digits of rint(x' * 2^(8s)) as int8
 //                          planes, line-major with k contiguous, zero padded
 //                          to whole tiles; LDS-tiled so both source layouts
@@ -15,7 +17,7 @@
 //                          v_mfma_i32_16x16x64_i8 with exact INT32
 //                          accumulation, added in FP64 (x 2^-8(d+2)),
 //                          smallest diagonal first; the last launch rescales
-//                          by 2^eA_i 2^eB_j and applies alpha/beta.
+//                          by 2^EA_i 2^EB_j and applies alpha/beta.
 // Each diagonal (and INT32-exact k-chunk) is one launch of an INT8 GEMM
 // over the concatenated slice pairs: 256 x 128 tiles, 128 x 64 per wave,
 // LDS double buffer fed from two register sets (prefetch distance two
@@ -47,9 +49,11 @@ constexpr int kExpEmpty = INT_MIN;
 // ---------------------------------------------------------------------
 // 1. line exponents
 // ---------------------------------------------------------------------
-// Line l of the operand: element (l, kk) at X[l*sl + kk*sk].
-__global__ __launch_bounds__(256) void ozaki_exp_kernel(const double *__restrict__ X, int lines, int K,
-                                                         int64_t sl, int64_t sk, int *__restrict__ ex)
+// Line l of the operand: element (l, kk) at X[l*sl + kk*sk].  mx[l] holds
+// the bits of the largest |x| seen (non-negative doubles order like their
+// bit patterns, so an integer atomicMax is exact); 0 means an all-zero line.
+__global__ __launch_bounds__(256) void ozaki_max_kernel(const double *__restrict__ X, int lines, int K,
+                                                         int64_t sl, int64_t sk, unsigned long long *mx)
 {
     // block: 64 lines x (K / gridDim.y chunk); thread layout follows the
     // contiguous direction so loads coalesce
@@ -59,33 +63,43 @@ __global__ __launch_bounds__(256) void ozaki_exp_kernel(const double *__restrict
     const int k0 = blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
     const int l0 = blockIdx.x * 64;
     __shared__ double red[256];
-    double mx = 0.0;
-    int myline;
+    double m = 0.0;
     if (line_contig) {
-        myline = l0 + (t & 63);
+        const int myline = l0 + (t & 63);
         for (int kk = k0 + (t >> 6); kk < k1; kk += 4)
-            if (myline < lines) mx = fmax(mx, fabs(X[myline + (int64_t) kk * sk]));
+            if (myline < lines) m = fmax(m, fabs(X[myline + (int64_t) kk * sk]));
     } else {
-        myline = l0 + (t >> 2);
+        const int myline = l0 + (t >> 2);
         for (int kk = k0 + (t & 3); kk < k1; kk += 4)
-            if (myline < lines) mx = fmax(mx, fabs(X[(int64_t) myline * sl + kk]));
+            if (myline < lines) m = fmax(m, fabs(X[(int64_t) myline * sl + kk]));
     }
-    red[t] = mx;
+    red[t] = m;
     __syncthreads();
     if (t < 64) {
         const int l = l0 + t;
-        double m = 0.0;
+        double v = 0.0;
         if (line_contig) {
-            for (int j = 0; j < 4; ++j) m = fmax(m, red[t + 64 * j]);
+            for (int j = 0; j < 4; ++j) v = fmax(v, red[t + 64 * j]);
         } else {
-            for (int j = 0; j < 4; ++j) m = fmax(m, red[4 * t + j]);
+            for (int j = 0; j < 4; ++j) v = fmax(v, red[4 * t + j]);
         }
-        if (l < lines && m > 0.0) {
-            int e;
-            frexp(m, &e);
-            atomicMax(&ex[l], e);
-        }
+        if (l < lines && v > 0.0) atomicMax(&mx[l], (unsigned long long) __double_as_longlong(v));
     }
+}
+
+// E[l] = kExpEmpty for a zero line, else e + 1 or e + 2 as above.
+__global__ void ozaki_scale_kernel(const unsigned long long *__restrict__ mx, int n, int *__restrict__ E)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const double v = __longlong_as_double((long long) mx[l]);
+    if (v == 0.0) {
+        E[l] = kExpEmpty;
+        return;
+    }
+    int e;
+    const double f = frexp(v, &e);
+    E[l] = (f <= 253.0 / 256.0) ? e + 1 : e + 2;
 }
 
 __global__ void fill_int_kernel(int *p, int n, int v)
@@ -96,7 +110,7 @@ __global__ void fill_int_kernel(int *p, int n, int v)
 // ---------------------------------------------------------------------
 // 2. slices
 // ---------------------------------------------------------------------
-// Balanced base-256 digits of x * 2^-E, |.| < 1/4, as digits of the integer
+// Balanced base-256 digits of x * 2^-E, |.| <= 253/512, as digits of the integer
 // N = rint(x * 2^(8s - E)) (digit s-1-t goes to slice t, t = 0 the most
 // significant).  N = X * 256^off with X = M << (z mod 8) (M the 53-bit
 // significand, |X| < 2^61) or, when the scale drops bits, X = rint(M 2^z),
@@ -151,7 +165,7 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(const double *__restri
     __syncthreads();
     const int li = t >> 2, kq = (t & 3) * 16;
     const int l = l0 + li;
-    const int E = (l < lines && ex[l] != kExpEmpty) ? ex[l] + 2 : 0;
+    const int E = (l < lines && ex[l] != kExpEmpty) ? ex[l] : 0;
     const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
     uint64_t U[16];
     int off[16];
@@ -203,7 +217,7 @@ struct OzArgs {
     const int8_t *PA;       // planes of Mp x Kp
     const int8_t *PB;       // planes of Np x Kp
     int64_t planeA, planeB;
-    const int *eA, *eB;     // line exponents (kExpEmpty for zero lines)
+    const int *eA, *eB;     // line scale exponents E (kExpEmpty for zero lines)
     double *W;              // FP64 accumulator, ld Mp
     double *C;
     int64_t ldc;
@@ -379,7 +393,7 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
                     continue;
                 }
                 const int eb = g.eB[col];
-                v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea + 2), eb + 2) : 0.0;
+                v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
                 double *cp = g.C + (int64_t) col * g.ldc + row;
                 v = __dmul_rn(g.alpha, v);
                 if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *cp));
@@ -407,7 +421,7 @@ __global__ __launch_bounds__(256) void ozaki_fold_kernel(const int *__restrict__
         for (int c = 0; c < nchunks; ++c) acc = __dadd_rn(acc, (double) p[(int64_t) (d * nchunks + c) * plane] * w);
     }
     const int ea = eA[i], eb = eB[j];
-    double v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(acc, ea + 2), eb + 2) : 0.0;
+    double v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(acc, ea), eb) : 0.0;
     double *cp = C + (int64_t) j * ldc + i;
     v = __dmul_rn(alpha, v);
     if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *cp));
@@ -451,7 +465,7 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 struct OzPlan {
     int Mp, Np, Kp, kchunk, nchunks, ntiles;
     bool split, upper;
-    size_t planesA, planesB, exps, acc, total;
+    size_t planesA, planesB, exps, maxes, acc, total;
 };
 
 OzPlan oz_plan(int m, int n, int k, int s, bool gram)
@@ -480,8 +494,9 @@ OzPlan oz_plan(int m, int n, int k, int s, bool gram)
     p.planesA = (size_t) s * p.Mp * p.Kp;
     p.planesB = gram ? 0 : (size_t) s * p.Np * p.Kp;
     p.exps = sizeof(int) * (size_t) (p.Mp + p.Np);
+    p.maxes = sizeof(unsigned long long) * (size_t) (p.Mp + p.Np);
     p.acc = p.split ? sizeof(int) * (size_t) s * p.nchunks * p.Mp * p.Np : sizeof(double) * (size_t) p.Mp * p.Np;
-    p.total = p.planesA + p.planesB + p.exps + 256 + p.acc;
+    p.total = p.planesA + p.planesB + p.exps + p.maxes + 256 + p.acc;
     return p;
 }
 
@@ -499,18 +514,23 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
     int8_t *PB = gram ? PA : PA + pl.planesA;
     int *eA = (int *) (ws + pl.planesA + pl.planesB);
     int *eB = gram ? eA : eA + Mp;
-    void *accp = (void *) (((uintptr_t) (ws + pl.planesA + pl.planesB + pl.exps) + 255) & ~(uintptr_t) 255);
+    unsigned long long *mxA = (unsigned long long *) (ws + pl.planesA + pl.planesB + pl.exps);
+    unsigned long long *mxB = gram ? mxA : mxA + Mp;
+    void *accp = (void *) (((uintptr_t) (ws + pl.planesA + pl.planesB + pl.exps + pl.maxes) + 255) &
+                           ~(uintptr_t) 255);
     hipStream_t st = ca3::stream();
-    fill_int_kernel<<<(Mp + Np + 255) / 256, 256, 0, st>>>(eA, gram ? Mp : Mp + Np, kExpEmpty);
+    const int nl = gram ? Mp : Mp + Np;       // lines with a scale (padding lines stay empty)
+    fill_int_kernel<<<(2 * nl + 255) / 256, 256, 0, st>>>((int *) mxA, 2 * nl, 0);
     // op(A): line i, element (i, l): A[i + l*ldA] (no trans) or A[l + i*ldA]
     const int64_t a_sl = ta ? ldA : 1, a_sk = ta ? 1 : ldA;
     // op(B): line j, element (l, j): B[l + j*ldB] (no trans) or B[j + l*ldB]
     const int64_t b_sl = tb ? 1 : ldB, b_sk = tb ? ldB : 1;
     const int ky = std::min(64, std::max(1, k / 512));
     if (k > 0) {
-        ozaki_exp_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA);
-        if (!gram) ozaki_exp_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB);
+        ozaki_max_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, mxA);
+        if (!gram) ozaki_max_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, mxB);
     }
+    ozaki_scale_kernel<<<(nl + 255) / 256, 256, 0, st>>>(mxA, nl, eA);
     ozaki_slice_kernel<<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA, s, PA,
                                                                (int64_t) Mp * Kp, Kp);
     if (!gram)

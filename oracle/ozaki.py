@@ -97,20 +97,31 @@ def products(s):
 # 8-bit balanced digits.  Each slice carries a full int8 digit in [-128, 127]
 # of base 256.  Top-down rounding cannot produce them (a remainder of exactly
 # half a unit would need +128), so the digits are taken bottom-up from the
-# exact integer N = rint(a' * 2^(8s)), |a'| < 1/4: N = M * 2^z with M the
+# exact integer N = rint(a' * 2^(8s)), |a'| <= 253/512: N = M * 2^z with M the
 # 53-bit significand, so only a window of at most 9 digits is non-zero; the
 # window X = M << (z mod 8) (<= 61 bits) is converted to balanced digits with
 # carries, c = ((X + 128) mod 256) - 128, X = (X - c) / 256.
 # ---------------------------------------------------------------------------
 DIGIT_BITS = 8
+TOP_MANT = 253.0 / 256.0
+
+
+def line_scale8(X, axis):
+    """Per-line scale exponent of the 8-bit scheme.  With f, e = frexp(max |x|)
+    of the line: E = e + 1 when f <= 253/256, else E = e + 2.  Then every
+    scaled entry a' = x 2^-E has |a'| <= 253/512, so N = rint(a' 2^(8s)) has
+    |N| / 256^(s-1) <= 126.5 and its top balanced digit stays in [-128, 127]
+    (the lower digits add at most 128/255 in magnitude).  Returns (E, zero mask)."""
+    mx = np.max(np.abs(X), axis=axis)
+    f, e = np.frexp(mx)
+    E = np.where(f <= TOP_MANT, e + 1, e + 2).astype(np.int64)
+    return E, mx == 0
 
 
 def split8(X, s, axis):
-    """Balanced base-256 slices of X scaled per line along `axis`.
-    Scale exponent E = e + 2 (|a'| < 1/4 so the top digit never carries out).
-    Returns (list of s int8 arrays, E, zero mask)."""
-    e, zero = scale_exponents(X, axis)
-    E = e + 2
+    """Balanced base-256 slices of X scaled per line along `axis` by 2^-E
+    (line_scale8).  Returns (list of s int8 arrays, E, zero mask)."""
+    E, zero = line_scale8(X, axis)
     shp = (-1, 1) if axis == 1 else (1, -1)
     a = np.ldexp(X, -E.reshape(shp))
     a = np.where(zero.reshape(shp), 0.0, a)

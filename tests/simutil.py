@@ -19,7 +19,9 @@ def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False, 
             graph=False, autotune=False):
     """Run the simulated p-rank grid with the example_AB 1D column layout.
     A, B: full matrices as stored (numpy, any order).  Returns the full C
-    gathered from the ranks' dst_C blocks (and the SimGrid when keep)."""
+    gathered from the ranks' dst_C blocks.  keep: also returns the SimGrid,
+    the plan and (Ap, ldA, Bp, ldB, Cp, ldC, dC, dA, dB); the raw addresses
+    stay valid only while that tuple (which owns dA, dB, dC) is alive."""
     w = O.ca3dmm_world(m, n, k, ta, tb, p, proc_grid=proc_grid, is_btb=is_btb)
     mm = n if is_btb else m
     dA = hip.DeviceArray.from_numpy(A) if not is_btb else None
@@ -40,7 +42,7 @@ def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False, 
     sim.exec(Ap, ldA, Bp, ldB, Cp, ldC)
     out = dC.numpy()
     if keep:
-        return out, sim, w, (Ap, ldA, Bp, ldB, Cp, ldC, dC)
+        return out, sim, w, (Ap, ldA, Bp, ldB, Cp, ldC, dC, dA, dB)
     sim.free()
     return out
 

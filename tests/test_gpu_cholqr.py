@@ -54,10 +54,16 @@ def test_potrf_matches_scipy_and_reports_info(hip):
     assert hip.potrf_upper(6, dB) == 3
 
 
+def _chol_factor(rnd, n):
+    """Upper R of a well-conditioned SPD matrix (kappa(R) of a few units)."""
+    X = rnd.standard_normal((4 * n, n))
+    return sla.cholesky(X.T @ X / (4 * n) + 0.5 * np.eye(n), lower=False)
+
+
 def test_trtri_and_trsm(hip):
     rnd = np.random.default_rng(5)
     n, m = 257, 5000
-    R = np.triu(rnd.standard_normal((n, n))) + 4.0 * np.eye(n)
+    R = _chol_factor(rnd, n)
     dR = hip.DeviceArray.from_numpy(R)
     dX = hip.DeviceArray.filled((n, n), 0xFF)
     hip.trtri_upper(n, dR, dX)
@@ -69,6 +75,18 @@ def test_trtri_and_trsm(hip):
     hip.trsm_right_upper(m, n, 2.0, dR, dB)
     want = 2.0 * sla.solve_triangular(R, B.T, trans="T", lower=False).T
     assert np.allclose(dB.numpy(), want, rtol=1e-11, atol=1e-12)
+
+
+def test_trtri_ill_conditioned_within_kappa_bound(hip):
+    """triu(N(0,1)) + 4 I at n = 257 has kappa ~ 7e5: the inverse is held to
+    ||R X - I||_F <= n * u * kappa(R) * 10 (scipy's own inverse sits at ~6e-11)."""
+    rnd = np.random.default_rng(5)
+    n = 257
+    R = np.triu(rnd.standard_normal((n, n))) + 4.0 * np.eye(n)
+    dR, dX = hip.DeviceArray.from_numpy(R), hip.DeviceArray.zeros((n, n))
+    hip.trtri_upper(n, dR, dX)
+    bound = 10 * n * 2.0 ** -53 * np.linalg.cond(R)
+    assert np.linalg.norm(R @ dX.numpy() - np.eye(n)) <= bound
 
 
 @pytest.mark.parametrize("gram_slices", [None, 8])

@@ -52,7 +52,7 @@ def test_grouped_graph_replays(hip):
     eager = run_sim(24, m, n, k, 0, 0, A, B, proc_grid=[4, 2, 3], grouped=True, graph=False)
     got, sim, w, args = run_sim(24, m, n, k, 0, 0, A, B, proc_grid=[4, 2, 3], grouped=True, graph=True,
                                 keep=True)
-    Ap, ldA, Bp, ldB, Cp, ldC, dC = args
+    Ap, ldA, Bp, ldB, Cp, ldC, dC, dA, dB = args     # dA, dB own the memory Ap, Bp point into
     assert np.array_equal(got, eager)
     junk = dev.DeviceArray.zeros((1000, 1000))
     for rep in range(4):

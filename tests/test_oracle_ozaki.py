@@ -21,14 +21,19 @@ def _rint_exact(v, s):
 @pytest.mark.parametrize("s", [1, 2, 5, 8, 9, 12, 16])
 def test_digits_are_exact_balanced_expansion(s):
     rnd = np.random.default_rng(s)
-    X = Z.phi_matrix(rnd, (12, 40), 4.0)
+    X = Z.phi_matrix(rnd, (14, 40), 4.0)
     X[2] = 0.0
     X[4, :6] = [0.24999999999999997, -0.2, 1e-300, 5e-324, -7.0, 3.0]
+    # lines whose max sits at the threshold of the scale rule, on both sides,
+    # and lines of values whose N lands on the top digit's limits
+    X[5, :3] = [253 / 256, -0.5, 0.25]
+    X[6, :3] = [-np.nextafter(253 / 256, 1), 0.5, 1e-3]
+    X[7, :4] = [-(253 / 256), 253 / 256, -np.nextafter(253 / 256, 0), 0.9999999999999999]
     S, E, zero = Z.split8(X, s, axis=1)
     assert zero[2] and not zero[4]
     a = np.ldexp(X, -E.reshape(-1, 1))
     a[zero] = 0
-    assert np.all(np.abs(a) < 0.25)
+    assert np.all(np.abs(a) <= 253 / 512)
     for (i, j), v in np.ndenumerate(a):
         digits = [int(S[t][i, j]) for t in range(s)]
         assert all(-128 <= d <= 127 for d in digits)
@@ -113,3 +118,18 @@ def test_device_bias_window_rule_equals_split8(s):
         want = [int(S[t][i, j]) for t in range(s)]
         got = [0] * s if zero[i] else _bias_window_digits(v, E[i], s)
         assert got == want, (i, j, v)
+
+
+def test_scale_rule_threshold():
+    """E = e + 1 up to a line-max mantissa of 253/256, e + 2 above it; the
+    digits of lines at the threshold stay within int8."""
+    X = np.array([[253 / 256, 0.1], [np.nextafter(253 / 256, 1), 0.1], [0.5, -0.5], [-1.0, 0.0],
+                  [3.0, 1.0], [0.0, 0.0]])
+    E, zero = Z.line_scale8(X, 1)
+    assert list(E[:5]) == [1, 2, 1, 2, 3] and list(zero) == [False] * 5 + [True]
+    # a' = 253/512 exactly: N = 126.5 * 256^(s-1), i.e. 126 (ties to even)
+    # for s = 1 and the balanced 127 * 256 - 128 for s >= 2
+    for s, want in ((1, [126]), (2, [127, -128]), (3, [127, -128, 0])):
+        S, E2, _ = Z.split8(X, s, 1)
+        assert np.array_equal(E2, E)
+        assert [int(q[0, 0]) for q in S] == want
