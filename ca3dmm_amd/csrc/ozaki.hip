@@ -207,7 +207,10 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(const double *__restri
 //    item and stores D as INT32 partials P32[d][c]; ozaki_fold_kernel then
 //    adds them per element in the same order (d = s-1 .. 0, c ascending).
 // Both give the same bits.
-constexpr int kBMo = 256, kBNo = 128;   // workgroup tile (4 waves of 128 x 64)
+// Workgroup tiles: 2 x 2 waves of WTM x WTN.  The default (WTM = WTN = 128,
+// 256 x 256 per workgroup, one workgroup per CU) keeps the 256 INT32
+// accumulators of a wave in the AGPR half of the register file; 128 x 64
+// (256 x 128 per workgroup, two per CU) is the smaller variant.
 
 struct OzArgs {
     int M, N, Mp, Np, Kp;
@@ -233,45 +236,55 @@ struct OzArgs {
 // MFMA operand land on 16 distinct 16-byte bank groups.
 __device__ __forceinline__ int lds_off(int row, int chunk) { return row * 64 + ((chunk ^ ((row >> 2) & 3)) << 4); }
 
+// One pipeline stage: PA (PB) passes of 64 rows of A (B), 16 bytes of k per
+// thread and pass (thread t: row t / 4 of the pass, 16-byte chunk t % 4).
+template <int PA, int PB>
 __device__ __forceinline__ void oz_gload(const OzArgs &g, const int8_t *gA, const int8_t *gB, int64_t row64,
-                                         int d, int kb0, int nkb, int z, u32x4 (&ra)[4], u32x4 (&rb)[2])
+                                         int d, int kb0, int nkb, int z, u32x4 (&ra)[PA], u32x4 (&rb)[PB])
 {
     const int t = z / nkb, b = z - t * nkb;
     const int64_t koff = (int64_t) (kb0 + b) * kKb;
     const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
     const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) ra[p] = *reinterpret_cast<const u32x4 *>(pa + p * row64);
+    for (int p = 0; p < PA; ++p) ra[p] = *reinterpret_cast<const u32x4 *>(pa + p * row64);
 #pragma unroll
-    for (int p = 0; p < 2; ++p) rb[p] = *reinterpret_cast<const u32x4 *>(pb + p * row64);
+    for (int p = 0; p < PB; ++p) rb[p] = *reinterpret_cast<const u32x4 *>(pb + p * row64);
 }
 
-__device__ __forceinline__ void oz_sstore(int8_t *As, int8_t *Bs, int lrow, int lchunk, const u32x4 (&ra)[4],
-                                          const u32x4 (&rb)[2])
+template <int PA, int PB>
+__device__ __forceinline__ void oz_sstore(int8_t *As, int8_t *Bs, int lrow, int lchunk, const u32x4 (&ra)[PA],
+                                          const u32x4 (&rb)[PB])
 {
 #pragma unroll
-    for (int p = 0; p < 4; ++p) *reinterpret_cast<u32x4 *>(&As[lds_off(lrow + 64 * p, lchunk)]) = ra[p];
+    for (int p = 0; p < PA; ++p) *reinterpret_cast<u32x4 *>(&As[lds_off(lrow + 64 * p, lchunk)]) = ra[p];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) *reinterpret_cast<u32x4 *>(&Bs[lds_off(lrow + 64 * p, lchunk)]) = rb[p];
+    for (int p = 0; p < PB; ++p) *reinterpret_cast<u32x4 *>(&Bs[lds_off(lrow + 64 * p, lchunk)]) = rb[p];
 }
 
+template <int FI, int FJ>
 __device__ __forceinline__ void oz_compute(const int8_t *As, const int8_t *Bs, int wm, int wn, int li, int kq,
-                                           i32x4 (&acc)[8][4])
+                                           i32x4 (&acc)[FI][FJ])
 {
-    i32x4 bf[4];
+    i32x4 bf[FJ];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const i32x4 *>(&Bs[lds_off(wn * 64 + j * 16 + li, kq)]);
+    for (int j = 0; j < FJ; ++j)
+        bf[j] = *reinterpret_cast<const i32x4 *>(&Bs[lds_off(wn * FJ * 16 + j * 16 + li, kq)]);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const i32x4 af = *reinterpret_cast<const i32x4 *>(&As[lds_off(wm * 128 + i * 16 + li, kq)]);
+    for (int i = 0; i < FI; ++i) {
+        const i32x4 af = *reinterpret_cast<const i32x4 *>(&As[lds_off(wm * FI * 16 + i * 16 + li, kq)]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af, acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af, acc[i][j], 0, 0, 0);
     }
 }
 
-template <bool SPLIT>
-__global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
+template <bool SPLIT, int WTM, int WTN>
+__global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(OzArgs g)
 {
+    constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;       // workgroup tile
+    constexpr int FI = WTM / 16, FJ = WTN / 16;          // 16 x 16 blocks per wave
+    constexpr int PA = kBMo / 64, PB = kBNo / 64;        // 64-row load passes per stage
+    static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
     __shared__ __attribute__((aligned(16))) int8_t As[2][kBMo * kKb];
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][kBNo * kKb];
 
@@ -306,13 +319,14 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
         kb0 = c * g.kcb;
         nkb = min(g.kcb, g.Kp / kKb - kb0);
         part_off = (int64_t) (d * g.nchunks + c) * g.p32plane;
-        if (g.upper) {        // row tm of tiles holds tn = 2 tm .. tiles_n - 1 (tiles touching j >= i)
+        if (g.upper) {        // row tm of tiles holds tn = tm * R .. tiles_n - 1 (tiles touching j >= i)
+            constexpr int R = kBMo / kBNo;
             tm = 0;
-            while (tm < tiles_m - 1 && tile >= tiles_n - 2 * tm) {
-                tile -= tiles_n - 2 * tm;
+            while (tm < tiles_m - 1 && tile >= tiles_n - R * tm) {
+                tile -= tiles_n - R * tm;
                 ++tm;
             }
-            tn = 2 * tm + tile;
+            tn = R * tm + tile;
         } else {
             tm = tile / tiles_n;
             tn = tile - tm * tiles_n;
@@ -326,48 +340,60 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
     const int64_t row64 = (int64_t) 64 * g.Kp;
     const int total = (d + 1) * nkb;          // stages: slice pair t, k-block b
 
-    i32x4 acc[8][4];
+    // Opaque zeros: without the empty asm the compiler shares one zero
+    // register among the accumulators and then copies them inside the loop.
+    i32x4 acc[FI][FJ];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < FI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+        for (int j = 0; j < FJ; ++j) {
+            acc[i][j] = i32x4{0, 0, 0, 0};
+            asm volatile("" : "+a"(acc[i][j]));
+        }
     const int li = lane & 15, kq = lane >> 4;
 
-    // two register sets: the load of stage z+2 is in flight while stages z
-    // and z+1 compute
-    u32x4 ra0[4], rb0[2], ra1[4], rb1[2];
-    // Loads past the last stage re-read the last stage (in bounds) and the
-    // stores they feed are never consumed; this keeps the register sets
-    // free of control flow so they stay in VGPRs.
+    // Two register staging sets and two LDS buffers.  At the top of a half
+    // iteration the loads of the stage two ahead are issued (the scheduling
+    // barrier keeps them there, so a whole stage of MFMAs covers their
+    // latency), then the current LDS buffer is consumed and the stage one
+    // ahead goes from registers into the other buffer; one barrier per
+    // stage.  Loads past the last stage re-read the last stage (in bounds)
+    // and are never consumed.  The odd last stage is peeled off the loop so
+    // the loop body has no data-dependent branch (a branch there makes the
+    // compiler copy accumulators).
+    u32x4 ra0[PA], rb0[PB], ra1[PA], rb1[PB];
     oz_gload(g, gA, gB, row64, d, kb0, nkb, 0, ra0, rb0);
     oz_gload(g, gA, gB, row64, d, kb0, nkb, min(1, total - 1), ra1, rb1);
     oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
     __syncthreads();
-    for (int z = 0; z < total; z += 2) {
+    int z = 0;
+    for (; z + 1 < total; z += 2) {
+        // buffer 0 holds stage z, set 1 stage z+1
         oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 2, total - 1), ra0, rb0);
+        __builtin_amdgcn_sched_barrier(0);
         oz_compute(As[0], Bs[0], wm, wn, li, kq, acc);
         oz_sstore(As[1], Bs[1], lrow, lchunk, ra1, rb1);
         __syncthreads();
-        if (z + 1 < total) {
-            oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 3, total - 1), ra1, rb1);
-            oz_compute(As[1], Bs[1], wm, wn, li, kq, acc);
-            oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
-            __syncthreads();
-        }
+        oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 3, total - 1), ra1, rb1);
+        __builtin_amdgcn_sched_barrier(0);
+        oz_compute(As[1], Bs[1], wm, wn, li, kq, acc);
+        oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
+        __syncthreads();
     }
+    if (z < total) oz_compute(As[0], Bs[0], wm, wn, li, kq, acc);   // odd last stage
 
     // epilogue: lane l, register r of block (i, j) holds D[m][n] with
     // m = l % 16, n = 4 * (l / 16) + r (operands fed as (B, A))
     if (SPLIT) {              // the padded Mp x Np plane is written whole
         int *P = g.P32 + part_off;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int row = m0 + wm * 128 + i * 16 + li;
+        for (int i = 0; i < FI; ++i) {
+            const int row = m0 + wm * WTM + i * 16 + li;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < FJ; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int col = n0 + wn * 64 + j * 16 + 4 * kq + r;
+                    const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
                     P[(int64_t) col * g.Mp + row] = acc[i][j][r];
                 }
         }
@@ -375,15 +401,15 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
     }
     const double w = ldexp(1.0, -8 * (d + 2));
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int row = m0 + wm * 128 + i * 16 + li;
+    for (int i = 0; i < FI; ++i) {
+        const int row = m0 + wm * WTM + i * 16 + li;
         if (row >= g.M) continue;
         const int ea = g.eA[row];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FJ; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int col = n0 + wn * 64 + j * 16 + 4 * kq + r;
+                const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
                 if (col >= g.N) continue;
                 double *wp = g.W + (int64_t) col * g.Mp + row;
                 double v = (double) acc[i][j][r] * w;
@@ -400,6 +426,8 @@ __global__ __launch_bounds__(256, 2) void ozaki_diag_kernel(OzArgs g)
                 *cp = v;
             }
         }
+        // keep each row block's W traffic together (bounds the live registers)
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -464,6 +492,7 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // Sizes and mode of one product.  gram: op(B) = op(A)^T shares A's planes.
 struct OzPlan {
     int Mp, Np, Kp, kchunk, nchunks, ntiles;
+    int bmo, bno;             // workgroup tile
     bool split, upper;
     size_t planesA, planesB, exps, maxes, acc, total;
 };
@@ -471,6 +500,10 @@ struct OzPlan {
 OzPlan oz_plan(int m, int n, int k, int s, bool gram)
 {
     OzPlan p;
+    // CA3DMM_OZAKI_TILE=256x128 selects the smaller workgroup tile
+    const char *tcfg = getenv("CA3DMM_OZAKI_TILE");
+    p.bmo = 256;
+    p.bno = (tcfg && strcmp(tcfg, "256x128") == 0) ? 128 : 256;
     p.Mp = (int) round_up(m, kTile);
     p.Np = gram ? p.Mp : (int) round_up(n, kTile);
     p.Kp = (int) round_up(std::max(k, 1), kKb);
@@ -478,11 +511,11 @@ OzPlan oz_plan(int m, int n, int k, int s, bool gram)
     p.kchunk = std::min(p.Kp, (int) (((1u << 31) - 1) / (16384u * (unsigned) s)) / kKb * kKb);
     p.nchunks = (p.Kp + p.kchunk - 1) / p.kchunk;
     p.upper = gram;
-    const int tiles_m = p.Mp / kBMo, tiles_n = p.Np / kBNo;
+    const int tiles_m = p.Mp / p.bmo, tiles_n = p.Np / p.bno;
     p.ntiles = tiles_m * tiles_n;
     if (gram) {
         p.ntiles = 0;
-        for (int tm = 0; tm < tiles_m; ++tm) p.ntiles += std::max(0, tiles_n - 2 * tm);
+        for (int tm = 0; tm < tiles_m; ++tm) p.ntiles += std::max(0, tiles_n - tm * (p.bmo / p.bno));
     }
     // split-K when the output tiles alone cannot fill the chip;
     // CA3DMM_OZAKI_SPLIT=0/1 forces the mode of a non-Gram product
@@ -554,14 +587,17 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         g.P32 = (int *) accp;
         g.p32plane = (int64_t) Mp * Np;
         const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
-        ozaki_diag_kernel<true><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
+        if (pl.bno == 256)
+            ozaki_diag_kernel<true, 128, 128><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
+        else
+            ozaki_diag_kernel<true, 128, 64><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
         ozaki_fold_kernel<<<dim3((m + 255) / 256, n), 256, 0, st>>>(g.P32, g.p32plane, s, pl.nchunks, Mp, m, n,
                                                                      eA, eB, alpha, beta, C, ldC,
                                                                      pl.upper ? 1 : 0, mirror ? 1 : 0);
         return ca3::finish();
     }
     g.W = (double *) accp;
-    const int grid = (Mp / kBMo) * (Np / kBNo);
+    const int grid = (Mp / pl.bmo) * (Np / pl.bno);
     for (int d = s - 1; d >= 0; --d) {
         for (int c = 0; c < pl.nchunks; ++c) {
             g.d = d;
@@ -569,7 +605,10 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
             g.nkb = std::min(pl.kchunk, Kp - c * pl.kchunk) / kKb;
             g.first = (d == s - 1 && c == 0);
             g.last = (d == 0 && c == pl.nchunks - 1);
-            ozaki_diag_kernel<false><<<grid, 256, 0, st>>>(g);
+            if (pl.bno == 256)
+                ozaki_diag_kernel<false, 128, 128><<<grid, 256, 0, st>>>(g);
+            else
+                ozaki_diag_kernel<false, 128, 64><<<grid, 256, 0, st>>>(g);
         }
     }
     return ca3::finish();
