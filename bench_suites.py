@@ -102,15 +102,17 @@ def rank_share(args):
     dsts = [hip.DeviceArray((max(1, red["req"][r][2] * red["req"][r][3]),)) for r in range(24)]
     pack, unpack, pack0 = [], [], []
     for r in range(24):
-        s = red["src"][r]
+        s = [int(v) for v in red["src"][r]]
         sl, sc, rl, rc = red["plans"][r]
+        sl = [[int(v) for v in e] for e in sl]
+        rl = [[int(v) for v in e] for e in rl]
         base = Afull.at(s[0] * 8000)        # rank r's column block (row-major rows = columns)
         for e in sl:
             d = (base + 8 * ((e[0] - s[0]) * 8000 + (e[1] - s[1])), 8000, sends[r].at(e[5]), e[3], e[2], e[3])
             pack.append(d)
             if r == 0:
                 pack0.append(d)
-        q = red["req"][r]
+        q = [int(v) for v in red["req"][r]]
         for e in rl:
             unpack.append((recvs[r].at(e[5]), e[3], dsts[r].at((e[0] - q[0]) * q[3] + (e[1] - q[1])), q[3], e[2], e[3]))
     check_blocks(pack, [Afull] + sends)
@@ -267,9 +269,9 @@ def pmc_calib(args):
     sends = [hip.DeviceArray((max(1, red["plans"][r][1]),)) for r in range(24)]
     pack = []
     for r in range(24):
-        s0 = red["src"][r]
+        s0 = [int(v) for v in red["src"][r]]
         base = Afull.at(s0[0] * 8000)
-        for e in red["plans"][r][0]:
+        for e in ([int(v) for v in e] for e in red["plans"][r][0]):
             pack.append((base + 8 * ((e[0] - s0[0]) * 8000 + (e[1] - s0[1])), 8000, sends[r].at(e[5]),
                          e[3], e[2], e[3]))
     check_blocks(pack, [Afull] + sends)

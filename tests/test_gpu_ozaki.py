@@ -99,17 +99,22 @@ def test_ozaki_gram_bit_exact(hip, monkeypatch, tile, n, k, s, mirror):
 
 
 def test_ozaki_gram_accuracy(hip):
-    """phi = 0.5, 256 columns, k = 100000, s = 8: max |G - G_dd| / (|A|^T |A|) below 1e-16."""
+    """phi = 0.5, 256 columns, k = 100000, s = 8: on 500 sampled entries the
+    error against double-double, over (|A|^T |A|), is no worse than the FP64
+    MFMA Gram's and below 1e-15 (the FP64 recombination of s * 7 chunk
+    partials costs a few ulp of |G|)."""
     rnd = np.random.default_rng(8)
     n, k = 256, 100000
     A = Z.phi_matrix(rnd, (k, n), 0.5)
     dA, dG = hip.DeviceArray.from_numpy(A), hip.DeviceArray.zeros((n, n))
-    hip.ozaki_gram(n, k, dA, dG, 8)
-    got = dG.numpy()
     ii, jj = rnd.integers(0, n, 500), rnd.integers(0, n, 500)
     Af = np.asfortranarray(A)
     ref, ab = O.dd_sample(1, 0, n, n, k, Af, k, Af, k, ii, jj)
-    assert np.max(np.abs(got[ii, jj] - ref) / ab) < 1e-16
+    hip.ozaki_gram(n, k, dA, dG, 8)
+    oz = np.max(np.abs(dG.numpy()[ii, jj] - ref) / ab)
+    hip.gram(n, k, dA, dG)
+    native = np.max(np.abs(dG.numpy()[ii, jj] - ref) / ab)
+    assert oz <= native and oz < 1e-15, (oz, native)
 
 
 def test_ozaki_accuracy_matches_fp64(hip):
