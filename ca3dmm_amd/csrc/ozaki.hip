@@ -42,7 +42,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 256;     // padding of lines (a multiple of both tile sides)
-constexpr int kKb = 64;        // k bytes per pipeline stage
+constexpr int kKb = 64;        // k bytes per MFMA step (one LDS sub-block)
+constexpr int kKp = 128;       // k padding and k-chunk granularity (a whole pipeline stage)
 constexpr int kMaxSlices = 16;
 constexpr int kExpEmpty = INT_MIN;
 
@@ -232,34 +233,56 @@ struct OzArgs {
     int64_t p32plane;
 };
 
-// 16-byte chunk swizzle within a 64-byte LDS row: rows r..r+15 read by one
-// MFMA operand land on 16 distinct 16-byte bank groups.
-__device__ __forceinline__ int lds_off(int row, int chunk) { return row * 64 + ((chunk ^ ((row >> 2) & 3)) << 4); }
-
-// One pipeline stage: PA (PB) passes of 64 rows of A (B), 16 bytes of k per
-// thread and pass (thread t: row t / 4 of the pass, 16-byte chunk t % 4).
-template <int PA, int PB>
-__device__ __forceinline__ void oz_gload(const OzArgs &g, const int8_t *gA, const int8_t *gB, int64_t row64,
-                                         int d, int kb0, int nkb, int z, u32x4 (&ra)[PA], u32x4 (&rb)[PB])
+// 16-byte chunk swizzle within a 64-byte LDS row.  An MFMA operand read
+// (ds_read_b128) has lanes 0..15 on rows r..r+15 (r a multiple of 16) at one
+// chunk; with f(row) below those 16 reads fall on 16 distinct 16-byte slots
+// of a 256-byte bank line, and each half (8 lanes) on 8 distinct slots of a
+// 128-byte line, so the read is conflict-free whether the LDS serves 8 or 16
+// such lanes per cycle.  Stores (4 lanes per row) are permutations within rows.
+__device__ __forceinline__ int lds_off(int row, int chunk)
 {
-    const int t = z / nkb, b = z - t * nkb;
-    const int64_t koff = (int64_t) (kb0 + b) * kKb;
-    const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
-    const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
-#pragma unroll
-    for (int p = 0; p < PA; ++p) ra[p] = *reinterpret_cast<const u32x4 *>(pa + p * row64);
-#pragma unroll
-    for (int p = 0; p < PB; ++p) rb[p] = *reinterpret_cast<const u32x4 *>(pb + p * row64);
+    const int f = (((row >> 1) ^ (row >> 3)) & 1) | (((row >> 2) & 1) << 1);
+    return row * 64 + ((chunk ^ f) << 4);
 }
 
-template <int PA, int PB>
-__device__ __forceinline__ void oz_sstore(int8_t *As, int8_t *Bs, int lrow, int lchunk, const u32x4 (&ra)[PA],
-                                          const u32x4 (&rb)[PB])
+// One pipeline stage = KS sub-blocks of 64 bytes of k; per sub-block PA (PB)
+// passes of 64 rows of A (B), 16 bytes per thread and pass (thread t: row
+// t / 4 of the pass, 16-byte chunk t % 4).  Sub-block j of stage z is
+// 64-byte block z * KS + j of the (slice pair, k-block) sequence of the
+// chunk (nkb, the 64-byte blocks per chunk, is a multiple of KS).
+template <int KS, int PA, int PB>
+__device__ __forceinline__ void oz_gload(const OzArgs &g, const int8_t *gA, const int8_t *gB, int64_t row64,
+                                         int d, int kb0, int nkb, int z, u32x4 (&ra)[KS * PA],
+                                         u32x4 (&rb)[KS * PB])
 {
 #pragma unroll
-    for (int p = 0; p < PA; ++p) *reinterpret_cast<u32x4 *>(&As[lds_off(lrow + 64 * p, lchunk)]) = ra[p];
+    for (int j = 0; j < KS; ++j) {
+        const int zz = z * KS + j;
+        const int t = zz / nkb, b = zz - t * nkb;
+        const int64_t koff = (int64_t) (kb0 + b) * kKb;
+        const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
+        const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
 #pragma unroll
-    for (int p = 0; p < PB; ++p) *reinterpret_cast<u32x4 *>(&Bs[lds_off(lrow + 64 * p, lchunk)]) = rb[p];
+        for (int p = 0; p < PA; ++p) ra[j * PA + p] = *reinterpret_cast<const u32x4 *>(pa + p * row64);
+#pragma unroll
+        for (int p = 0; p < PB; ++p) rb[j * PB + p] = *reinterpret_cast<const u32x4 *>(pb + p * row64);
+    }
+}
+
+// LDS sub-block j of a buffer: rows of 64 bytes, A at j * rowsA * 64.
+template <int KS, int PA, int PB>
+__device__ __forceinline__ void oz_sstore(int8_t *As, int8_t *Bs, int lrow, int lchunk, const u32x4 (&ra)[KS * PA],
+                                          const u32x4 (&rb)[KS * PB])
+{
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+#pragma unroll
+        for (int p = 0; p < PA; ++p)
+            *reinterpret_cast<u32x4 *>(&As[j * PA * 64 * 64 + lds_off(lrow + 64 * p, lchunk)]) = ra[j * PA + p];
+#pragma unroll
+        for (int p = 0; p < PB; ++p)
+            *reinterpret_cast<u32x4 *>(&Bs[j * PB * 64 * 64 + lds_off(lrow + 64 * p, lchunk)]) = rb[j * PB + p];
+    }
 }
 
 template <int FI, int FJ>
@@ -278,15 +301,16 @@ __device__ __forceinline__ void oz_compute(const int8_t *As, const int8_t *Bs, i
     }
 }
 
-template <bool SPLIT, int WTM, int WTN>
+template <bool SPLIT, int WTM, int WTN, int KS>
 __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(OzArgs g)
 {
     constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;       // workgroup tile
     constexpr int FI = WTM / 16, FJ = WTN / 16;          // 16 x 16 blocks per wave
-    constexpr int PA = kBMo / 64, PB = kBNo / 64;        // 64-row load passes per stage
+    constexpr int PA = kBMo / 64, PB = kBNo / 64;        // 64-row load passes per sub-block
     static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
-    __shared__ __attribute__((aligned(16))) int8_t As[2][kBMo * kKb];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][kBNo * kKb];
+    static_assert((kKp / kKb) % KS == 0, "a stage divides the k padding");
+    __shared__ __attribute__((aligned(16))) int8_t As[2][KS * kBMo * kKb];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][KS * kBNo * kKb];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -338,7 +362,7 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
     const int8_t *gA = g.PA + (int64_t) (m0 + lrow) * g.Kp + lchunk * 16;
     const int8_t *gB = g.PB + (int64_t) (n0 + lrow) * g.Kp + lchunk * 16;
     const int64_t row64 = (int64_t) 64 * g.Kp;
-    const int total = (d + 1) * nkb;          // stages: slice pair t, k-block b
+    const int total = (d + 1) * nkb / KS;     // stages of KS 64-byte blocks over (slice pair, k-block)
 
     // Opaque zeros: without the empty asm the compiler shares one zero
     // register among the accumulators and then copies them inside the loop.
@@ -352,35 +376,63 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
         }
     const int li = lane & 15, kq = lane >> 4;
 
-    // Two register staging sets and two LDS buffers.  At the top of a half
-    // iteration the loads of the stage two ahead are issued (the scheduling
-    // barrier keeps them there, so a whole stage of MFMAs covers their
-    // latency), then the current LDS buffer is consumed and the stage one
-    // ahead goes from registers into the other buffer; one barrier per
-    // stage.  Loads past the last stage re-read the last stage (in bounds)
-    // and are never consumed.  The odd last stage is peeled off the loop so
-    // the loop body has no data-dependent branch (a branch there makes the
-    // compiler copy accumulators).
-    u32x4 ra0[PA], rb0[PB], ra1[PA], rb1[PB];
-    oz_gload(g, gA, gB, row64, d, kb0, nkb, 0, ra0, rb0);
-    oz_gload(g, gA, gB, row64, d, kb0, nkb, min(1, total - 1), ra1, rb1);
-    oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
-    __syncthreads();
-    int z = 0;
-    for (; z + 1 < total; z += 2) {
-        // buffer 0 holds stage z, set 1 stage z+1
-        oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 2, total - 1), ra0, rb0);
-        __builtin_amdgcn_sched_barrier(0);
-        oz_compute(As[0], Bs[0], wm, wn, li, kq, acc);
-        oz_sstore(As[1], Bs[1], lrow, lchunk, ra1, rb1);
+    // Pipeline.  DEPTH 2 (small tiles): two register staging sets; at the
+    // top of a half iteration the loads of the stage two ahead are issued,
+    // then the current LDS buffer is consumed and the stage one ahead goes
+    // from registers into the other buffer; the odd last stage is peeled
+    // off the loop.  DEPTH 1 (large tiles, whose KS = 2 stages are long
+    // enough to cover a load): one staging set, loads one stage ahead.
+    // Either way one barrier per stage, no data-dependent branch inside the
+    // loop (a branch there makes the compiler copy accumulators), and the
+    // scheduling barrier keeps each stage's loads ahead of its MFMAs.
+    // Loads past the last stage re-read the last stage (in bounds) and are
+    // never consumed.
+    constexpr int DEPTH = (KS == 1) ? 2 : 1;
+    if (DEPTH == 2) {
+        u32x4 ra0[KS * PA], rb0[KS * PB], ra1[KS * PA], rb1[KS * PB];
+        oz_gload<KS, PA, PB>(g, gA, gB, row64, d, kb0, nkb, 0, ra0, rb0);
+        oz_gload<KS, PA, PB>(g, gA, gB, row64, d, kb0, nkb, min(1, total - 1), ra1, rb1);
+        oz_sstore<KS, PA, PB>(As[0], Bs[0], lrow, lchunk, ra0, rb0);
         __syncthreads();
-        oz_gload(g, gA, gB, row64, d, kb0, nkb, min(z + 3, total - 1), ra1, rb1);
-        __builtin_amdgcn_sched_barrier(0);
-        oz_compute(As[1], Bs[1], wm, wn, li, kq, acc);
-        oz_sstore(As[0], Bs[0], lrow, lchunk, ra0, rb0);
+        int z = 0;
+        for (; z + 1 < total; z += 2) {
+            // buffer 0 holds stage z, set 1 stage z+1
+            oz_gload<KS, PA, PB>(g, gA, gB, row64, d, kb0, nkb, min(z + 2, total - 1), ra0, rb0);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KS; ++j)
+                oz_compute(&As[0][j * kBMo * kKb], &Bs[0][j * kBNo * kKb], wm, wn, li, kq, acc);
+            oz_sstore<KS, PA, PB>(As[1], Bs[1], lrow, lchunk, ra1, rb1);
+            __syncthreads();
+            oz_gload<KS, PA, PB>(g, gA, gB, row64, d, kb0, nkb, min(z + 3, total - 1), ra1, rb1);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KS; ++j)
+                oz_compute(&As[1][j * kBMo * kKb], &Bs[1][j * kBNo * kKb], wm, wn, li, kq, acc);
+            oz_sstore<KS, PA, PB>(As[0], Bs[0], lrow, lchunk, ra0, rb0);
+            __syncthreads();
+        }
+        if (z < total) {      // odd last stage
+#pragma unroll
+            for (int j = 0; j < KS; ++j)
+                oz_compute(&As[0][j * kBMo * kKb], &Bs[0][j * kBNo * kKb], wm, wn, li, kq, acc);
+        }
+    } else {
+        u32x4 ra[KS * PA], rb[KS * PB];
+        oz_gload<KS, PA, PB>(g, gA, gB, row64, d, kb0, nkb, 0, ra, rb);
+        oz_sstore<KS, PA, PB>(As[0], Bs[0], lrow, lchunk, ra, rb);
         __syncthreads();
+        for (int z = 0; z < total; ++z) {
+            const int cur = z & 1;
+            oz_gload<KS, PA, PB>(g, gA, gB, row64, d, kb0, nkb, min(z + 1, total - 1), ra, rb);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KS; ++j)
+                oz_compute(&As[cur][j * kBMo * kKb], &Bs[cur][j * kBNo * kKb], wm, wn, li, kq, acc);
+            oz_sstore<KS, PA, PB>(As[cur ^ 1], Bs[cur ^ 1], lrow, lchunk, ra, rb);
+            __syncthreads();
+        }
     }
-    if (z < total) oz_compute(As[0], Bs[0], wm, wn, li, kq, acc);   // odd last stage
 
     // epilogue: lane l, register r of block (i, j) holds D[m][n] with
     // m = l % 16, n = 4 * (l / 16) + r (operands fed as (B, A))
@@ -506,9 +558,9 @@ OzPlan oz_plan(int m, int n, int k, int s, bool gram)
     p.bno = (tcfg && strcmp(tcfg, "256x128") == 0) ? 128 : 256;
     p.Mp = (int) round_up(m, kTile);
     p.Np = gram ? p.Mp : (int) round_up(n, kTile);
-    p.Kp = (int) round_up(std::max(k, 1), kKb);
+    p.Kp = (int) round_up(std::max(k, 1), kKp);
     // INT32 exactness of one diagonal: s * kchunk * 128 * 128 < 2^31
-    p.kchunk = std::min(p.Kp, (int) (((1u << 31) - 1) / (16384u * (unsigned) s)) / kKb * kKb);
+    p.kchunk = std::min(p.Kp, (int) (((1u << 31) - 1) / (16384u * (unsigned) s)) / kKp * kKp);
     p.nchunks = (p.Kp + p.kchunk - 1) / p.kchunk;
     p.upper = gram;
     const int tiles_m = p.Mp / p.bmo, tiles_n = p.Np / p.bno;
@@ -588,9 +640,9 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         g.p32plane = (int64_t) Mp * Np;
         const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
         if (pl.bno == 256)
-            ozaki_diag_kernel<true, 128, 128><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
+            ozaki_diag_kernel<true, 128, 128, 2><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
         else
-            ozaki_diag_kernel<true, 128, 64><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
+            ozaki_diag_kernel<true, 128, 64, 1><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
         ozaki_fold_kernel<<<dim3((m + 255) / 256, n), 256, 0, st>>>(g.P32, g.p32plane, s, pl.nchunks, Mp, m, n,
                                                                      eA, eB, alpha, beta, C, ldC,
                                                                      pl.upper ? 1 : 0, mirror ? 1 : 0);
@@ -606,9 +658,9 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
             g.first = (d == s - 1 && c == 0);
             g.last = (d == 0 && c == pl.nchunks - 1);
             if (pl.bno == 256)
-                ozaki_diag_kernel<false, 128, 128><<<grid, 256, 0, st>>>(g);
+                ozaki_diag_kernel<false, 128, 128, 2><<<grid, 256, 0, st>>>(g);
             else
-                ozaki_diag_kernel<false, 128, 64><<<grid, 256, 0, st>>>(g);
+                ozaki_diag_kernel<false, 128, 64, 1><<<grid, 256, 0, st>>>(g);
         }
     }
     return ca3::finish();

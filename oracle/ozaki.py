@@ -164,10 +164,11 @@ def reconstruct8(slices, E, axis):
 
 def kchunk8(k, s):
     """k-block length over which one diagonal stays exact in INT32:
-    s * kchunk * 128 * 128 < 2^31, a multiple of 64, at most k rounded up
-    to 64 (the rule of the device kernel)."""
-    kp = -(-max(k, 1) // 64) * 64
-    return min(kp, ((2 ** 31 - 1) // (16384 * s)) // 64 * 64)
+    s * kchunk * 128 * 128 < 2^31, a multiple of 128, at most k rounded up
+    to 128 (the rule of the device kernel, whose pipeline stages hold 128
+    bytes of k)."""
+    kp = -(-max(k, 1) // 128) * 128
+    return min(kp, ((2 ** 31 - 1) // (16384 * s)) // 128 * 128)
 
 
 def ozaki_dgemm8(opA, opB, s, alpha=1.0, beta=0.0, Cin=None):

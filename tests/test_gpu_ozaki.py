@@ -47,7 +47,7 @@ def test_ozaki_alpha_beta(hip):
 
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_ozaki_k_chunks(hip, monkeypatch, split):
-    """s = 16 limits an INT32-exact k-chunk to 8128: k = 20000 runs 3 chunks,
+    """s = 16 limits an INT32-exact k-chunk to 8064: k = 20000 runs 3 chunks,
     through the per-diagonal launches (split = 0) and the split-K launch with
     INT32 partials and a separate fold (split = 1)."""
     monkeypatch.setenv("CA3DMM_OZAKI_SPLIT", split)
