@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    mainloop(la, lb, 0, g.K, As, Bs, acc);
+    mainloop(la, lb, 0, g.K, m0 + kBM <= g.M && n0 + kBN <= g.N, As, Bs, acc);
 
     epilogue(g, m0, n0, acc);
 }

@@ -111,6 +111,21 @@ struct OperandLoader {
         }
     }
 
+    // The same stage with every element in range (interior tile, whole
+    // k-stage): no predicates, so the loads issue without branches.
+    __device__ __forceinline__ void load_full(int k0)
+    {
+        if (XC) {
+            const double *q = base + (int64_t) (k0 + kk0) * ld;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) r[p] = q[(int64_t) (2 * p) * ld];
+        } else {
+            const double *q = base + (k0 + kk0);
+#pragma unroll
+            for (int p = 0; p < 8; ++p) r[p] = q[(int64_t) (16 * p) * ld];
+        }
+    }
+
     __device__ __forceinline__ void store(double *S) const
     {
         if (XC) {
@@ -148,11 +163,12 @@ __device__ __forceinline__ void stage_mfma(const double *As, const double *Bs, f
     }
 }
 
-// Accumulate op(A)[m0:+128, kbeg:kend) * op(B)[kbeg:kend, n0:+128) into acc.
+// Accumulate op(A)[m0:+128, kbeg:kend) * op(B)[kbeg:kend, n0:+128) into acc
+// with bounds checks on every element (edge tiles and partial k-stages).
 template <class LA, class LB>
-__device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend,
-                                         double (*As)[kBK * kBM], double (*Bs)[kBK * kBN],
-                                         f64x4 (&acc)[4][4])
+__device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int kend,
+                                                 double (*As)[kBK * kBM], double (*Bs)[kBK * kBN],
+                                                 f64x4 (&acc)[4][4])
 {
     if (kend <= kbeg) return;
     const int nk = (kend - kbeg + kBK - 1) / kBK;
@@ -175,6 +191,39 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend,
         }
         __syncthreads();
     }
+}
+
+// The same accumulation.  A tile that is interior in m and n (interior =
+// m0 + 128 <= M and n0 + 128 <= N) runs its whole k-stages through a loop
+// without predicates or data-dependent branches (one register stage in
+// flight, the last stage's load repeated in bounds and never consumed);
+// edge tiles and the k tail go through mainloop_checked.  Both add the
+// k-steps in the same order, so results do not depend on the path.
+template <class LA, class LB>
+__device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, bool interior,
+                                         double (*As)[kBK * kBM], double (*Bs)[kBK * kBN],
+                                         f64x4 (&acc)[4][4])
+{
+    const int nfull = (interior && kend > kbeg) ? (kend - kbeg) / kBK : 0;
+    if (nfull > 0) {
+        la.load_full(kbeg);
+        lb.load_full(kbeg);
+        la.store(As[0]);
+        lb.store(Bs[0]);
+        __syncthreads();
+        for (int kt = 0; kt < nfull; ++kt) {
+            const int cur = kt & 1;
+            const int kn = kbeg + min(kt + 1, nfull - 1) * kBK;
+            la.load_full(kn);
+            lb.load_full(kn);
+            stage_mfma(As[cur], Bs[cur], acc);
+            la.store(As[cur ^ 1]);
+            lb.store(Bs[cur ^ 1]);
+            __syncthreads();
+        }
+    }
+    const int kt0 = kbeg + nfull * kBK;
+    if (kt0 < kend) mainloop_checked(la, lb, kt0, kend, As, Bs, acc);
 }
 
 // C := fold(P) + (alpha*acc + beta*C) for the tile at (m0, n0).

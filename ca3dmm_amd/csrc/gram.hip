@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-    mainloop(la, lb, kbeg, kend, As, Bs, acc);
+    mainloop(la, lb, kbeg, kend, (ti + 1) * kBM <= g.n && (tj + 1) * kBN <= g.n, As, Bs, acc);
     // store the full 128 x 128 partial (rows fastest)
     double *out = g.P + ((int64_t) ks * g.npairs + pair) * (kBM * kBN);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kThreads, 1) void grouped_kernel(const Problem *__r
             OperandLoader<0> lb;     // B block as stored: contiguous along k
             la.init(P.seg[s].A, P.seg[s].lda, m0, P.M);
             lb.init(P.seg[s].B, P.seg[s].ldb, n0, P.N);
-            mainloop(la, lb, 0, P.seg[s].k, As, Bs, acc);
+            mainloop(la, lb, 0, P.seg[s].k, m0 + kBM <= P.M && n0 + kBN <= P.N, As, Bs, acc);
         }
         // store the tile (lane l, register r of block (i, j): row l % 16, column 4r + l / 16)
         const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
