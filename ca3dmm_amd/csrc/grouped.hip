@@ -64,7 +64,7 @@ __device__ __forceinline__ void grouped_coords(int t, int tiles_m, int tiles_n, 
     *tn = r / gsz;
 }
 
-__global__ __launch_bounds__(kThreads, 1) void grouped_kernel(const Problem *__restrict__ probs, int nprob,
+__global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Problem *__restrict__ probs, int nprob,
                                                                const int *__restrict__ tile0,
                                                                const Fold *__restrict__ folds,
                                                                const int *__restrict__ ftile0, int *counters,

@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "ca3dmm_ext.h"
 #include "hip_common.h"
@@ -43,7 +44,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 256;     // padding of lines (a multiple of both tile sides)
 constexpr int kKb = 64;        // k bytes per MFMA step (one LDS sub-block)
-constexpr int kKp = 128;       // k padding and k-chunk granularity (a whole pipeline stage)
+constexpr int kKp = 256;       // k padding and k-chunk granularity (whole pipeline rounds)
 constexpr int kMaxSlices = 16;
 constexpr int kExpEmpty = INT_MIN;
 
@@ -301,22 +302,15 @@ __device__ __forceinline__ void oz_compute(const int8_t *As, const int8_t *Bs, i
     }
 }
 
-template <bool SPLIT, int WTM, int WTN, int KS>
-__global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(OzArgs g)
+// The work item of this workgroup: output tile (tm, tn), diagonal d and the
+// k-chunk [kb0, kb0 + nkb) in 64-byte blocks; false for an idle workgroup of
+// the split form's last round.
+template <bool SPLIT, int kBMo, int kBNo>
+__device__ __forceinline__ bool oz_item(const OzArgs &g, int &tm, int &tn, int &d, int &kb0, int &nkb,
+                                        int64_t &part_off)
 {
-    constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;       // workgroup tile
-    constexpr int FI = WTM / 16, FJ = WTN / 16;          // 16 x 16 blocks per wave
-    constexpr int PA = kBMo / 64, PB = kBNo / 64;        // 64-row load passes per sub-block
-    static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
-    static_assert((kKp / kKb) % KS == 0, "a stage divides the k padding");
-    __shared__ __attribute__((aligned(16))) int8_t As[2][KS * kBMo * kKb];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][KS * kBNo * kKb];
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
     const int tiles_m = g.Mp / kBMo, tiles_n = g.Np / kBNo;
-    int tm, tn, d, kb0, nkb;
-    int64_t part_off = 0;
+    part_off = 0;
     if (!SPLIT) {
         int bid = blockIdx.x;
         const int nb = gridDim.x;
@@ -336,7 +330,7 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
         const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
         const int jq = j / g.ntiles;
         const int grp = jq * 8 + x;
-        if (grp >= g.ngroups) return;
+        if (grp >= g.ngroups) return false;
         int tile = j - jq * g.ntiles;
         d = g.s - 1 - grp / g.nchunks;
         const int c = grp - (grp / g.nchunks) * g.nchunks;
@@ -356,6 +350,97 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
             tn = tile - tm * tiles_n;
         }
     }
+    return true;
+}
+
+// Opaque zeros: without the empty asm the compiler shares one zero register
+// among the accumulators and then copies them inside the main loop.
+template <int FI, int FJ>
+__device__ __forceinline__ void oz_zero(i32x4 (&acc)[FI][FJ])
+{
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+            acc[i][j] = i32x4{0, 0, 0, 0};
+            asm volatile("" : "+a"(acc[i][j]));
+        }
+}
+
+// Epilogue: lane l, register r of block (i, j) holds D[m][n] with m = l % 16,
+// n = 4 * (l / 16) + r (operands fed as (B, A)).
+template <bool SPLIT, int WTM, int WTN, int FI, int FJ>
+__device__ __forceinline__ void oz_epilogue(const OzArgs &g, const i32x4 (&acc)[FI][FJ], int m0, int n0, int d,
+                                            int64_t part_off)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, kq = lane >> 4;
+    if (SPLIT) {              // the padded Mp x Np plane is written whole
+        int *P = g.P32 + part_off;
+#pragma unroll
+        for (int i = 0; i < FI; ++i) {
+            const int row = m0 + wm * WTM + i * 16 + li;
+#pragma unroll
+            for (int j = 0; j < FJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
+                    P[(int64_t) col * g.Mp + row] = acc[i][j][r];
+                }
+        }
+        return;
+    }
+    const double w = ldexp(1.0, -8 * (d + 2));
+#pragma unroll
+    for (int i = 0; i < FI; ++i) {
+        const int row = m0 + wm * WTM + i * 16 + li;
+        if (row >= g.M) continue;
+        const int ea = g.eA[row];
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
+                if (col >= g.N) continue;
+                double *wp = g.W + (int64_t) col * g.Mp + row;
+                double v = (double) acc[i][j][r] * w;
+                if (!g.first) v = *wp + v;
+                if (!g.last) {
+                    *wp = v;
+                    continue;
+                }
+                const int eb = g.eB[col];
+                v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
+                double *cp = g.C + (int64_t) col * g.ldc + row;
+                v = __dmul_rn(g.alpha, v);
+                if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *cp));
+                *cp = v;
+            }
+        }
+        // keep each row block's W traffic together (bounds the live registers)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+
+// Register-staged form: global -> registers -> LDS, two LDS buffers.
+template <bool SPLIT, int WTM, int WTN, int KS>
+__global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(OzArgs g)
+{
+    constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;       // workgroup tile
+    constexpr int FI = WTM / 16, FJ = WTN / 16;          // 16 x 16 blocks per wave
+    constexpr int PA = kBMo / 64, PB = kBNo / 64;        // 64-row load passes per sub-block
+    static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
+    static_assert((kKp / kKb) % KS == 0, "a stage divides the k padding");
+    __shared__ __attribute__((aligned(16))) int8_t As[2][KS * kBMo * kKb];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][KS * kBNo * kKb];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn, d, kb0, nkb;
+    int64_t part_off;
+    if (!oz_item<SPLIT, kBMo, kBNo>(g, tm, tn, d, kb0, nkb, part_off)) return;
     const int m0 = tm * kBMo, n0 = tn * kBNo;
 
     const int lrow = tid >> 2, lchunk = tid & 3;
@@ -364,16 +449,8 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
     const int64_t row64 = (int64_t) 64 * g.Kp;
     const int total = (d + 1) * nkb / KS;     // stages of KS 64-byte blocks over (slice pair, k-block)
 
-    // Opaque zeros: without the empty asm the compiler shares one zero
-    // register among the accumulators and then copies them inside the loop.
     i32x4 acc[FI][FJ];
-#pragma unroll
-    for (int i = 0; i < FI; ++i)
-#pragma unroll
-        for (int j = 0; j < FJ; ++j) {
-            acc[i][j] = i32x4{0, 0, 0, 0};
-            asm volatile("" : "+a"(acc[i][j]));
-        }
+    oz_zero(acc);
     const int li = lane & 15, kq = lane >> 4;
 
     // Pipeline.  DEPTH 2 (small tiles): two register staging sets; at the
@@ -434,53 +511,122 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
         }
     }
 
-    // epilogue: lane l, register r of block (i, j) holds D[m][n] with
-    // m = l % 16, n = 4 * (l / 16) + r (operands fed as (B, A))
-    if (SPLIT) {              // the padded Mp x Np plane is written whole
-        int *P = g.P32 + part_off;
+    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
+}
+
+// Direct-to-LDS form: global_load_lds_dwordx4 writes lane l's 16 bytes at
+// M0 + 16 l, so one instruction fills a 1 KB block of 16 LDS rows; lane l
+// fetches row l / 4 of the block at the logical chunk (l % 4) ^ f(row), which
+// lands it where lds_off puts that chunk.  No staging registers: NBUF LDS
+// stages of 64 bytes of k, the loads of stage z + NBUF - 1 issued right
+// after the barrier that opens stage z.  Each wave loads WA blocks of A and
+// WB of B per stage; before the barrier it waits until its own loads of
+// stage z are done (vmcnt <= (NBUF - 2) * (WA + WB)), the barrier then makes
+// every wave's part visible.  Loads past the last stage re-read the last
+// stage (in bounds) into a buffer that is never consumed.
+template <bool SPLIT, int WTM, int WTN, int NBUF>
+__global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(OzArgs g)
+{
+    constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;
+    constexpr int FI = WTM / 16, FJ = WTN / 16;
+    constexpr int WA = kBMo / 64, WB = kBNo / 64;       // 1 KB blocks per wave per stage
+    constexpr int L = WA + WB;
+    static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
+    static_assert(NBUF >= 3 && NBUF <= 4 && (NBUF - 2) * L <= 63, "ring size / vmcnt range");
+    // one LDS variable per ring slot, so the compiler can tell a slot being
+    // filled from the slot being read and need not drain the loads in flight
+    __shared__ __attribute__((aligned(1024))) int8_t A0[kBMo * kKb], A1[kBMo * kKb], A2[kBMo * kKb];
+    __shared__ __attribute__((aligned(1024))) int8_t B0[kBNo * kKb], B1[kBNo * kKb], B2[kBNo * kKb];
+    __shared__ __attribute__((aligned(1024))) int8_t A3[NBUF > 3 ? kBMo * kKb : 16], B3[NBUF > 3 ? kBNo * kKb : 16];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn, d, kb0, nkb;
+    int64_t part_off;
+    if (!oz_item<SPLIT, kBMo, kBNo>(g, tm, tn, d, kb0, nkb, part_off)) return;
+    const int m0 = tm * kBMo, n0 = tn * kBNo;
+
+    // this lane's part of every 1 KB block: row r = 16 * blk + lane / 4
+    const int lr = lane >> 2;
+    const int lc = ((lane & 3) ^ ((((lr >> 1) ^ (lr >> 3)) & 1) | (((lr >> 2) & 1) << 1))) << 4;
+    const int8_t *gA = g.PA + (int64_t) (m0 + wave * WA * 16 + lr) * g.Kp + lc;
+    const int8_t *gB = g.PB + (int64_t) (n0 + wave * WB * 16 + lr) * g.Kp + lc;
+    const int64_t blk16 = (int64_t) 16 * g.Kp;
+    const int total = (d + 1) * nkb;          // stages of 64 bytes of k over (slice pair, k-block)
+
+    auto slotA = [&](auto J) -> int8_t * {
+        if constexpr (J.value == 0) return A0;
+        else if constexpr (J.value == 1) return A1;
+        else if constexpr (J.value == 2) return A2;
+        else return A3;
+    };
+    auto slotB = [&](auto J) -> int8_t * {
+        if constexpr (J.value == 0) return B0;
+        else if constexpr (J.value == 1) return B1;
+        else if constexpr (J.value == 2) return B2;
+        else return B3;
+    };
+    auto issue = [&](int z, auto J) __attribute__((always_inline)) {
+        const int t = z / nkb, b = z - t * nkb;
+        const int64_t koff = (int64_t) (kb0 + b) * kKb;
+        const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
+        const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
+        int8_t *sa = slotA(J), *sb = slotB(J);
 #pragma unroll
-        for (int i = 0; i < FI; ++i) {
-            const int row = m0 + wm * WTM + i * 16 + li;
+        for (int i = 0; i < WA; ++i)
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) (pa + i * blk16),
+                                             (void __attribute__((address_space(3))) *) (sa + (wave * WA + i) * 1024),
+                                             16, 0, 0);
 #pragma unroll
-            for (int j = 0; j < FJ; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
-                    P[(int64_t) col * g.Mp + row] = acc[i][j][r];
-                }
-        }
-        return;
+        for (int i = 0; i < WB; ++i)
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) (pb + i * blk16),
+                                             (void __attribute__((address_space(3))) *) (sb + (wave * WB + i) * 1024),
+                                             16, 0, 0);
+    };
+
+    i32x4 acc[FI][FJ];
+    oz_zero(acc);
+    const int li = lane & 15, kq = lane >> 4;
+
+    // s_waitcnt: vmcnt = (NBUF - 2) * L (bits 3:0 and 15:14), expcnt and lgkmcnt not waited on
+    constexpr int kVm = (NBUF - 2) * L;
+    constexpr int kWait = (kVm & 15) | (((kVm >> 4) & 3) << 14) | (7 << 4) | (15 << 8);
+    // Stage z uses slot z % NBUF; after the barrier that opens it, the loads
+    // of stage z + NBUF - 1 go into slot (z - 1) % NBUF.
+    auto step = [&](int z, auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        // own loads of stage z landed; then every wave's (no fence: a fenced
+        // barrier would also wait for the later stages still in flight).  A
+        // wave's LDS reads of stage z - 1 have returned before it gets here,
+        // since the MFMAs consumed them.
+        __builtin_amdgcn_s_waitcnt(kWait);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        issue(min(z + NBUF - 1, total - 1), std::integral_constant<int, (j + NBUF - 1) % NBUF>{});
+        oz_compute(slotA(J), slotB(J), wm, wn, li, kq, acc);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    issue(0, I0{});
+    issue(min(1, total - 1), I1{});
+    if constexpr (NBUF > 3) issue(min(2, total - 1), I2{});
+    // total = (d + 1) * nkb with nkb a multiple of kKp / kKb = 4, so the
+    // ring turns a whole number of times (no remainder steps, which would
+    // make the compiler copy accumulators)
+    static_assert((kKp / kKb) % NBUF == 0, "ring size divides the k padding");
+    for (int z = 0; z < total; z += NBUF) {
+        step(z, I0{});
+        step(z + 1, I1{});
+        step(z + 2, I2{});
+        if constexpr (NBUF > 3) step(z + 3, I3{});
     }
-    const double w = ldexp(1.0, -8 * (d + 2));
-#pragma unroll
-    for (int i = 0; i < FI; ++i) {
-        const int row = m0 + wm * WTM + i * 16 + li;
-        if (row >= g.M) continue;
-        const int ea = g.eA[row];
-#pragma unroll
-        for (int j = 0; j < FJ; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
-                if (col >= g.N) continue;
-                double *wp = g.W + (int64_t) col * g.Mp + row;
-                double v = (double) acc[i][j][r] * w;
-                if (!g.first) v = *wp + v;
-                if (!g.last) {
-                    *wp = v;
-                    continue;
-                }
-                const int eb = g.eB[col];
-                v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
-                double *cp = g.C + (int64_t) col * g.ldc + row;
-                v = __dmul_rn(g.alpha, v);
-                if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *cp));
-                *cp = v;
-            }
-        }
-        // keep each row block's W traffic together (bounds the live registers)
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    // the repeated loads of the last stage may still be landing in LDS:
+    // drain them before the workgroup can end
+    __builtin_amdgcn_s_waitcnt(0);
+    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
 }
 
 // Split mode, step 2: per element, the INT32 partials added in FP64 in the
@@ -545,6 +691,7 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 struct OzPlan {
     int Mp, Np, Kp, kchunk, nchunks, ntiles;
     int bmo, bno;             // workgroup tile
+    bool dtl;                 // direct-to-LDS loads
     bool split, upper;
     size_t planesA, planesB, exps, maxes, acc, total;
 };
@@ -552,7 +699,10 @@ struct OzPlan {
 OzPlan oz_plan(int m, int n, int k, int s, bool gram)
 {
     OzPlan p;
-    // CA3DMM_OZAKI_TILE=256x128 selects the smaller workgroup tile
+    // CA3DMM_OZAKI_TILE=256x128 selects the smaller workgroup tile,
+    // CA3DMM_OZAKI_LOADS=reg the register-staged loads instead of direct-to-LDS
+    const char *lcfg = getenv("CA3DMM_OZAKI_LOADS");
+    p.dtl = !(lcfg && strcmp(lcfg, "reg") == 0);
     const char *tcfg = getenv("CA3DMM_OZAKI_TILE");
     p.bmo = 256;
     p.bno = (tcfg && strcmp(tcfg, "256x128") == 0) ? 128 : 256;
@@ -639,10 +789,14 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         g.P32 = (int *) accp;
         g.p32plane = (int64_t) Mp * Np;
         const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
-        if (pl.bno == 256)
-            ozaki_diag_kernel<true, 128, 128, 2><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
-        else
-            ozaki_diag_kernel<true, 128, 64, 1><<<(unsigned) (slots * 8), 256, 0, st>>>(g);
+        const unsigned nwg = (unsigned) (slots * 8);
+        if (pl.dtl) {
+            if (pl.bno == 256) ozaki_dtl_kernel<true, 128, 128, 4><<<nwg, 256, 0, st>>>(g);
+            else ozaki_dtl_kernel<true, 128, 64, 4><<<nwg, 256, 0, st>>>(g);
+        } else {
+            if (pl.bno == 256) ozaki_diag_kernel<true, 128, 128, 2><<<nwg, 256, 0, st>>>(g);
+            else ozaki_diag_kernel<true, 128, 64, 1><<<nwg, 256, 0, st>>>(g);
+        }
         ozaki_fold_kernel<<<dim3((m + 255) / 256, n), 256, 0, st>>>(g.P32, g.p32plane, s, pl.nchunks, Mp, m, n,
                                                                      eA, eB, alpha, beta, C, ldC,
                                                                      pl.upper ? 1 : 0, mirror ? 1 : 0);
@@ -657,10 +811,13 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
             g.nkb = std::min(pl.kchunk, Kp - c * pl.kchunk) / kKb;
             g.first = (d == s - 1 && c == 0);
             g.last = (d == 0 && c == pl.nchunks - 1);
-            if (pl.bno == 256)
-                ozaki_diag_kernel<false, 128, 128, 2><<<grid, 256, 0, st>>>(g);
-            else
-                ozaki_diag_kernel<false, 128, 64, 1><<<grid, 256, 0, st>>>(g);
+            if (pl.dtl) {
+                if (pl.bno == 256) ozaki_dtl_kernel<false, 128, 128, 4><<<grid, 256, 0, st>>>(g);
+                else ozaki_dtl_kernel<false, 128, 64, 4><<<grid, 256, 0, st>>>(g);
+            } else {
+                if (pl.bno == 256) ozaki_diag_kernel<false, 128, 128, 2><<<grid, 256, 0, st>>>(g);
+                else ozaki_diag_kernel<false, 128, 64, 1><<<grid, 256, 0, st>>>(g);
+            }
         }
     }
     return ca3::finish();

@@ -164,11 +164,11 @@ def reconstruct8(slices, E, axis):
 
 def kchunk8(k, s):
     """k-block length over which one diagonal stays exact in INT32:
-    s * kchunk * 128 * 128 < 2^31, a multiple of 128, at most k rounded up
-    to 128 (the rule of the device kernel, whose pipeline stages hold 128
-    bytes of k)."""
-    kp = -(-max(k, 1) // 128) * 128
-    return min(kp, ((2 ** 31 - 1) // (16384 * s)) // 128 * 128)
+    s * kchunk * 128 * 128 < 2^31, a multiple of 256, at most k rounded up
+    to 256 (the rule of the device kernel, whose LDS ring of four 64-byte
+    stages turns a whole number of times per chunk)."""
+    kp = -(-max(k, 1) // 256) * 256
+    return min(kp, ((2 ** 31 - 1) // (16384 * s)) // 256 * 256)
 
 
 def ozaki_dgemm8(opA, opB, s, alpha=1.0, beta=0.0, Cin=None):

@@ -10,15 +10,18 @@ from oracle import ozaki as Z
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("loads", ["dtl", "reg"])
 @pytest.mark.parametrize("tile", ["256x256", "256x128"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("m,n,k,s", [(1, 1, 1, 1), (17, 33, 65, 3), (128, 128, 64, 8),
                                      (200, 130, 300, 8), (257, 129, 1000, 12), (64, 300, 129, 5),
                                      (300, 520, 257, 7)])
-def test_ozaki_bit_exact_vs_spec(hip, monkeypatch, tile, ta, tb, m, n, k, s):
-    """Both workgroup tiles (CA3DMM_OZAKI_TILE) against the specification;
-    k = 1 and 65 give a one-stage (odd) main loop, k = 257 five stages."""
+def test_ozaki_bit_exact_vs_spec(hip, monkeypatch, loads, tile, ta, tb, m, n, k, s):
+    """Both workgroup tiles (CA3DMM_OZAKI_TILE) and both load paths
+    (CA3DMM_OZAKI_LOADS: direct-to-LDS or register-staged) against the
+    specification, at k from one pipeline round to several."""
     monkeypatch.setenv("CA3DMM_OZAKI_TILE", tile)
+    monkeypatch.setenv("CA3DMM_OZAKI_LOADS", loads)
     rnd = np.random.default_rng(m * 7 + n + k + s + 2 * ta + tb)
     opA = Z.phi_matrix(rnd, (m, k), 2.0)
     opB = Z.phi_matrix(rnd, (k, n), 2.0)
@@ -47,7 +50,7 @@ def test_ozaki_alpha_beta(hip):
 
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_ozaki_k_chunks(hip, monkeypatch, split):
-    """s = 16 limits an INT32-exact k-chunk to 8064: k = 20000 runs 3 chunks,
+    """s = 16 limits an INT32-exact k-chunk to 7936: k = 20000 runs 3 chunks,
     through the per-diagonal launches (split = 0) and the split-K launch with
     INT32 partials and a separate fold (split = 1)."""
     monkeypatch.setenv("CA3DMM_OZAKI_SPLIT", split)
@@ -76,13 +79,15 @@ def test_ozaki_split_mode_bit_exact(hip, monkeypatch, ta, tb, m, n, k, s):
     assert np.array_equal(dC.numpy(), want)
 
 
+@pytest.mark.parametrize("loads", ["dtl", "reg"])
 @pytest.mark.parametrize("tile", ["256x256", "256x128"])
 @pytest.mark.parametrize("n,k,s,mirror", [(1, 5, 3, True), (100, 20000, 12, True), (300, 1000, 8, False),
                                           (513, 700, 8, True)])
-def test_ozaki_gram_bit_exact(hip, monkeypatch, tile, n, k, s, mirror):
+def test_ozaki_gram_bit_exact(hip, monkeypatch, loads, tile, n, k, s, mirror):
     """hip_ozaki_gram equals ozaki_dgemm8(A^T, A) on the upper triangle; the
     lower triangle is its mirror, or C untouched when mirror = 0."""
     monkeypatch.setenv("CA3DMM_OZAKI_TILE", tile)
+    monkeypatch.setenv("CA3DMM_OZAKI_LOADS", loads)
     rnd = np.random.default_rng(n + k + s)
     A = Z.phi_matrix(rnd, (k, n), 1.0)
     if n > 7:

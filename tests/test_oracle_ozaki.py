@@ -79,13 +79,13 @@ def test_zero_lines_give_zero():
 
 def test_k_chunking_rule():
     assert Z.kchunk8(8192, 8) == 8192
-    assert Z.kchunk8(20000, 16) == 8064          # floor(8191.99 / 128) * 128
-    assert Z.kchunk8(1, 3) == 128
-    assert Z.kchunk8(100000, 8) == 16256
+    assert Z.kchunk8(20000, 16) == 7936          # floor(8191.99 / 256) * 256
+    assert Z.kchunk8(1, 3) == 256
+    assert Z.kchunk8(100000, 8) == 16128
     for s in range(1, 17):
         kc = Z.kchunk8(10 ** 7, s)
-        assert s * kc * 128 * 128 < 2 ** 31 and kc % 128 == 0
-        assert s * (kc + 128) * 128 * 128 >= 2 ** 31     # the largest such multiple
+        assert s * kc * 128 * 128 < 2 ** 31 and kc % 256 == 0
+        assert s * (kc + 256) * 128 * 128 >= 2 ** 31     # the largest such multiple
 
 
 def _bias_window_digits(v, E, s):
