@@ -629,6 +629,153 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(Oz
     oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
 }
 
+// Software-pipelined form (the default).  Direct-to-LDS loads fill a ring
+// of four 64-byte stages; the fragments of stage z+1 are read from LDS into
+// a second register set while the MFMAs of stage z run, so the MFMA pipe
+// does not wait for LDS.  At the top of stage z a wave waits for its own
+// loads of stage z+1 (vmcnt <= L: only stage z+2's loads may still be in
+// flight), then one fence-free barrier makes every wave's part of stage z+1
+// visible and frees the slot of stage z-1 (its fragments were read during
+// stage z-2), into which the loads of stage z+3 then go.
+template <int FI, int FJ>
+__device__ __forceinline__ void oz_frags(const int8_t *As, const int8_t *Bs, int wm, int wn, int li, int kq,
+                                         i32x4 (&af)[FI], i32x4 (&bf)[FJ])
+{
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+        bf[j] = *reinterpret_cast<const i32x4 *>(&Bs[lds_off(wn * FJ * 16 + j * 16 + li, kq)]);
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+        af[i] = *reinterpret_cast<const i32x4 *>(&As[lds_off(wm * FI * 16 + i * 16 + li, kq)]);
+}
+
+template <int FI, int FJ>
+__device__ __forceinline__ void oz_mfmas(const i32x4 (&af)[FI], const i32x4 (&bf)[FJ], i32x4 (&acc)[FI][FJ])
+{
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af[i], acc[i][j], 0, 0, 0);
+}
+
+template <bool SPLIT, int WTM, int WTN>
+__global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
+{
+    constexpr int NBUF = 4;
+    constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;
+    constexpr int FI = WTM / 16, FJ = WTN / 16;
+    constexpr int WA = kBMo / 64, WB = kBNo / 64;       // 1 KB blocks per wave per stage
+    constexpr int L = WA + WB;
+    static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
+    static_assert((kKp / kKb) % NBUF == 0 && L <= 63, "ring turns per chunk / vmcnt range");
+    __shared__ __attribute__((aligned(1024))) int8_t A0[kBMo * kKb], A1[kBMo * kKb], A2[kBMo * kKb], A3[kBMo * kKb];
+    __shared__ __attribute__((aligned(1024))) int8_t B0[kBNo * kKb], B1[kBNo * kKb], B2[kBNo * kKb], B3[kBNo * kKb];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn, d, kb0, nkb;
+    int64_t part_off;
+    if (!oz_item<SPLIT, kBMo, kBNo>(g, tm, tn, d, kb0, nkb, part_off)) return;
+    const int m0 = tm * kBMo, n0 = tn * kBNo;
+
+    const int lr = lane >> 2;
+    const int lc = ((lane & 3) ^ ((((lr >> 1) ^ (lr >> 3)) & 1) | (((lr >> 2) & 1) << 1))) << 4;
+    const int8_t *gA = g.PA + (int64_t) (m0 + wave * WA * 16 + lr) * g.Kp + lc;
+    const int8_t *gB = g.PB + (int64_t) (n0 + wave * WB * 16 + lr) * g.Kp + lc;
+    const int64_t blk16 = (int64_t) 16 * g.Kp;
+    const int total = (d + 1) * nkb;          // 64-byte stages; a multiple of NBUF
+
+    auto slotA = [&](auto J) -> int8_t * {
+        if constexpr (J.value == 0) return A0;
+        else if constexpr (J.value == 1) return A1;
+        else if constexpr (J.value == 2) return A2;
+        else return A3;
+    };
+    auto slotB = [&](auto J) -> int8_t * {
+        if constexpr (J.value == 0) return B0;
+        else if constexpr (J.value == 1) return B1;
+        else if constexpr (J.value == 2) return B2;
+        else return B3;
+    };
+    auto issue = [&](int z, auto J) __attribute__((always_inline)) {
+        const int t = z / nkb, b = z - t * nkb;
+        const int64_t koff = (int64_t) (kb0 + b) * kKb;
+        const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
+        const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
+        int8_t *sa = slotA(J), *sb = slotB(J);
+#pragma unroll
+        for (int i = 0; i < WA; ++i)
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) (pa + i * blk16),
+                                             (void __attribute__((address_space(3))) *) (sa + (wave * WA + i) * 1024),
+                                             16, 0, 0);
+#pragma unroll
+        for (int i = 0; i < WB; ++i)
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) (pb + i * blk16),
+                                             (void __attribute__((address_space(3))) *) (sb + (wave * WB + i) * 1024),
+                                             16, 0, 0);
+    };
+    constexpr int kWaitL = (L & 15) | (((L >> 4) & 3) << 14) | (7 << 4) | (15 << 8);      // vmcnt <= L
+    constexpr int kWaitL2 = ((2 * L) & 15) | ((((2 * L) >> 4) & 3) << 14) | (7 << 4) | (15 << 8);
+    auto sync = [&](int wait) __attribute__((always_inline)) {
+        (void) wait;
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+    };
+
+    i32x4 acc[FI][FJ];
+    oz_zero(acc);
+    const int li = lane & 15, kq = lane >> 4;
+    i32x4 af0[FI], bf0[FJ], af1[FI], bf1[FJ];
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    // prologue: stages 0, 1, 2 in flight; fragments of stage 0
+    issue(0, I0{});
+    issue(min(1, total - 1), I1{});
+    issue(min(2, total - 1), I2{});
+    __builtin_amdgcn_s_waitcnt(kWaitL2);      // own loads of stage 0 done
+    sync(0);
+    oz_frags(A0, B0, wm, wn, li, kq, af0, bf0);
+
+    // stage z (slot j = z % 4, fragments in set z % 2): publish stage z+1,
+    // refill slot (z-1) % 4 with stage z+3, read the fragments of stage z+1,
+    // run the MFMAs of stage z
+    auto step = [&](int z, auto J, i32x4 (&afc)[FI], i32x4 (&bfc)[FJ], i32x4 (&afn)[FI], i32x4 (&bfn)[FJ])
+        __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        // scheduling barriers keep the order: barrier, loads, fragment reads
+        // of stage z+1, MFMAs of stage z (MFMAs touch only registers, so the
+        // compiler would otherwise move them across the LDS barrier and wait
+        // on the fragment reads it was meant to hide)
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitL);
+        sync(0);
+        issue(min(z + 3, total - 1), std::integral_constant<int, (j + 3) % 4>{});
+        using JN = std::integral_constant<int, (j + 1) % 4>;
+        oz_frags(slotA(JN{}), slotB(JN{}), wm, wn, li, kq, afn, bfn);
+        __builtin_amdgcn_sched_barrier(0);
+        oz_mfmas(afc, bfc, acc);
+        // the fragment reads had the whole MFMA block to land: settle them
+        // here, where it costs nothing, so the next step's MFMAs start
+        // without a wait (lgkmcnt(0); vmcnt and expcnt not waited on)
+        __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4));
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    for (int z = 0; z < total; z += 4) {
+        step(z, I0{}, af0, bf0, af1, bf1);
+        step(z + 1, I1{}, af1, bf1, af0, bf0);
+        step(z + 2, I2{}, af0, bf0, af1, bf1);
+        step(z + 3, I3{}, af1, bf1, af0, bf0);
+    }
+    // the loads and fragment reads past the last stage were repeats of it;
+    // drain the loads before the workgroup can end
+    __builtin_amdgcn_s_waitcnt(0);
+    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
+}
+
 // Split mode, step 2: per element, the INT32 partials added in FP64 in the
 // order d = s-1 .. 0, chunks ascending, then scaled and written like the W
 // mode's last launch.  upper: only j >= i; mirror: C(j, i) := C(i, j).
@@ -691,7 +838,7 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 struct OzPlan {
     int Mp, Np, Kp, kchunk, nchunks, ntiles;
     int bmo, bno;             // workgroup tile
-    bool dtl;                 // direct-to-LDS loads
+    int loads;                // 0 pipelined, 1 direct-to-LDS ring, 2 register-staged
     bool split, upper;
     size_t planesA, planesB, exps, maxes, acc, total;
 };
@@ -699,10 +846,14 @@ struct OzPlan {
 OzPlan oz_plan(int m, int n, int k, int s, bool gram)
 {
     OzPlan p;
-    // CA3DMM_OZAKI_TILE=256x128 selects the smaller workgroup tile,
-    // CA3DMM_OZAKI_LOADS=reg the register-staged loads instead of direct-to-LDS
+    // CA3DMM_OZAKI_TILE=256x128 selects the smaller workgroup tile;
+    // CA3DMM_OZAKI_LOADS selects the main loop: pipe (default: direct-to-LDS
+    // ring with fragment prefetch), dtl (direct-to-LDS ring) or reg
+    // (register-staged loads)
     const char *lcfg = getenv("CA3DMM_OZAKI_LOADS");
-    p.dtl = !(lcfg && strcmp(lcfg, "reg") == 0);
+    p.loads = 0;
+    if (lcfg && strcmp(lcfg, "dtl") == 0) p.loads = 1;
+    if (lcfg && strcmp(lcfg, "reg") == 0) p.loads = 2;
     const char *tcfg = getenv("CA3DMM_OZAKI_TILE");
     p.bmo = 256;
     p.bno = (tcfg && strcmp(tcfg, "256x128") == 0) ? 128 : 256;
@@ -790,7 +941,10 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         g.p32plane = (int64_t) Mp * Np;
         const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
         const unsigned nwg = (unsigned) (slots * 8);
-        if (pl.dtl) {
+        if (pl.loads == 0) {
+            if (pl.bno == 256) ozaki_pipe_kernel<true, 128, 128><<<nwg, 256, 0, st>>>(g);
+            else ozaki_pipe_kernel<true, 128, 64><<<nwg, 256, 0, st>>>(g);
+        } else if (pl.loads == 1) {
             if (pl.bno == 256) ozaki_dtl_kernel<true, 128, 128, 4><<<nwg, 256, 0, st>>>(g);
             else ozaki_dtl_kernel<true, 128, 64, 4><<<nwg, 256, 0, st>>>(g);
         } else {
@@ -811,7 +965,10 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
             g.nkb = std::min(pl.kchunk, Kp - c * pl.kchunk) / kKb;
             g.first = (d == s - 1 && c == 0);
             g.last = (d == 0 && c == pl.nchunks - 1);
-            if (pl.dtl) {
+            if (pl.loads == 0) {
+                if (pl.bno == 256) ozaki_pipe_kernel<false, 128, 128><<<grid, 256, 0, st>>>(g);
+                else ozaki_pipe_kernel<false, 128, 64><<<grid, 256, 0, st>>>(g);
+            } else if (pl.loads == 1) {
                 if (pl.bno == 256) ozaki_dtl_kernel<false, 128, 128, 4><<<grid, 256, 0, st>>>(g);
                 else ozaki_dtl_kernel<false, 128, 64, 4><<<grid, 256, 0, st>>>(g);
             } else {

@@ -10,7 +10,7 @@ from oracle import ozaki as Z
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("loads", ["dtl", "reg"])
+@pytest.mark.parametrize("loads", ["pipe", "dtl", "reg"])
 @pytest.mark.parametrize("tile", ["256x256", "256x128"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("m,n,k,s", [(1, 1, 1, 1), (17, 33, 65, 3), (128, 128, 64, 8),
@@ -79,7 +79,7 @@ def test_ozaki_split_mode_bit_exact(hip, monkeypatch, ta, tb, m, n, k, s):
     assert np.array_equal(dC.numpy(), want)
 
 
-@pytest.mark.parametrize("loads", ["dtl", "reg"])
+@pytest.mark.parametrize("loads", ["pipe", "dtl", "reg"])
 @pytest.mark.parametrize("tile", ["256x256", "256x128"])
 @pytest.mark.parametrize("n,k,s,mirror", [(1, 5, 3, True), (100, 20000, 12, True), (300, 1000, 8, False),
                                           (513, 700, 8, True)])
