@@ -371,7 +371,7 @@ __device__ __forceinline__ void oz_zero(i32x4 (&acc)[FI][FJ])
 // n = 4 * (l / 16) + r (operands fed as (B, A)).
 template <bool SPLIT, int WTM, int WTN, int FI, int FJ>
 __device__ __forceinline__ void oz_epilogue(const OzArgs &g, const i32x4 (&acc)[FI][FJ], int m0, int n0, int d,
-                                            int64_t part_off)
+                                            int64_t part_off, bool first, bool last)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -405,8 +405,8 @@ __device__ __forceinline__ void oz_epilogue(const OzArgs &g, const i32x4 (&acc)[
                 if (col >= g.N) continue;
                 double *wp = g.W + (int64_t) col * g.Mp + row;
                 double v = (double) acc[i][j][r] * w;
-                if (!g.first) v = *wp + v;
-                if (!g.last) {
+                if (!first) v = *wp + v;
+                if (!last) {
                     *wp = v;
                     continue;
                 }
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
         }
     }
 
-    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
+    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, g.first != 0, g.last != 0);
 }
 
 // Direct-to-LDS form: global_load_lds_dwordx4 writes lane l's 16 bytes at
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(Oz
     // the repeated loads of the last stage may still be landing in LDS:
     // drain them before the workgroup can end
     __builtin_amdgcn_s_waitcnt(0);
-    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
+    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, g.first != 0, g.last != 0);
 }
 
 // Software-pipelined form (the default).  Direct-to-LDS loads fill a ring
@@ -658,9 +658,16 @@ __device__ __forceinline__ void oz_mfmas(const i32x4 (&af)[FI], const i32x4 (&bf
         for (int j = 0; j < FJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af[i], acc[i][j], 0, 0, 0);
 }
 
-template <bool SPLIT, int WTM, int WTN>
+// FUSED (per-diagonal form only): one launch for all diagonals; each
+// workgroup walks its tile's (diagonal, k-chunk) segments in the order of
+// the specification (d = s-1 .. 0, chunks ascending), folding each into W
+// (the last into C), so every element sees the same FP64 additions as with
+// one launch per segment, while one tile's W traffic overlaps the other
+// tiles' MFMAs and only one launch tail remains.
+template <bool SPLIT, int WTM, int WTN, bool FUSED>
 __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
 {
+    static_assert(!(SPLIT && FUSED), "the split form has its own fold");
     constexpr int NBUF = 4;
     constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;
     constexpr int FI = WTM / 16, FJ = WTN / 16;
@@ -683,6 +690,17 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
     const int8_t *gA = g.PA + (int64_t) (m0 + wave * WA * 16 + lr) * g.Kp + lc;
     const int8_t *gB = g.PB + (int64_t) (n0 + wave * WB * 16 + lr) * g.Kp + lc;
     const int64_t blk16 = (int64_t) 16 * g.Kp;
+    const int nseg = FUSED ? g.s * g.nchunks : 1;
+    for (int sg = 0; sg < nseg; ++sg) {
+    bool first = g.first != 0, last = g.last != 0;
+    if (FUSED) {
+        d = g.s - 1 - sg / g.nchunks;
+        const int c = sg - (sg / g.nchunks) * g.nchunks;
+        kb0 = c * g.kcb;
+        nkb = min(g.kcb, g.Kp / kKb - kb0);
+        first = (sg == 0);
+        last = (sg == nseg - 1);
+    }
     const int total = (d + 1) * nkb;          // 64-byte stages; a multiple of NBUF
 
     auto slotA = [&](auto J) -> int8_t * {
@@ -771,9 +789,11 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
         step(z + 3, I3{}, af1, bf1, af0, bf0);
     }
     // the loads and fragment reads past the last stage were repeats of it;
-    // drain the loads before the workgroup can end
+    // drain the loads before the workgroup can end (or reuse the ring)
     __builtin_amdgcn_s_waitcnt(0);
-    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off);
+    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, first, last);
+    if (FUSED) __syncthreads();               // every wave is done with the ring
+    }
 }
 
 // Split mode, step 2: per element, the INT32 partials added in FP64 in the
@@ -942,8 +962,8 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
         const unsigned nwg = (unsigned) (slots * 8);
         if (pl.loads == 0) {
-            if (pl.bno == 256) ozaki_pipe_kernel<true, 128, 128><<<nwg, 256, 0, st>>>(g);
-            else ozaki_pipe_kernel<true, 128, 64><<<nwg, 256, 0, st>>>(g);
+            if (pl.bno == 256) ozaki_pipe_kernel<true, 128, 128, false><<<nwg, 256, 0, st>>>(g);
+            else ozaki_pipe_kernel<true, 128, 64, false><<<nwg, 256, 0, st>>>(g);
         } else if (pl.loads == 1) {
             if (pl.bno == 256) ozaki_dtl_kernel<true, 128, 128, 4><<<nwg, 256, 0, st>>>(g);
             else ozaki_dtl_kernel<true, 128, 64, 4><<<nwg, 256, 0, st>>>(g);
@@ -958,6 +978,14 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
     }
     g.W = (double *) accp;
     const int grid = (Mp / pl.bmo) * (Np / pl.bno);
+    if (pl.loads == 0) {      // all segments in one launch
+        g.s = s;
+        g.nchunks = pl.nchunks;
+        g.kcb = pl.kchunk / kKb;
+        if (pl.bno == 256) ozaki_pipe_kernel<false, 128, 128, true><<<grid, 256, 0, st>>>(g);
+        else ozaki_pipe_kernel<false, 128, 64, true><<<grid, 256, 0, st>>>(g);
+        return ca3::finish();
+    }
     for (int d = s - 1; d >= 0; --d) {
         for (int c = 0; c < pl.nchunks; ++c) {
             g.d = d;
@@ -965,10 +993,7 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
             g.nkb = std::min(pl.kchunk, Kp - c * pl.kchunk) / kKb;
             g.first = (d == s - 1 && c == 0);
             g.last = (d == 0 && c == pl.nchunks - 1);
-            if (pl.loads == 0) {
-                if (pl.bno == 256) ozaki_pipe_kernel<false, 128, 128><<<grid, 256, 0, st>>>(g);
-                else ozaki_pipe_kernel<false, 128, 64><<<grid, 256, 0, st>>>(g);
-            } else if (pl.loads == 1) {
+            if (pl.loads == 1) {
                 if (pl.bno == 256) ozaki_dtl_kernel<false, 128, 128, 4><<<grid, 256, 0, st>>>(g);
                 else ozaki_dtl_kernel<false, 128, 64, 4><<<grid, 256, 0, st>>>(g);
             } else {
