@@ -424,6 +424,52 @@ __device__ __forceinline__ void oz_epilogue(const OzArgs &g, const i32x4 (&acc)[
 }
 
 
+// W-only epilogue of the fused form: the whole padded tile (W is Mp x Np),
+// W = D * 2^-8(d+2) for the first segment, W + D * 2^-8(d+2) afterwards.
+template <int WTM, int WTN, int FI, int FJ>
+__device__ __forceinline__ void oz_epilogue_w(const OzArgs &g, const i32x4 (&acc)[FI][FJ], int m0, int n0, int d,
+                                              bool first)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, kq = lane >> 4;
+    const double w = ldexp(1.0, -8 * (d + 2));
+#pragma unroll
+    for (int i = 0; i < FI; ++i) {
+        const int row = m0 + wm * WTM + i * 16 + li;
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
+                double *wp = g.W + (int64_t) col * g.Mp + row;
+                const double v = (double) acc[i][j][r] * w;
+                *wp = first ? v : *wp + v;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// C := alpha * 2^EA_i 2^EB_j W + beta * C on the m x n result (the same
+// operations as the last launch of the per-segment form).
+__global__ __launch_bounds__(256) void ozaki_finalize_kernel(const double *__restrict__ W, int64_t ldw, int M, int N,
+                                                              const int *__restrict__ eA, const int *__restrict__ eB,
+                                                              double alpha, double beta, double *C, int64_t ldc)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    for (int j = blockIdx.y; j < N; j += gridDim.y) {
+        if (i >= M) return;
+        const int ea = eA[i], eb = eB[j];
+        double v = W[(int64_t) j * ldw + i];
+        v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
+        double *cp = C + (int64_t) j * ldc + i;
+        v = __dmul_rn(alpha, v);
+        if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *cp));
+        *cp = v;
+    }
+}
+
 // Register-staged form: global -> registers -> LDS, two LDS buffers.
 template <bool SPLIT, int WTM, int WTN, int KS>
 __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(OzArgs g)
@@ -678,7 +724,8 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
     __shared__ __attribute__((aligned(1024))) int8_t A0[kBMo * kKb], A1[kBMo * kKb], A2[kBMo * kKb], A3[kBMo * kKb];
     __shared__ __attribute__((aligned(1024))) int8_t B0[kBNo * kKb], B1[kBNo * kKb], B2[kBNo * kKb], B3[kBNo * kKb];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);     // uniform: LDS targets stay scalar
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn, d, kb0, nkb;
     int64_t part_off;
@@ -691,6 +738,7 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
     const int8_t *gB = g.PB + (int64_t) (n0 + wave * WB * 16 + lr) * g.Kp + lc;
     const int64_t blk16 = (int64_t) 16 * g.Kp;
     const int nseg = FUSED ? g.s * g.nchunks : 1;
+    int it = 0, ib = 0, inext = 0;
     for (int sg = 0; sg < nseg; ++sg) {
     bool first = g.first != 0, last = g.last != 0;
     if (FUSED) {
@@ -699,9 +747,11 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
         kb0 = c * g.kcb;
         nkb = min(g.kcb, g.Kp / kKb - kb0);
         first = (sg == 0);
-        last = (sg == nseg - 1);
     }
     const int total = (d + 1) * nkb;          // 64-byte stages; a multiple of NBUF
+    it = 0;
+    ib = 0;
+    inext = 0;
 
     auto slotA = [&](auto J) -> int8_t * {
         if constexpr (J.value == 0) return A0;
@@ -715,8 +765,15 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
         else if constexpr (J.value == 2) return B2;
         else return B3;
     };
-    auto issue = [&](int z, auto J) __attribute__((always_inline)) {
-        const int t = z / nkb, b = z - t * nkb;
+    // issue pointer: (pair it, block ib) of the next stage to load, advanced
+    // by one stage per issue and held at the last stage past the end
+    auto issue = [&](int /*z*/, auto J) __attribute__((always_inline)) {
+        const int t = it, b = ib;
+        const bool more = (inext + 1 < total);
+        inext = more ? inext + 1 : inext;
+        const int nb = ib + 1;
+        ib = more ? (nb == nkb ? 0 : nb) : ib;
+        it = more ? (nb == nkb ? it + 1 : it) : it;
         const int64_t koff = (int64_t) (kb0 + b) * kKb;
         const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
         const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
@@ -764,22 +821,32 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
     auto step = [&](int z, auto J, i32x4 (&afc)[FI], i32x4 (&bfc)[FJ], i32x4 (&afn)[FI], i32x4 (&bfn)[FJ])
         __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
-        // scheduling barriers keep the order: barrier, loads, fragment reads
-        // of stage z+1, MFMAs of stage z (MFMAs touch only registers, so the
-        // compiler would otherwise move them across the LDS barrier and wait
-        // on the fragment reads it was meant to hide)
+        // scheduling barriers keep everything of stage z after the LDS
+        // barrier (MFMAs touch only registers, so the compiler would
+        // otherwise move them across it)
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(kWaitL);
         sync(0);
-        issue(min(z + 3, total - 1), std::integral_constant<int, (j + 3) % 4>{});
+        __builtin_amdgcn_sched_barrier(0);
+        issue(z + 3, std::integral_constant<int, (j + 3) % 4>{});
         using JN = std::integral_constant<int, (j + 1) % 4>;
         oz_frags(slotA(JN{}), slotB(JN{}), wm, wn, li, kq, afn, bfn);
-        __builtin_amdgcn_sched_barrier(0);
         oz_mfmas(afc, bfc, acc);
-        // the fragment reads had the whole MFMA block to land: settle them
-        // here, where it costs nothing, so the next step's MFMAs start
-        // without a wait (lgkmcnt(0); vmcnt and expcnt not waited on)
-        __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4));
+        // One wave per SIMD: nothing else fills the MFMA pipe, so the reads
+        // of stage z+1 and the loads of stage z+3 are threaded between the
+        // MFMAs of stage z (one after each MFMA), then the rest of the MFMAs
+        // run back to back.  Masks: 0x008 MFMA, 0x100 LDS read, 0x020 VMEM read.
+#pragma unroll
+        for (int q = 0; q < FI + FJ; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < L; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, FI * FJ - FI - FJ - L, 0);
         __builtin_amdgcn_sched_barrier(0);
     };
     for (int z = 0; z < total; z += 4) {
@@ -791,8 +858,12 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
     // the loads and fragment reads past the last stage were repeats of it;
     // drain the loads before the workgroup can end (or reuse the ring)
     __builtin_amdgcn_s_waitcnt(0);
-    oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, first, last);
-    if (FUSED) __syncthreads();               // every wave is done with the ring
+    if (FUSED) {
+        oz_epilogue_w<WTM, WTN>(g, acc, m0, n0, d, first);
+        __syncthreads();                      // every wave is done with the ring
+    } else {
+        oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, first, last);
+    }
     }
 }
 
@@ -984,6 +1055,8 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         g.kcb = pl.kchunk / kKb;
         if (pl.bno == 256) ozaki_pipe_kernel<false, 128, 128, true><<<grid, 256, 0, st>>>(g);
         else ozaki_pipe_kernel<false, 128, 64, true><<<grid, 256, 0, st>>>(g);
+        ozaki_finalize_kernel<<<dim3((m + 255) / 256, std::min(n, 65535)), 256, 0, st>>>(g.W, Mp, m, n, eA, eB, alpha,
+                                                                                       beta, C, ldC);
         return ca3::finish();
     }
     for (int d = s - 1; d >= 0; --d) {
