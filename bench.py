@@ -22,6 +22,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 MFMA
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.txt")
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the committed counter
+    summary (FETCH_SIZE / WRITE_SIZE passes, scaled by a plain copy of known
+    size); None when absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                kv = dict(x.split("=", 1) for x in line.split() if "=" in x)
+                if kv.get("kernel") == kernel and int(kv.get("n", 0)) == n:
+                    return int(kv["read_bytes"]) + int(kv["write_bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
 METRIC = "FP64-accurate GEMM TFLOP/s: DGEMM-MFMA vs INT8-Ozaki at equal max rel err"
 
 
@@ -30,7 +46,10 @@ def log(msg):
 
 
 def ozaki_at_equal_error(hip, n, rnd):
-    """Native FP64-MFMA DGEMM vs Ozaki INT8 on phi = 0.5 inputs at n^3."""
+    """Native FP64-MFMA DGEMM vs Ozaki INT8 on phi = 0.5 inputs at n^3: the
+    smallest slice count whose errors (max |C - C_dd| / (|A||B|) and max
+    relative error, on sampled entries against double-double) are both no
+    worse than the native product's."""
     import numpy as np
 
     from oracle import oracle as O
@@ -39,11 +58,12 @@ def ozaki_at_equal_error(hip, n, rnd):
     B = Z.phi_matrix(rnd, (n, n), 0.5)
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
     dC = hip.DeviceArray.zeros((n, n))
-    ii, jj = rnd.integers(0, n, 384), rnd.integers(0, n, 384)
+    ii, jj = rnd.integers(0, n, 4096), rnd.integers(0, n, 4096)
     ref, ab = O.dd_sample(0, 0, n, n, n, np.asfortranarray(A), n, np.asfortranarray(B), n, ii, jj)
+    del A, B
     timer = hip.Timer()
 
-    def timed(f, reps=2):
+    def timed(f, reps=3):
         f()
         hip.sync()
         timer.start()
@@ -51,23 +71,30 @@ def ozaki_at_equal_error(hip, n, rnd):
             f()
         return timer.stop() / reps
 
+    def errs():
+        got = dC.sample(ii, jj)
+        e = np.abs(got - ref)
+        return float(np.max(e / ab)), float(np.max(e / np.abs(ref)))
+
     hip.set_sync(False)
     nat_ms = timed(lambda: hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC))
-    nat_err = float(np.max(np.abs(dC.numpy()[ii, jj] - ref) / ab))
+    nat_ab, nat_rel = errs()
     tried = []
     best = None
     for s in range(6, 12):
         ms = timed(lambda: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, s))
-        err = float(np.max(np.abs(dC.numpy()[ii, jj] - ref) / ab))
-        tried.append({"s": s, "ms": round(ms, 3), "max_err_over_absAabsB": err})
-        if err <= nat_err:
-            best = {"s": s, "ms": round(ms, 3), "max_err_over_absAabsB": err}
+        e_ab, e_rel = errs()
+        rec = {"s": s, "ms": round(ms, 3), "max_err_over_absAabsB": e_ab, "max_rel_err": e_rel}
+        tried.append(rec)
+        if e_ab <= nat_ab and e_rel <= nat_rel:
+            best = dict(rec)
             break
     hip.set_sync(True)
     flops = 2.0 * n ** 3
-    out = {"inputs": "uniform[-0.5,0.5]*exp(0.5*N(0,1)), phi=0.5", "m": n, "n": n, "k": n,
-           "native_ms": round(nat_ms, 3), "native_tflops": round(flops / nat_ms / 1e9, 3),
-           "native_max_err_over_absAabsB": nat_err, "tried": tried}
+    out = {"inputs": "uniform[-0.5,0.5]*exp(0.5*N(0,1)), phi=0.5", "m": n, "n": n, "k": n, "samples": 4096,
+           "oracle": "double-double dot products", "native_ms": round(nat_ms, 3),
+           "native_tflops": round(flops / nat_ms / 1e9, 3), "native_max_err_over_absAabsB": nat_ab,
+           "native_max_rel_err": nat_rel, "tried": tried}
     if best:
         best["effective_tflops"] = round(flops / best["ms"] / 1e9, 3)
         best["int8_tops"] = round(best["s"] * (best["s"] + 1) / 2 * flops / best["ms"] / 1e9, 1)
@@ -198,7 +225,7 @@ def main():
                    "m": n, "n": n, "k": n, "method": "dgemm-mfma"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "dgemm_kernel",
+                     "traffic": measured_traffic("dgemm_kernel", n), "kernel": "dgemm_kernel",
                      "avg_launch_ms": round(gpu_ms / launches, 4)},
         "accuracy": err_stats,
         "ozaki": ozaki,

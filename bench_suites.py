@@ -245,9 +245,10 @@ def cholqr(args):
 # ---------------------------------------------------------------------------
 def pmc_calib(args):
     """Counter-run program: a plain copy of 1 GiB (the scale of FETCH_SIZE /
-    WRITE_SIZE), then one launch of each hot kernel at its bench size: NN
-    DGEMM n^3, the kp = 3 fold over 2000 x 1334, the pack of A for the 24
-    ranks of the 4x2x3 grid on 8000^3, and an 8-slice Ozaki product n^3."""
+    WRITE_SIZE), then one launch of each hot kernel at its bench size: the
+    four DGEMM trans cases at n^3, the kp = 3 fold over 2000 x 1334, the pack
+    of A for the 24 ranks of the 4x2x3 grid on 8000^3, and a 7-slice Ozaki
+    product n^3."""
     from oracle import oracle as O
     nel = (1 << 30) // 8
     src = hip.DeviceArray.zeros((nel,))
@@ -258,8 +259,10 @@ def pmc_calib(args):
     n = args.n
     rnd = np.random.default_rng(5)
     dA = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+    dB = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
     dC = hip.DeviceArray.zeros((n, n))
-    hip.dgemm(0, 0, n, n, n, 1.0, dA, dA, 0.0, dC)
+    for ta, tb in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
     P = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (2000, 1334))) for _ in range(3)]
     out = hip.DeviceArray.zeros((2000, 1334))
     hip.fold_sum(P, 2000, 1334, out)
@@ -276,11 +279,11 @@ def pmc_calib(args):
                          e[3], e[2], e[3]))
     check_blocks(pack, [Afull] + sends)
     hip.copy_blocks(8, pack)
-    hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dA, 0.0, dC, 8)
+    hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 7)
     hip.sync()
     emit({"suite": "pmc-calib", "copy_bytes_read": nel * 8, "copy_bytes_written": nel * 8,
-          "dgemm_n": n, "fold": "3 x 2000x1334", "pack_elements": sum(d[4] * d[5] for d in pack),
-          "ozaki": f"{n}^3 s=8"})
+          "dgemm_n": n, "dgemm_cases": "NN,NT,TN,TT", "fold": "3 x 2000x1334",
+          "pack_elements": sum(d[4] * d[5] for d in pack), "ozaki": f"{n}^3 s=7"})
 
 
 def yardstick(args):
