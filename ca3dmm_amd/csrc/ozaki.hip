@@ -710,14 +710,22 @@ __device__ __forceinline__ void oz_mfmas(const i32x4 (&af)[FI], const i32x4 (&bf
 // (the last into C), so every element sees the same FP64 additions as with
 // one launch per segment, while one tile's W traffic overlaps the other
 // tiles' MFMAs and only one launch tail remains.
-template <bool SPLIT, int WTM, int WTN, bool FUSED>
-__global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
+// NW waves per workgroup in a 2 x NW/2 grid (wave w: rows (w & 1) * WTM,
+// columns (w >> 1) * WTN); with NW = 8 two waves share each SIMD and cover
+// each other's waits.
+template <bool SPLIT, int WTM, int WTN, bool FUSED, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
 {
     static_assert(!(SPLIT && FUSED), "the split form has its own fold");
     constexpr int NBUF = 4;
-    constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;
+    constexpr int kBMo = 2 * WTM, kBNo = (NW / 2) * WTN;
     constexpr int FI = WTM / 16, FJ = WTN / 16;
-    constexpr int WA = kBMo / 64, WB = kBNo / 64;       // 1 KB blocks per wave per stage
+    constexpr int WA = kBMo / (16 * NW), WB = kBNo / (16 * NW);  // 1 KB blocks per wave per stage
+    static_assert(WA * 16 * NW == kBMo && WB * 16 * NW == kBNo && WA > 0 && WB > 0, "load split");
+    // 4 waves: fragments of stage z+1 prefetched into a second register set;
+    // 8 waves: no room for it (128 VGPRs per wave), the other wave of the
+    // SIMD covers the fragment reads
+    constexpr bool PREFETCH = (NW == 4);
     constexpr int L = WA + WB;
     static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
     static_assert((kKp / kKb) % NBUF == 0 && L <= 63, "ring turns per chunk / vmcnt range");
@@ -811,13 +819,16 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
     issue(0, I0{});
     issue(min(1, total - 1), I1{});
     issue(min(2, total - 1), I2{});
-    __builtin_amdgcn_s_waitcnt(kWaitL2);      // own loads of stage 0 done
-    sync(0);
-    oz_frags(A0, B0, wm, wn, li, kq, af0, bf0);
+    if constexpr (PREFETCH) {
+        __builtin_amdgcn_s_waitcnt(kWaitL2);  // own loads of stage 0 done
+        sync(0);
+        oz_frags(A0, B0, wm, wn, li, kq, af0, bf0);
+    }
 
-    // stage z (slot j = z % 4, fragments in set z % 2): publish stage z+1,
-    // refill slot (z-1) % 4 with stage z+3, read the fragments of stage z+1,
-    // run the MFMAs of stage z
+    // stage z (slot j = z % 4).  PREFETCH: publish stage z+1, refill slot
+    // (z-1) % 4 with stage z+3, read the fragments of stage z+1 into set
+    // (z+1) % 2, run the MFMAs of stage z from set z % 2.  Otherwise: publish
+    // stage z, refill slot (z-1) % 4, read and run stage z.
     auto step = [&](int z, auto J, i32x4 (&afc)[FI], i32x4 (&bfc)[FJ], i32x4 (&afn)[FI], i32x4 (&bfn)[FJ])
         __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
@@ -825,28 +836,35 @@ __global__ __launch_bounds__(256, 1) void ozaki_pipe_kernel(OzArgs g)
         // barrier (MFMAs touch only registers, so the compiler would
         // otherwise move them across it)
         __builtin_amdgcn_sched_barrier(0);
-        __builtin_amdgcn_s_waitcnt(kWaitL);
+        __builtin_amdgcn_s_waitcnt(PREFETCH ? kWaitL : kWaitL2);
         sync(0);
         __builtin_amdgcn_sched_barrier(0);
         issue(z + 3, std::integral_constant<int, (j + 3) % 4>{});
-        using JN = std::integral_constant<int, (j + 1) % 4>;
-        oz_frags(slotA(JN{}), slotB(JN{}), wm, wn, li, kq, afn, bfn);
-        oz_mfmas(afc, bfc, acc);
+        if constexpr (PREFETCH) {
+            using JN = std::integral_constant<int, (j + 1) % 4>;
+            oz_frags(slotA(JN{}), slotB(JN{}), wm, wn, li, kq, afn, bfn);
+            oz_mfmas(afc, bfc, acc);
+        } else {
+            oz_frags(slotA(J), slotB(J), wm, wn, li, kq, afc, bfc);
+            oz_mfmas(afc, bfc, acc);
+        }
         // One wave per SIMD: nothing else fills the MFMA pipe, so the reads
         // of stage z+1 and the loads of stage z+3 are threaded between the
         // MFMAs of stage z (one after each MFMA), then the rest of the MFMAs
         // run back to back.  Masks: 0x008 MFMA, 0x100 LDS read, 0x020 VMEM read.
+        if constexpr (PREFETCH) {
 #pragma unroll
-        for (int q = 0; q < FI + FJ; ++q) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        }
+            for (int q = 0; q < FI + FJ; ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            }
 #pragma unroll
-        for (int q = 0; q < L; ++q) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            for (int q = 0; q < L; ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, FI * FJ - FI - FJ - L, 0);
         }
-        __builtin_amdgcn_sched_group_barrier(0x008, FI * FJ - FI - FJ - L, 0);
         __builtin_amdgcn_sched_barrier(0);
     };
     for (int z = 0; z < total; z += 4) {
@@ -930,6 +948,7 @@ struct OzPlan {
     int Mp, Np, Kp, kchunk, nchunks, ntiles;
     int bmo, bno;             // workgroup tile
     int loads;                // 0 pipelined, 1 direct-to-LDS ring, 2 register-staged
+    int waves;                // pipelined form: waves per workgroup (4 or 8)
     bool split, upper;
     size_t planesA, planesB, exps, maxes, acc, total;
 };
@@ -945,6 +964,10 @@ OzPlan oz_plan(int m, int n, int k, int s, bool gram)
     p.loads = 0;
     if (lcfg && strcmp(lcfg, "dtl") == 0) p.loads = 1;
     if (lcfg && strcmp(lcfg, "reg") == 0) p.loads = 2;
+    // CA3DMM_OZAKI_WAVES=4 runs the pipelined form with 4 waves of 128 x 128
+    // instead of 8 waves of 128 x 64 per 256 x 256 tile
+    const char *wcfg = getenv("CA3DMM_OZAKI_WAVES");
+    p.waves = (wcfg && strcmp(wcfg, "4") == 0) ? 4 : 8;
     const char *tcfg = getenv("CA3DMM_OZAKI_TILE");
     p.bmo = 256;
     p.bno = (tcfg && strcmp(tcfg, "256x128") == 0) ? 128 : 256;
@@ -1033,8 +1056,9 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         const int64_t slots = (int64_t) ((g.ngroups + 7) / 8) * g.ntiles;
         const unsigned nwg = (unsigned) (slots * 8);
         if (pl.loads == 0) {
-            if (pl.bno == 256) ozaki_pipe_kernel<true, 128, 128, false><<<nwg, 256, 0, st>>>(g);
-            else ozaki_pipe_kernel<true, 128, 64, false><<<nwg, 256, 0, st>>>(g);
+            if (pl.bno == 256 && pl.waves == 8) ozaki_pipe_kernel<true, 128, 64, false, 8><<<nwg, 512, 0, st>>>(g);
+            else if (pl.bno == 256) ozaki_pipe_kernel<true, 128, 128, false, 4><<<nwg, 256, 0, st>>>(g);
+            else ozaki_pipe_kernel<true, 128, 64, false, 4><<<nwg, 256, 0, st>>>(g);
         } else if (pl.loads == 1) {
             if (pl.bno == 256) ozaki_dtl_kernel<true, 128, 128, 4><<<nwg, 256, 0, st>>>(g);
             else ozaki_dtl_kernel<true, 128, 64, 4><<<nwg, 256, 0, st>>>(g);
@@ -1053,8 +1077,9 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
         g.s = s;
         g.nchunks = pl.nchunks;
         g.kcb = pl.kchunk / kKb;
-        if (pl.bno == 256) ozaki_pipe_kernel<false, 128, 128, true><<<grid, 256, 0, st>>>(g);
-        else ozaki_pipe_kernel<false, 128, 64, true><<<grid, 256, 0, st>>>(g);
+        if (pl.bno == 256 && pl.waves == 8) ozaki_pipe_kernel<false, 128, 64, true, 8><<<grid, 512, 0, st>>>(g);
+        else if (pl.bno == 256) ozaki_pipe_kernel<false, 128, 128, true, 4><<<grid, 256, 0, st>>>(g);
+        else ozaki_pipe_kernel<false, 128, 64, true, 4><<<grid, 256, 0, st>>>(g);
         ozaki_finalize_kernel<<<dim3((m + 255) / 256, std::min(n, 65535)), 256, 0, st>>>(g.W, Mp, m, n, eA, eB, alpha,
                                                                                        beta, C, ldC);
         return ca3::finish();

@@ -38,6 +38,25 @@ def test_ozaki_bit_exact_vs_spec(hip, monkeypatch, loads, tile, ta, tb, m, n, k,
     assert np.array_equal(dC.numpy(), want)
 
 
+@pytest.mark.parametrize("waves", ["4", "8"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k,s", [(300, 520, 257, 7), (513, 260, 3000, 8), (40, 50, 20000, 16)])
+def test_ozaki_pipelined_wave_counts(hip, monkeypatch, waves, ta, tb, m, n, k, s):
+    """The pipelined form with 4 waves (fragment prefetch) and 8 waves per
+    256 x 256 tile, per-segment (fused) and split-K, against the specification."""
+    monkeypatch.setenv("CA3DMM_OZAKI_WAVES", waves)
+    rnd = np.random.default_rng(m + n + k + s + ta + int(waves))
+    opA, opB = Z.phi_matrix(rnd, (m, k), 2.0), Z.phi_matrix(rnd, (k, n), 2.0)
+    want = Z.ozaki_dgemm8(opA, opB, s)
+    dA = hip.DeviceArray.from_numpy(opA.T if ta else opA)
+    dB = hip.DeviceArray.from_numpy(opB.T if tb else opB)
+    dC = hip.DeviceArray.filled((m, n), 0xFF)
+    for split in ("0", "1"):
+        monkeypatch.setenv("CA3DMM_OZAKI_SPLIT", split)
+        hip.ozaki_dgemm(ta, tb, m, n, k, 1.0, dA, dB, 0.0, dC, s)
+        assert np.array_equal(dC.numpy(), want), split
+
+
 def test_ozaki_alpha_beta(hip):
     rnd = np.random.default_rng(5)
     m, n, k = 70, 90, 110
