@@ -235,15 +235,17 @@ struct OzArgs {
 };
 
 // 16-byte chunk swizzle within a 64-byte LDS row.  An MFMA operand read
-// (ds_read_b128) has lanes 0..15 on rows r..r+15 (r a multiple of 16) at one
-// chunk; with f(row) below those 16 reads fall on 16 distinct 16-byte slots
-// of a 256-byte bank line, and each half (8 lanes) on 8 distinct slots of a
-// 128-byte line, so the read is conflict-free whether the LDS serves 8 or 16
-// such lanes per cycle.  Stores (4 lanes per row) are permutations within rows.
+// (ds_read_b128) has lanes 16 q .. 16 q + 15 on rows r..r+15 (r a multiple of
+// 16) at logical chunk q.  Measured on gfx950 with SQ_LDS_BANK_CONFLICT over
+// this exact read pattern: f(row) = (row >> 1) & 3 is conflict-free (4 LDS
+// cycles per read, as a linear 1 KB read), while no swizzle, row & 3,
+// (row >> 2) & 3 and the 3-bit mixes tried all take 8 cycles (two-way).
+// Stores (4 lanes per row) are permutations within rows.
+__device__ __forceinline__ int oz_swz(int row) { return (row >> 1) & 3; }
+
 __device__ __forceinline__ int lds_off(int row, int chunk)
 {
-    const int f = (((row >> 1) ^ (row >> 3)) & 1) | (((row >> 2) & 1) << 1);
-    return row * 64 + ((chunk ^ f) << 4);
+    return row * 64 + ((chunk ^ oz_swz(row)) << 4);
 }
 
 // One pipeline stage = KS sub-blocks of 64 bytes of k; per sub-block PA (PB)
@@ -594,7 +596,7 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(Oz
 
     // this lane's part of every 1 KB block: row r = 16 * blk + lane / 4
     const int lr = lane >> 2;
-    const int lc = ((lane & 3) ^ ((((lr >> 1) ^ (lr >> 3)) & 1) | (((lr >> 2) & 1) << 1))) << 4;
+    const int lc = ((lane & 3) ^ oz_swz(lr)) << 4;
     const int8_t *gA = g.PA + (int64_t) (m0 + wave * WA * 16 + lr) * g.Kp + lc;
     const int8_t *gB = g.PB + (int64_t) (n0 + wave * WB * 16 + lr) * g.Kp + lc;
     const int64_t blk16 = (int64_t) 16 * g.Kp;
@@ -741,7 +743,7 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
     const int m0 = tm * kBMo, n0 = tn * kBNo;
 
     const int lr = lane >> 2;
-    const int lc = ((lane & 3) ^ ((((lr >> 1) ^ (lr >> 3)) & 1) | (((lr >> 2) & 1) << 1))) << 4;
+    const int lc = ((lane & 3) ^ oz_swz(lr)) << 4;
     const int8_t *gA = g.PA + (int64_t) (m0 + wave * WA * 16 + lr) * g.Kp + lc;
     const int8_t *gB = g.PB + (int64_t) (n0 + wave * WB * 16 + lr) * g.Kp + lc;
     const int64_t blk16 = (int64_t) 16 * g.Kp;
