@@ -68,6 +68,8 @@ SIGNATURES = {
     "hip_transpose_cm": (I, [I, I, P, I, P, I]),
     "hip_ozaki_dgemm": (I, [I, I, I, I, I, D, P, I, P, I, D, P, I, I]),
     "hip_ozaki_workspace_bytes": (SZ, [I, I, I, I]),
+    "hip_ozaki2_dgemm": (I, [I, I, I, I, I, D, P, I, P, I, D, P, I, I]),
+    "hip_ozaki2_workspace_bytes": (SZ, [I, I, I, I]),
     "hip_dsyrk_gram": (I, [I, I, D, P, I, D, P, I, I]),
     "hip_ozaki_gram": (I, [I, I, D, P, I, D, P, I, I, I]),
     "hip_dtrtri_upper": (I, [I, P, I, P, I]),

@@ -183,10 +183,10 @@ __global__ __launch_bounds__(256) void fold_sum_kernel(FoldArgs f, int nrow, int
 #pragma unroll
             for (int u = 0; u < kFoldU; ++u) {
                 if constexpr (V == 2) {
-                    acc[u].x = __dadd_rn(acc[u].x, x[u].x);
-                    acc[u].y = __dadd_rn(acc[u].y, x[u].y);
+                    acc[u].x = ca3::add_rn(acc[u].x, x[u].x);
+                    acc[u].y = ca3::add_rn(acc[u].y, x[u].y);
                 } else {
-                    acc[u] = __dadd_rn(acc[u], x[u]);
+                    acc[u] = ca3::add_rn(acc[u], x[u]);
                 }
             }
         }
@@ -239,10 +239,10 @@ __global__ __launch_bounds__(256) void geam_kernel(int m, int n, double alpha, c
         int gi = i0 + tx, gj = j0 + jj;
         if (gi >= m || gj >= n) continue;
         double a = ta ? ta_s[tx][jj] : A[(int64_t) gj * lda + gi];
-        double v = __dmul_rn(alpha, a);
+        double v = ca3::mul_rn(alpha, a);
         if (beta != 0.0) {
             double b = tb ? tb_s[tx][jj] : B[(int64_t) gj * ldb + gi];
-            v = __dadd_rn(v, __dmul_rn(beta, b));
+            v = ca3::add_rn(v, ca3::mul_rn(beta, b));
         }
         Cm[(int64_t) gj * ldc + gi] = v;
     }

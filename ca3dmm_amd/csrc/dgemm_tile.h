@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "hip_common.h"
+
 namespace ca3 {
 namespace gemm {
 
@@ -242,12 +244,12 @@ __device__ __forceinline__ void epilogue(const GemmArgs &g, int m0, int n0, cons
                 const int col = n0 + wn * 64 + j * 16 + 4 * r + (lane >> 4);
                 if (col >= g.N) continue;
                 double *c = g.C + (int64_t) col * g.ldc + row;
-                double v = __dmul_rn(g.alpha, acc[i][j][r]);
-                if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *c));
+                double v = ca3::mul_rn(g.alpha, acc[i][j][r]);
+                if (g.beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(g.beta, *c));
                 if (g.nsrc > 0) {
                     double s = g.P[0][(int64_t) col * g.ldp[0] + row];
-                    for (int t = 1; t < g.nsrc; ++t) s = __dadd_rn(s, g.P[t][(int64_t) col * g.ldp[t] + row]);
-                    v = __dadd_rn(s, v);
+                    for (int t = 1; t < g.nsrc; ++t) s = ca3::add_rn(s, g.P[t][(int64_t) col * g.ldp[t] + row]);
+                    v = ca3::add_rn(s, v);
                 }
                 *c = v;
             }

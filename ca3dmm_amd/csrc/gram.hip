@@ -83,10 +83,10 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restri
         const int gi = ti * kBM + row, gj = tj * kBN + col;
         if (gi >= n || gj >= n || gi > gj) continue;
         double s = P[(int64_t) pair * (kBM * kBN) + e];
-        for (int ks = 1; ks < ksplit; ++ks) s = __dadd_rn(s, P[((int64_t) ks * npairs + pair) * (kBM * kBN) + e]);
+        for (int ks = 1; ks < ksplit; ++ks) s = ca3::add_rn(s, P[((int64_t) ks * npairs + pair) * (kBM * kBN) + e]);
         double *c = C + (int64_t) gj * ldc + gi;
-        double v = __dmul_rn(alpha, s);
-        if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *c));
+        double v = ca3::mul_rn(alpha, s);
+        if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *c));
         *c = v;
         if (mirror && gi != gj) C[(int64_t) gi * ldc + gj] = v;
     }

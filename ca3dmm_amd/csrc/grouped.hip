@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Problem *__r
             if (row >= F.M || col >= F.N) continue;
             const int64_t off = (int64_t) col * F.ldl + row;
             double s = F.Cl[0][off];
-            for (int l = 1; l < F.kp; ++l) s = __dadd_rn(s, F.Cl[l][off]);
+            for (int l = 1; l < F.kp; ++l) s = ca3::add_rn(s, F.Cl[l][off]);
             int sl = 0;
             while (col >= F.cs[sl + 1]) ++sl;
             F.Cout[sl][(int64_t) (col - F.cs[sl]) * F.ldo + row] = s;

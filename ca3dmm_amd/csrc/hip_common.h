@@ -30,6 +30,22 @@ int finish();
 // Number of CUs of the current device (cached).
 int cu_count();
 
+// FP64 multiply and add rounded separately.  hipcc contracts a * b + c into
+// one fused multiply-add by default (also through the __dmul_rn / __dadd_rn
+// wrappers, which are plain operators); the epilogues whose rounding order
+// is part of the result use these instead.
+__device__ __forceinline__ double mul_rn(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+__device__ __forceinline__ double add_rn(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 }  // namespace ca3
 
 #define CA3_CHECK(expr)                                   \

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "ca3dmm_ext.h"
 #include "hip_common.h"
@@ -232,6 +233,9 @@ struct OzArgs {
     int ngroups;            // s * nchunks
     int *P32;               // INT32 partials, one Mp x Np plane per (d, c)
     int64_t p32plane;
+    // CRT form: residue planes R_t (int8, Mp x Np, ld Mp), s = number of moduli
+    int8_t *R8;
+    int64_t rplane;
 };
 
 // 16-byte chunk swizzle within a 64-byte LDS row.  An MFMA operand read
@@ -415,8 +419,8 @@ __device__ __forceinline__ void oz_epilogue(const OzArgs &g, const i32x4 (&acc)[
                 const int eb = g.eB[col];
                 v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
                 double *cp = g.C + (int64_t) col * g.ldc + row;
-                v = __dmul_rn(g.alpha, v);
-                if (g.beta != 0.0) v = __dadd_rn(v, __dmul_rn(g.beta, *cp));
+                v = ca3::mul_rn(g.alpha, v);
+                if (g.beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(g.beta, *cp));
                 *cp = v;
             }
         }
@@ -466,8 +470,8 @@ __global__ __launch_bounds__(256) void ozaki_finalize_kernel(const double *__res
         double v = W[(int64_t) j * ldw + i];
         v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
         double *cp = C + (int64_t) j * ldc + i;
-        v = __dmul_rn(alpha, v);
-        if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *cp));
+        v = ca3::mul_rn(alpha, v);
+        if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
         *cp = v;
     }
 }
@@ -677,6 +681,304 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(Oz
     oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, g.first != 0, g.last != 0);
 }
 
+// ---------------------------------------------------------------------
+// CRT form (Ozaki scheme II): one INT8 product per modulus
+// ---------------------------------------------------------------------
+// Lines are scaled by 2^e to integers a' = rint(x 2^e) with ||a'||_2 <= 2^T,
+// 2^(2T+1) < P = p_0 ... p_{N-1}, so |C'| < P/2 is fixed by its residues.
+// Residue planes (balanced int8) go through the same pipelined INT8 kernel,
+// one segment per (modulus, k-chunk), whose epilogue keeps Z mod p as int8
+// planes R_t (ld Mp); ozaki2_crt_kernel turns the N residues of an element
+// into balanced mixed-radix digits (Garner) and evaluates them in FP64 by
+// Horner from the top digit (rounded multiply, then rounded add).
+constexpr int kCrtMax = 20;
+constexpr int kCrtMin = 4;
+constexpr int kCrtKMax = 1 << 23;   // the integer norm sums stay below 2^63
+// INT32-exact k-chunk: kchunk * 128 * 128 + 128 < 2^31, a multiple of kKp
+constexpr int kCrtChunk = (int) ((((1u << 31) - 1 - 128) / 16384u) / kKp * kKp);
+
+constexpr int kCrtModuli[kCrtMax] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                     223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+// T(N) = (bitlen(P) - 2) / 2 for the first N moduli (N = 0 .. 20)
+constexpr int kCrtT[kCrtMax + 1] = {0, 3, 7, 11, 15, 19, 23, 27, 31, 35, 39,
+                                    43, 46, 50, 54, 58, 62, 65, 69, 73, 77};
+
+constexpr int crt_inv(int a, int p)                 // a^-1 mod p (extended Euclid)
+{
+    int t = 0, nt = 1, r = p, nr = a % p;
+    while (nr != 0) {
+        const int q = r / nr, t2 = t - q * nt, r2 = r - q * nr;
+        t = nt; nt = t2; r = nr; nr = r2;
+    }
+    return t < 0 ? t + p : t;
+}
+
+constexpr int crt_pow2(int e, int p)                // 2^e mod p, balanced
+{
+    int r = 1;
+    for (int i = 0; i < e; ++i) r = (2 * r) % p;
+    return r > p / 2 ? r - p : r;
+}
+
+struct CrtTables {
+    float p[kCrtMax], invp[kCrtMax];                 // modulus, 1/p (rounded)
+    double pd[kCrtMax], invpd[kCrtMax];
+    float ginv[kCrtMax][kCrtMax];                    // p_j^-1 mod p_t, j < t
+    int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax];    // 2^21, 2^42, 2^63 mod p (balanced)
+};
+
+constexpr CrtTables crt_tables()
+{
+    CrtTables c{};
+    for (int t = 0; t < kCrtMax; ++t) {
+        const int p = kCrtModuli[t];
+        c.p[t] = (float) p;
+        c.invp[t] = 1.0f / (float) p;
+        c.pd[t] = (double) p;
+        c.invpd[t] = 1.0 / (double) p;
+        c.c21[t] = crt_pow2(21, p);
+        c.c42[t] = crt_pow2(42, p);
+        c.c63[t] = crt_pow2(63, p);
+        for (int j = 0; j < kCrtMax; ++j) c.ginv[t][j] = j < t ? (float) crt_inv(kCrtModuli[j] % p, p) : 0.0f;
+    }
+    return c;
+}
+
+__constant__ CrtTables c_crt = crt_tables();
+
+// Balanced residue of an integer-valued y (|y| < 2^45) modulo p: the
+// quotient rint(y / p) from the rounded reciprocal is exact or off by one,
+// and the two corrections bring r into [-(p-1)/2, (p-1)/2] ([-128, 127]
+// for p = 256).
+__device__ __forceinline__ double crt_bal(double y, double p, double invp)
+{
+    double r = fma(-rint(y * invp), p, y);
+    const double hi = floor((p - 1.0) * 0.5);
+    r = r > hi ? r - p : r;
+    r = r < hi - p + 1.0 ? r + p : r;
+    return r;
+}
+
+// Same in FP32 for |y| < 2^17 (Garner steps).
+__device__ __forceinline__ float crt_balf(float y, float p, float invp)
+{
+    float r = fmaf(-rintf(y * invp), p, y);
+    const float hi = floorf((p - 1.0f) * 0.5f);
+    r = r > hi ? r - p : r;
+    r = r < hi - p + 1.0f ? r + p : r;
+    return r;
+}
+
+// Per line: S = sum over elements of ceil(m4^2 2^(2(ex - E) + 32)), with E
+// the frexp exponent of the line's max |x| (from ozaki_max_kernel), ex that
+// of the element and m4 = ceil(16 |frac|); x^2 <= 4^E 2^-40 * term.  Integer
+// atomics: the sum does not depend on the order.
+__device__ __forceinline__ unsigned long long crt_norm_term(double x, int E)
+{
+    if (x == 0.0) return 0ull;
+    int ex;
+    const double f = frexp(fabs(x), &ex);
+    const unsigned long long m2 = (unsigned long long) ceil(f * 16.0) * (unsigned long long) ceil(f * 16.0);
+    const int sh = 2 * (ex - E) + 32;                 // <= 32: ex <= E
+    if (sh >= 0) return m2 << sh;
+    return ((m2 - 1) >> min(-sh, 62)) + 1;
+}
+
+__global__ __launch_bounds__(256) void ozaki2_norm_kernel(const double *__restrict__ X, int lines, int K,
+                                                           int64_t sl, int64_t sk,
+                                                           const unsigned long long *__restrict__ mx,
+                                                           unsigned long long *S)
+{
+    const bool line_contig = (sl == 1);
+    const int t = threadIdx.x;
+    const int kchunk = (K + gridDim.y - 1) / gridDim.y;
+    const int k0 = blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+    const int l0 = blockIdx.x * 64;
+    __shared__ unsigned long long red[256];
+    unsigned long long acc = 0;
+    const int myline = line_contig ? l0 + (t & 63) : l0 + (t >> 2);
+    if (myline < lines) {
+        int E;
+        (void) frexp(__longlong_as_double((long long) mx[myline]), &E);
+        if (line_contig) {
+            for (int kk = k0 + (t >> 6); kk < k1; kk += 4) acc += crt_norm_term(X[myline + (int64_t) kk * sk], E);
+        } else {
+            for (int kk = k0 + (t & 3); kk < k1; kk += 4) acc += crt_norm_term(X[(int64_t) myline * sl + kk], E);
+        }
+    }
+    red[t] = acc;
+    __syncthreads();
+    if (t < 64) {
+        const int l = l0 + t;
+        unsigned long long v = 0;
+        if (line_contig) {
+            for (int j = 0; j < 4; ++j) v += red[t + 64 * j];
+        } else {
+            for (int j = 0; j < 4; ++j) v += red[4 * t + j];
+        }
+        if (l < lines && v != 0) atomicAdd(&S[l], v);
+    }
+}
+
+// e = T + 19 - E - ceil(bitlen(S - 1) / 2): the largest power with
+// 2^e ||line||_2 <= 2^(T-1); kExpEmpty for a zero line.
+__global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, const unsigned long long *__restrict__ S,
+                                    int n, int T, int *__restrict__ e)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const double v = __longlong_as_double((long long) mx[l]);
+    if (v == 0.0) {
+        e[l] = kExpEmpty;
+        return;
+    }
+    int E;
+    (void) frexp(v, &E);
+    const unsigned long long s = S[l];
+    const int b = s >= 2 ? 64 - __clzll((long long) (s - 1)) : 0;
+    e[l] = T + 19 - E - (b + 1) / 2;
+}
+
+// Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| < 2^78)
+// is cut into signed 21-bit pieces a' = x3 2^63 + x2 2^42 + x1 2^21 + x0
+// (each step exact), and a' mod p = x3 c63 + x2 c42 + x1 c21 + x0 (mod p),
+// an INT32 sum below 2^30, reduced once.  Plane t: P[t * plane + line * Kp + kk].
+__global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__restrict__ X, int lines, int K,
+                                                              int64_t sl, int64_t sk, const int *__restrict__ ex,
+                                                              int nmod, int8_t *__restrict__ P, int64_t plane, int Kp)
+{
+    __shared__ double tile[64][65];
+    const int t = threadIdx.x;
+    const int l0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    if (sl == 1) {
+        for (int p = 0; p < 16; ++p) {
+            const int li = t & 63, ki = (t >> 6) + 4 * p;
+            const int l = l0 + li, kk = k0 + ki;
+            tile[li][ki] = (l < lines && kk < K) ? X[l + (int64_t) kk * sk] : 0.0;
+        }
+    } else {
+        for (int p = 0; p < 16; ++p) {
+            const int ki = t & 63, li = (t >> 6) + 4 * p;
+            const int l = l0 + li, kk = k0 + ki;
+            tile[li][ki] = (l < lines && kk < K) ? X[(int64_t) l * sl + kk] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int li = t >> 2, kq = (t & 3) * 16;
+    const int l = l0 + li;
+    const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
+    const int e = zero_line ? 0 : ex[l];
+    int x3[16], x2[16], x1[16], x0[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        double a = zero_line ? 0.0 : rint(ldexp(tile[li][kq + j], e));
+        const double h3 = trunc(ldexp(a, -63));
+        a -= ldexp(h3, 63);
+        const double h2 = trunc(ldexp(a, -42));
+        a -= ldexp(h2, 42);
+        const double h1 = trunc(ldexp(a, -21));
+        a -= ldexp(h1, 21);
+        x3[j] = (int) h3; x2[j] = (int) h2; x1[j] = (int) h1; x0[j] = (int) a;
+    }
+    int8_t *base = P + (int64_t) l * Kp + k0 + kq;
+    for (int m = 0; m < nmod; ++m) {
+        const int c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
+        const double p = c_crt.pd[m], ip = c_crt.invpd[m];
+        u32x4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * q + b;
+                const int s = x3[j] * c63 + x2[j] * c42 + x1[j] * c21 + x0[j];
+                const int r = (int) crt_bal((double) s, p, ip);
+                w |= ((uint32_t) r & 255u) << (8 * b);
+            }
+            v[q] = w;
+        }
+        *reinterpret_cast<u32x4 *>(base + (int64_t) m * plane) = v;
+    }
+}
+
+// Epilogue of a CRT segment (modulus t, one k-chunk): R_t = (R_t + Z) mod p_t
+// (R_t = Z mod p_t on the first chunk), over the whole padded tile.
+template <int WTM, int WTN, int FI, int FJ>
+__device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&acc)[FI][FJ], int m0, int n0, int t,
+                                                bool first)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, kq = lane >> 4;
+    const double p = c_crt.pd[t], ip = c_crt.invpd[t];
+    int8_t *R = g.R8 + (int64_t) t * g.rplane;
+#pragma unroll
+    for (int i = 0; i < FI; ++i) {
+        const int row = m0 + wm * WTM + i * 16 + li;
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
+                int8_t *rp = R + (int64_t) col * g.Mp + row;
+                double v = (double) acc[i][j][r];
+                if (!first) v += (double) *rp;
+                *rp = (int8_t) (int) crt_bal(v, p, ip);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Per element (4 consecutive rows per thread): the N residues, Garner's
+// balanced mixed-radix digits v_0..v_{N-1} (FP32, exact: every operand is
+// below 2^17), x = v_0 + p_0 (v_1 + p_1 (...)) in FP64 from the top digit,
+// C = alpha 2^-eA 2^-eB x (+ beta C).
+template <int NM>
+__global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restrict__ R, int64_t rplane, int Mp, int M,
+                                                          int N, const int *__restrict__ eA,
+                                                          const int *__restrict__ eB, double alpha, double beta,
+                                                          double *C, int64_t ldc)
+{
+    const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i0 >= M) return;
+    for (int j = blockIdx.y; j < N; j += gridDim.y) {
+        float v[4][NM];
+#pragma unroll
+        for (int t = 0; t < NM; ++t) {
+            const uint32_t w = *reinterpret_cast<const uint32_t *>(R + (int64_t) t * rplane + (int64_t) j * Mp + i0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q][t] = (float) (int8_t) (w >> (8 * q));
+        }
+#pragma unroll
+        for (int t = 1; t < NM; ++t) {
+            const float p = c_crt.p[t], ip = c_crt.invp[t];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float x = v[q][t];
+#pragma unroll
+                for (int s = 0; s < t; ++s) x = crt_balf((x - v[q][s]) * c_crt.ginv[t][s], p, ip);
+                v[q][t] = x;
+            }
+        }
+        const int eb = eB[j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = i0 + q;
+            if (i >= M) break;
+            double x = (double) v[q][NM - 1];
+#pragma unroll
+            for (int t = NM - 2; t >= 0; --t) x = ca3::add_rn(ca3::mul_rn(x, c_crt.pd[t]), (double) v[q][t]);
+            const int ea = eA[i];
+            x = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(x, -ea), -eb) : 0.0;
+            double *cp = C + (int64_t) j * ldc + i;
+            x = ca3::mul_rn(alpha, x);
+            if (beta != 0.0) x = ca3::add_rn(x, ca3::mul_rn(beta, *cp));
+            *cp = x;
+        }
+    }
+}
+
 // Software-pipelined form (the default).  Direct-to-LDS loads fill a ring
 // of four 64-byte stages; the fragments of stage z+1 are read from LDS into
 // a second register set while the MFMAs of stage z run, so the MFMA pipe
@@ -715,10 +1017,11 @@ __device__ __forceinline__ void oz_mfmas(const i32x4 (&af)[FI], const i32x4 (&bf
 // NW waves per workgroup in a 2 x NW/2 grid (wave w: rows (w & 1) * WTM,
 // columns (w >> 1) * WTN); with NW = 8 two waves share each SIMD and cover
 // each other's waits.
-template <bool SPLIT, int WTM, int WTN, bool FUSED, int NW>
+template <bool SPLIT, int WTM, int WTN, bool FUSED, int NW, bool CRT = false>
 __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
 {
     static_assert(!(SPLIT && FUSED), "the split form has its own fold");
+    static_assert(!CRT || (FUSED && !SPLIT), "the CRT form runs fused");
     constexpr int NBUF = 4;
     constexpr int kBMo = 2 * WTM, kBNo = (NW / 2) * WTN;
     constexpr int FI = WTM / 16, FJ = WTN / 16;
@@ -752,13 +1055,14 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
     for (int sg = 0; sg < nseg; ++sg) {
     bool first = g.first != 0, last = g.last != 0;
     if (FUSED) {
-        d = g.s - 1 - sg / g.nchunks;
+        // slices: diagonal d = s-1 .. 0; CRT: modulus d = 0 .. s-1 (pair (d, d))
+        d = CRT ? sg / g.nchunks : g.s - 1 - sg / g.nchunks;
         const int c = sg - (sg / g.nchunks) * g.nchunks;
         kb0 = c * g.kcb;
         nkb = min(g.kcb, g.Kp / kKb - kb0);
-        first = (sg == 0);
+        first = CRT ? (c == 0) : (sg == 0);
     }
-    const int total = (d + 1) * nkb;          // 64-byte stages; a multiple of NBUF
+    const int total = (CRT ? 1 : d + 1) * nkb;  // 64-byte stages; a multiple of NBUF
     it = 0;
     ib = 0;
     inext = 0;
@@ -785,7 +1089,7 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
         ib = more ? (nb == nkb ? 0 : nb) : ib;
         it = more ? (nb == nkb ? it + 1 : it) : it;
         const int64_t koff = (int64_t) (kb0 + b) * kKb;
-        const int8_t *pa = gA + (int64_t) t * g.planeA + koff;
+        const int8_t *pa = gA + (int64_t) (CRT ? d : t) * g.planeA + koff;
         const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
         int8_t *sa = slotA(J), *sb = slotB(J);
 #pragma unroll
@@ -878,7 +1182,10 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
     // the loads and fragment reads past the last stage were repeats of it;
     // drain the loads before the workgroup can end (or reuse the ring)
     __builtin_amdgcn_s_waitcnt(0);
-    if (FUSED) {
+    if (CRT) {
+        oz_epilogue_crt<WTM, WTN>(g, acc, m0, n0, d, first);
+        __syncthreads();
+    } else if (FUSED) {
         oz_epilogue_w<WTM, WTN>(g, acc, m0, n0, d, first);
         __syncthreads();                      // every wave is done with the ring
     } else {
@@ -902,13 +1209,13 @@ __global__ __launch_bounds__(256) void ozaki_fold_kernel(const int *__restrict__
     double acc = 0.0;
     for (int d = s - 1; d >= 0; --d) {
         const double w = ldexp(1.0, -8 * (d + 2));
-        for (int c = 0; c < nchunks; ++c) acc = __dadd_rn(acc, (double) p[(int64_t) (d * nchunks + c) * plane] * w);
+        for (int c = 0; c < nchunks; ++c) acc = ca3::add_rn(acc, (double) p[(int64_t) (d * nchunks + c) * plane] * w);
     }
     const int ea = eA[i], eb = eB[j];
     double v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(acc, ea), eb) : 0.0;
     double *cp = C + (int64_t) j * ldc + i;
-    v = __dmul_rn(alpha, v);
-    if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, *cp));
+    v = ca3::mul_rn(alpha, v);
+    if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
     *cp = v;
     if (mirror && i != j) C[(int64_t) i * ldc + j] = v;
 }
@@ -1105,9 +1412,125 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
     return ca3::finish();
 }
 
+// Sizes of one CRT-form product.
+struct Oz2Plan {
+    int Mp, Np, Kp, nchunks;
+    size_t planesA, planesB, exps, words, res, total;
+};
+
+Oz2Plan oz2_plan(int m, int n, int k, int nmod)
+{
+    Oz2Plan p;
+    p.Mp = (int) round_up(m, kTile);
+    p.Np = (int) round_up(n, kTile);
+    p.Kp = (int) round_up(std::max(k, 1), kKp);
+    p.nchunks = (p.Kp + kCrtChunk - 1) / kCrtChunk;
+    p.planesA = (size_t) nmod * p.Mp * p.Kp;
+    p.planesB = (size_t) nmod * p.Np * p.Kp;
+    p.exps = sizeof(int) * (size_t) (p.Mp + p.Np);
+    p.words = 2 * sizeof(unsigned long long) * (size_t) (p.Mp + p.Np);   // line maxima, then norm sums
+    p.res = (size_t) nmod * p.Mp * p.Np;
+    p.total = p.planesA + p.planesB + p.exps + p.words + 256 + p.res;
+    return p;
+}
+
+template <int NM>
+void launch_crt(dim3 grid, hipStream_t st, const int8_t *R, int64_t rplane, int Mp, int m, int n, const int *eA,
+                const int *eB, double alpha, double beta, double *C, int64_t ldc)
+{
+    ozaki2_crt_kernel<NM><<<grid, 256, 0, st>>>(R, rplane, Mp, m, n, eA, eB, alpha, beta, C, ldc);
+}
+
+template <int... Ns>
+void launch_crt_n(int nmod, std::integer_sequence<int, Ns...>, dim3 grid, hipStream_t st, const int8_t *R,
+                  int64_t rplane, int Mp, int m, int n, const int *eA, const int *eB, double alpha, double beta,
+                  double *C, int64_t ldc)
+{
+    ((nmod == Ns + kCrtMin ? launch_crt<Ns + kCrtMin>(grid, st, R, rplane, Mp, m, n, eA, eB, alpha, beta, C, ldc)
+                           : void()), ...);
+}
+
+// C := alpha op(A) op(B) + beta C through the CRT form with nmod moduli.
+int ozaki2_core(int ta, int tb, int m, int n, int k, double alpha, const double *A, int ldA, const double *B,
+                int ldB, double beta, double *C, int ldC, int nmod)
+{
+    const Oz2Plan pl = oz2_plan(m, n, k, nmod);
+    const int Mp = pl.Mp, Np = pl.Np, Kp = pl.Kp;
+    int rc = 0;
+    char *ws = (char *) workspace(pl.total, &rc);
+    if (rc) return rc;
+    int8_t *PA = (int8_t *) ws;
+    int8_t *PB = PA + pl.planesA;
+    int *eA = (int *) (ws + pl.planesA + pl.planesB);
+    int *eB = eA + Mp;
+    unsigned long long *mx = (unsigned long long *) (ws + pl.planesA + pl.planesB + pl.exps);
+    unsigned long long *S = mx + (Mp + Np);
+    int8_t *R = (int8_t *) (((uintptr_t) (ws + pl.planesA + pl.planesB + pl.exps + pl.words) + 255) &
+                            ~(uintptr_t) 255);
+    hipStream_t st = ca3::stream();
+    const int nl = Mp + Np;
+    fill_int_kernel<<<(4 * nl + 255) / 256, 256, 0, st>>>((int *) mx, 4 * nl, 0);
+    const int64_t a_sl = ta ? ldA : 1, a_sk = ta ? 1 : ldA;
+    const int64_t b_sl = tb ? 1 : ldB, b_sk = tb ? ldB : 1;
+    const int ky = std::min(64, std::max(1, k / 512));
+    if (k > 0) {
+        ozaki_max_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, mx);
+        ozaki_max_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, mx + Mp);
+        ozaki2_norm_kernel<<<dim3((m + 63) / 64, ky), 256, 0, st>>>(A, m, k, a_sl, a_sk, mx, S);
+        ozaki2_norm_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, mx + Mp, S + Mp);
+    }
+    ozaki2_scale_kernel<<<(nl + 255) / 256, 256, 0, st>>>(mx, S, nl, kCrtT[nmod], eA);
+    ozaki2_residue_kernel<<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA, nmod, PA,
+                                                                  (int64_t) Mp * Kp, Kp);
+    ozaki2_residue_kernel<<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB, nmod, PB,
+                                                                  (int64_t) Np * Kp, Kp);
+    OzArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = m; g.N = n; g.Mp = Mp; g.Np = Np; g.Kp = Kp;
+    g.alpha = alpha; g.beta = beta;
+    g.PA = PA; g.PB = PB;
+    g.planeA = (int64_t) Mp * Kp; g.planeB = (int64_t) Np * Kp;
+    g.eA = eA; g.eB = eB;
+    g.C = C; g.ldc = ldC;
+    g.s = nmod;
+    g.nchunks = pl.nchunks;
+    g.kcb = kCrtChunk / kKb;
+    g.R8 = R;
+    g.rplane = (int64_t) Mp * Np;
+    ozaki_pipe_kernel<false, 128, 64, true, 8, true><<<(Mp / 256) * (Np / 256), 512, 0, st>>>(g);
+    const dim3 grid((m + 1023) / 1024, std::min(n, 65535));
+    launch_crt_n(nmod, std::make_integer_sequence<int, kCrtMax - kCrtMin + 1>{}, grid, st, R, g.rplane, Mp, m, n,
+                 eA, eB, alpha, beta, C, ldC);
+    return ca3::finish();
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t hip_ozaki2_workspace_bytes(int m, int n, int k, int nmoduli)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || nmoduli < kCrtMin || nmoduli > kCrtMax || k > kCrtKMax) return 0;
+    return oz2_plan(m, n, k, nmoduli).total;
+}
+
+int hip_ozaki2_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n,
+                     const int k, const double alpha, const double *A, const int ldA, const double *B,
+                     const int ldB, const double beta, double *C, const int ldC, int nmoduli)
+{
+    const int ta = (transA != HipNoTrans), tb = (transB != HipNoTrans);
+    if (m < 0 || n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: negative size");
+    if (m == 0 || n == 0) return HIP_PROXY_OK;
+    if (nmoduli < kCrtMin || nmoduli > kCrtMax)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: nmoduli %d outside %d..%d", nmoduli, kCrtMin, kCrtMax);
+    if (k > kCrtKMax) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: k %d above %d", k, kCrtKMax);
+    if (!C || ldC < std::max(1, m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: bad C / ldC");
+    if (k > 0) {
+        if (!A || ldA < std::max(1, ta ? k : m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: bad A / ldA");
+        if (!B || ldB < std::max(1, tb ? n : k)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: bad B / ldB");
+    }
+    return ozaki2_core(ta, tb, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, nmoduli);
+}
 
 size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices)
 {

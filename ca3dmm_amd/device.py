@@ -124,6 +124,12 @@ def ozaki_dgemm(transA, transB, m, n, k, alpha, A, B, beta, Cm, nslices, lda=Non
                  "hip_ozaki_dgemm")
 
 
+def ozaki2_dgemm(transA, transB, m, n, k, alpha, A, B, beta, Cm, nmoduli, lda=None, ldb=None, ldc=None):
+    return check(lib.hip_ozaki2_dgemm(transA, transB, m, n, k, alpha, A.ptr, lda or _ld(A), B.ptr,
+                                      ldb or _ld(B), beta, Cm.ptr, ldc or _ld(Cm), nmoduli),
+                 "hip_ozaki2_dgemm")
+
+
 def dgemm_fold(transA, transB, m, n, k, alpha, A, B, beta, Cm, parts):
     arr = (C.c_void_p * len(parts))(*[p.ptr for p in parts])
     lds = (C.c_int * len(parts))(*[_ld(p) for p in parts])

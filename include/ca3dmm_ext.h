@@ -68,6 +68,22 @@ int hip_ozaki_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
 /* Device workspace the library keeps for hip_ozaki_dgemm at this size. */
 size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices);
 
+/* C := alpha * op(A) * op(B) + beta * C through the integer-modular (CRT)
+ * form of the Ozaki scheme (Ozaki scheme II): lines of op(A) and op(B) are
+ * scaled by powers of two to integers whose 2-norms fit the product of the
+ * first nmoduli moduli (4 <= nmoduli <= 20), one exact INT8 product per
+ * modulus (v_mfma_i32_16x16x64_i8), then Garner's mixed-radix digits are
+ * evaluated in FP64 in a fixed order, so results are reproducible bit for
+ * bit (oracle: ozaki2_dgemm).  k <= 2^23.  No reference counterpart: an
+ * alternative to hip_dgemm (the local DGEMM of src/cannon.c:405-427). */
+int hip_ozaki2_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
+                     const int m, const int n, const int k, const double alpha,
+                     const double *A, const int ldA, const double *B, const int ldB,
+                     const double beta, double *C, const int ldC, int nmoduli);
+
+/* Device workspace the library keeps for hip_ozaki2_dgemm at this size. */
+size_t hip_ozaki2_workspace_bytes(int m, int n, int k, int nmoduli);
+
 /* hip_dsyrk_gram (below) through the Ozaki scheme: C := alpha * A^T A +
  * beta * C for A k x n (column-major), upper triangle only, mirror != 0
  * copies it to the lower triangle.  A is sliced once and serves as both

@@ -204,3 +204,153 @@ def diagonals8(SA, SB, s):
             acc = P if acc is None else acc + P
         out.append(acc)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Ozaki scheme II: the integer-modular (CRT) form on INT8 products
+# (Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
+# floating-point matrix multiplication using an integer modular technique",
+# 2025).  One INT8 product per modulus instead of s(s+1)/2 slice products.
+#
+#   * every line i of op(A) is scaled by a power of two 2^e_i and rounded to
+#     the integer a'_il = rint(a_il 2^e_i) (exact as a double), likewise the
+#     columns of op(B) by 2^f_j;
+#   * e_i keeps ||a'_i||_2 <= 2^T and f_j keeps ||b'_j||_2 <= 2^T with
+#     2^(2T+1) < P = prod p_t, so |C'_ij| = |a'_i . b'_j| < P/2 (Cauchy-Schwarz)
+#     and C' is fixed by its residues modulo the N moduli;
+#   * residue planes r_t(a') in [-(p_t-1)/2, (p_t-1)/2] ([-128, 127] for 256)
+#     are int8; Z_t = R_t(A') R_t(B') is exact in INT32 over k-chunks, and
+#     z_t = Z_t mod p_t (balanced);
+#   * Garner's algorithm turns (z_t) into the balanced mixed-radix digits v_t
+#     of the unique x in [-P/2, P/2) with x = C' (mod P), i.e. x = C';
+#   * x is evaluated in FP64 by Horner from the top digit, one rounded
+#     multiply and one rounded add per step (no fused multiply-add), and
+#     C_ij = alpha 2^-e_i 2^-f_j x (+ beta C_in).
+# Everything up to the digits is exact integer arithmetic, so any correct
+# implementation agrees; the Horner sum fixes the rounding, and the device
+# kernel follows the same order bit for bit.
+#
+# The line norm bound is order-independent (integer sums): with E the frexp
+# exponent of the line's max |x| and, per element, ex its exponent and
+# m4 = ceil(16 |frac|) in [8, 16], x^2 <= m4^2 4^(ex-4), so
+# ||line||^2 <= 4^E 2^-40 S with S = sum ceil(m4^2 2^(2(ex-E)+32)) (an
+# integer <= k 2^40), and e = T + 19 - E - ceil(bitlen(S - 1) / 2) is the
+# largest power with 2^e ||line|| <= 2^(T-1).  Rounding adds at most
+# sqrt(k)/2 <= 2^(T-1) to the norm (k <= 4^T), hence ||a'|| <= 2^T.
+# ---------------------------------------------------------------------------
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+              223, 217, 211, 199, 197, 193, 191, 181, 179, 173)
+CRT_MIN, CRT_MAX = 4, len(CRT_MODULI)
+CRT_KMAX = 1 << 23          # S <= k 2^40 stays below 2^63
+
+
+def crt_product(N):
+    P = 1
+    for p in CRT_MODULI[:N]:
+        P *= p
+    return P
+
+
+def crt_T(N):
+    """Norm budget T: the largest integer with 2^(2T+1) < P, taken from the
+    bit length b of P (2^(b-1) <= P): T = (b - 2) // 2."""
+    return (crt_product(N).bit_length() - 2) // 2
+
+
+def crt_line_exponents(X, axis, N):
+    """Scale exponents e (int64) and zero mask of the lines of X along
+    `axis` (axis=1: rows of op(A); axis=0: columns of op(B))."""
+    T = crt_T(N)
+    mx = np.max(np.abs(X), axis=axis)
+    _, E = np.frexp(mx)
+    E = E.astype(np.int64)
+    f, ex = np.frexp(np.abs(X))
+    m4 = np.ceil(f * 16.0).astype(np.int64)        # 0 for x == 0
+    Eb = E.reshape((-1, 1) if axis == 1 else (1, -1))
+    sh = (2 * (ex.astype(np.int64) - Eb) + 32)        # <= 32
+    m2 = m4 * m4
+    up = np.where(sh >= 0, m2 << np.clip(sh, 0, 62), 0)
+    dn = np.where(m2 > 0, ((m2 - 1) >> np.clip(-sh, 0, 62)) + 1, 0)
+    w = np.where(sh >= 0, up, dn)
+    S = np.sum(w, axis=axis, dtype=np.uint64).astype(object)
+    b = np.array([int(v - 1).bit_length() if v >= 1 else 0 for v in S.ravel()], dtype=np.int64)
+    e = T + 19 - E - (b + 1) // 2
+    return e, mx == 0
+
+
+def crt_scaled(X, axis, N):
+    """Integer-valued doubles a' = rint(x 2^e) per line; zero lines 0."""
+    e, zero = crt_line_exponents(X, axis, N)
+    shp = (-1, 1) if axis == 1 else (1, -1)
+    a = np.rint(np.ldexp(X, np.where(zero, 0, e).reshape(shp)))
+    return np.where(zero.reshape(shp), 0.0, a), e, zero
+
+
+def balanced_mod(x, p):
+    """Balanced residue of integer-valued x (numpy int64 or Python int
+    arrays): [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for p = 256."""
+    r = np.mod(x, p)
+    half = p // 2
+    return np.where(r > (half if p % 2 else half - 1), r - p, r)
+
+
+def crt_residues(Aint, N):
+    """Residue planes (int8) of an array of integer-valued doubles, exact:
+    |a'| < 2^78 is split into three signed 26-bit pieces first."""
+    x2 = np.trunc(np.ldexp(Aint, -52))
+    rem = Aint - np.ldexp(x2, 52)
+    x1 = np.trunc(np.ldexp(rem, -26))
+    x0 = rem - np.ldexp(x1, 26)
+    parts = [v.astype(np.int64) for v in (x2, x1, x0)]
+    out = []
+    for p in CRT_MODULI[:N]:
+        c26, c52 = pow(2, 26, p), pow(2, 52, p)
+        r = (parts[0] % p) * c52 + (parts[1] % p) * c26 + (parts[2] % p)
+        out.append(balanced_mod(r, p).astype(np.int8))
+    return out
+
+
+def crt_garner(Z, N):
+    """Balanced mixed-radix digits v_0..v_{N-1} (int64 arrays) of the x in
+    [-P/2, P/2) with x = Z_t (mod p_t) for every t; Z: list of residues."""
+    ps = CRT_MODULI[:N]
+    v = []
+    for t in range(N):
+        x = Z[t].astype(np.int64)
+        for j in range(t):
+            x = ((x - v[j]) * pow(ps[j], -1, ps[t])) % ps[t]
+        v.append(balanced_mod(x, ps[t]))
+    return v
+
+
+def crt_horner(v, N):
+    """x = v_0 + p_0 (v_1 + p_1 (v_2 + ...)) in FP64, from the top digit,
+    a rounded multiply then a rounded add per step."""
+    ps = CRT_MODULI[:N]
+    x = v[N - 1].astype(np.float64)
+    for t in range(N - 2, -1, -1):
+        x = x * float(ps[t])
+        x = x + v[t].astype(np.float64)
+    return x
+
+
+def ozaki2_dgemm(opA, opB, N, alpha=1.0, beta=0.0, Cin=None):
+    """The CRT form: the reference result the device kernels reproduce bit
+    for bit.  Residue products are exact (float64 BLAS on int8 operands:
+    |Z| <= k 2^14 < 2^53)."""
+    k = opA.shape[1]
+    assert CRT_MIN <= N <= CRT_MAX and k <= CRT_KMAX
+    Ai, eA, zA = crt_scaled(opA, 1, N)
+    Bi, eB, zB = crt_scaled(opB, 0, N)
+    RA, RB = crt_residues(Ai, N), crt_residues(Bi, N)
+    Z = []
+    for t, p in enumerate(CRT_MODULI[:N]):
+        P = RA[t].astype(np.float64) @ RB[t].astype(np.float64)
+        Z.append(balanced_mod(P.astype(np.int64), p))
+    x = crt_horner(crt_garner(Z, N), N)
+    C = np.ldexp(np.ldexp(x, -np.where(zA, 0, eA).reshape(-1, 1)), -np.where(zB, 0, eB).reshape(1, -1))
+    C = np.where(zA.reshape(-1, 1) | zB.reshape(1, -1), 0.0, C)
+    C = alpha * C
+    if beta != 0.0:
+        C = C + beta * Cin
+    return C

@@ -67,6 +67,21 @@ def test_dgemm_alpha_beta_and_ld(hip):
     assert np.array_equal(dC.numpy(), want)          # rows m..ldc untouched
 
 
+def test_dgemm_general_alpha_beta_rounded_separately(hip):
+    """alpha * AB and beta * C are rounded separately, then added (the C
+    oracle's order): with exact integer products, alpha = 1.1 and beta = -0.3
+    a fused multiply-add in the epilogue would differ in the last bit."""
+    rnd = np.random.default_rng(8)
+    m, n, k = 133, 97, 70
+    A = rnd.integers(-16, 17, (m, k)).astype(np.float64)
+    B = rnd.integers(-16, 17, (k, n)).astype(np.float64)
+    Cm = rnd.standard_normal((m, n))
+    want = 1.1 * (A @ B) + (-0.3) * Cm
+    dA, dB, dC = (hip.DeviceArray.from_numpy(x) for x in (A, B, Cm))
+    hip.dgemm(0, 0, m, n, k, 1.1, dA, dB, -0.3, dC)
+    assert np.array_equal(dC.numpy(), want)
+
+
 def test_dgemm_k_zero_scales_c(hip):
     Cm = np.arange(12.0).reshape(3, 4)
     dC = hip.DeviceArray.from_numpy(Cm)

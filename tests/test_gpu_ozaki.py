@@ -68,6 +68,20 @@ def test_ozaki_alpha_beta(hip):
 
 
 @pytest.mark.parametrize("split", ["0", "1"])
+def test_ozaki_general_alpha_beta(hip, monkeypatch, split):
+    """alpha = 1.1, beta = -0.3: the two products are rounded separately and
+    then added, in both forms (a fused multiply-add would differ)."""
+    monkeypatch.setenv("CA3DMM_OZAKI_SPLIT", split)
+    rnd = np.random.default_rng(15)
+    m, n, k = 90, 70, 300
+    A, B, Cin = Z.phi_matrix(rnd, (m, k), 2.0), Z.phi_matrix(rnd, (k, n), 2.0), rnd.standard_normal((m, n))
+    want = Z.ozaki_dgemm8(A, B, 8, alpha=1.1, beta=-0.3, Cin=Cin)
+    dA, dB, dC = (hip.DeviceArray.from_numpy(x) for x in (A, B, Cin))
+    hip.ozaki_dgemm(0, 0, m, n, k, 1.1, dA, dB, -0.3, dC, 8)
+    assert np.array_equal(dC.numpy(), want)
+
+
+@pytest.mark.parametrize("split", ["0", "1"])
 def test_ozaki_k_chunks(hip, monkeypatch, split):
     """s = 16 limits an INT32-exact k-chunk to 7936: k = 20000 runs 3 chunks,
     through the per-diagonal launches (split = 0) and the split-K launch with
