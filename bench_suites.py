@@ -6,8 +6,11 @@ Each suite prints JSON lines (one per measurement) on stdout:
                over 2000 x 1334, and the mat_redist pack / unpack of the
                1D-column-partitioned 8000 x 8000 A for all 24 ranks (and rank
                0 alone), with achieved HBM GB/s on algorithmic bytes;
-  ozaki-sweep  native FP64-MFMA DGEMM vs Ozaki INT8 at 8192^3 for
-               phi in {0.5, 2, 4} and s in 3..11, error against double-double;
+  ozaki-sweep  native FP64-MFMA DGEMM vs the two Ozaki INT8 forms at 8192^3
+               and 65536 x 65536 x 1024 for phi in {0.5, 2, 4}, s in 3..11
+               slices and 10..20 CRT moduli, error against double-double;
+  ozaki-forms  native DGEMM, slices s = 8 and CRT with 15 moduli at n^3
+               (phi = 0.5), three timed calls each: the kernel-trace workload;
   grouped      the simulated 4x2x3 grid on 10007^3: one launch per rank and
                round vs the persistent grouped launch (untuned / tuned / graph);
   cholqr       CholeskyQR2 of a 2e6 x 512 matrix with condition 1e2, 1e5, 1e8;
@@ -128,11 +131,16 @@ def rank_share(args):
 
 # ---------------------------------------------------------------------------
 def ozaki_sweep(args):
-    """Error on 4096 sampled entries against double-double, per (shape, phi, s)."""
+    """Error on 4096 sampled entries against double-double, per (shape, phi,
+    form, count): the slice form for s = 3..11 and the CRT form for 10..20
+    moduli; `no_worse_than_native` needs both error measures at or below the
+    native product's."""
     from oracle import oracle as O
     from oracle import ozaki as Z
     rnd = np.random.default_rng(7)
     shapes = [(args.n, args.n, args.n), (65536, 65536, 1024)]
+    forms = [("ozaki-int8", "s", range(3, 12), hip.ozaki_dgemm, Z.products),
+             ("ozaki-int8-crt", "moduli", range(10, 21), hip.ozaki2_dgemm, lambda c: c)]
     for m, n, k in shapes:
         flops = 2.0 * m * n * k
         for phi in (0.5, 2.0, 4.0):
@@ -146,22 +154,57 @@ def ozaki_sweep(args):
             nat_ms = ms = timed(lambda: hip.dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC), reps=2)
             got = dC.sample(ii, jj)
             nat = float(np.max(np.abs(got - ref) / ab))
+            nat_rel = float(np.max(np.abs(got - ref) / np.abs(ref)))
             base = {"suite": "ozaki-sweep", "m": m, "n": n, "k": k, "phi": phi, "samples": 4096}
             emit(dict(base, method="dgemm-mfma", ms=round(ms, 3), tflops=round(flops / ms / 1e9, 2),
-                      max_err_over_absAabsB=nat,
-                      max_rel_err=float(np.max(np.abs(got - ref) / np.abs(ref)))))
-            for s in range(3, 12):
-                ms = timed(lambda: hip.ozaki_dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, s), reps=2)
-                got = dC.sample(ii, jj)
-                err = float(np.max(np.abs(got - ref) / ab))
-                emit(dict(base, method="ozaki-int8", s=s, ms=round(ms, 3),
-                          effective_tflops=round(flops / ms / 1e9, 2),
-                          int8_tops=round(Z.products(s) * flops / ms / 1e9, 1),
-                          max_err_over_absAabsB=err,
-                          max_rel_err=float(np.max(np.abs(got - ref) / np.abs(ref))),
-                          no_worse_than_native=err <= nat, speedup_vs_native=round(nat_ms / ms, 3)))
+                      max_err_over_absAabsB=nat, max_rel_err=nat_rel))
+            for name, key, counts, fn, products in forms:
+                for c in counts:
+                    ms = timed(lambda: fn(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, c), reps=2)
+                    got = dC.sample(ii, jj)
+                    err = float(np.max(np.abs(got - ref) / ab))
+                    rel = float(np.max(np.abs(got - ref) / np.abs(ref)))
+                    emit(dict(base, method=name, **{key: c}, ms=round(ms, 3),
+                              effective_tflops=round(flops / ms / 1e9, 2),
+                              int8_tops=round(products(c) * flops / ms / 1e9, 1),
+                              max_err_over_absAabsB=err, max_rel_err=rel,
+                              no_worse_than_native=(err <= nat and rel <= nat_rel),
+                              speedup_vs_native=round(nat_ms / ms, 3)))
             hip.set_sync(True)
             del dA, dB, dC
+
+
+def ozaki_forms(args):
+    """One shape, the three FP64-accurate products side by side (for kernel
+    traces): native FP64-MFMA DGEMM, the slice form at s = 8 and the CRT
+    form at 15 moduli, n^3 with phi = 0.5 inputs; one warm-up and three timed
+    calls each, error on 1024 sampled entries against double-double."""
+    from oracle import oracle as O
+    from oracle import ozaki as Z
+    n = args.n
+    rnd = np.random.default_rng(1)
+    A, B = Z.phi_matrix(rnd, (n, n), 0.5), Z.phi_matrix(rnd, (n, n), 0.5)
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.zeros((n, n))
+    ii, jj = rnd.integers(0, n, 1024), rnd.integers(0, n, 1024)
+    ref, ab = O.dd_sample(0, 0, n, n, n, np.asfortranarray(A), n, np.asfortranarray(B), n, ii, jj)
+    del A, B
+    flops = 2.0 * n ** 3
+    hip.set_sync(False)
+    for name, f, prods in (("dgemm-mfma", lambda: hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC), None),
+                           ("ozaki-int8 s=8", lambda: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 8), 36),
+                           ("ozaki-int8-crt moduli=15",
+                            lambda: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 15), 15)):
+        ms = timed(f, reps=3)
+        got = dC.sample(ii, jj)
+        e = np.abs(got - ref)
+        rec = {"suite": "ozaki-forms", "n": n, "phi": 0.5, "method": name, "ms": round(ms, 3),
+               "effective_tflops": round(flops / ms / 1e9, 2), "max_err_over_absAabsB": float(np.max(e / ab)),
+               "max_rel_err": float(np.max(e / np.abs(ref)))}
+        if prods:
+            rec["int8_tops"] = round(prods * flops / ms / 1e9, 1)
+        emit(rec)
+    hip.set_sync(True)
 
 
 # ---------------------------------------------------------------------------
@@ -247,8 +290,8 @@ def pmc_calib(args):
     """Counter-run program: a plain copy of 1 GiB (the scale of FETCH_SIZE /
     WRITE_SIZE), then one launch of each hot kernel at its bench size: the
     four DGEMM trans cases at n^3, the kp = 3 fold over 2000 x 1334, the pack
-    of A for the 24 ranks of the 4x2x3 grid on 8000^3, and a 7-slice Ozaki
-    product n^3."""
+    of A for the 24 ranks of the 4x2x3 grid on 8000^3, a 7-slice Ozaki
+    product n^3 and a 15-modulus CRT-form product n^3."""
     from oracle import oracle as O
     nel = (1 << 30) // 8
     src = hip.DeviceArray.zeros((nel,))
@@ -280,10 +323,12 @@ def pmc_calib(args):
     check_blocks(pack, [Afull] + sends)
     hip.copy_blocks(8, pack)
     hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 7)
+    hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 15)
     hip.sync()
     emit({"suite": "pmc-calib", "copy_bytes_read": nel * 8, "copy_bytes_written": nel * 8,
           "dgemm_n": n, "dgemm_cases": "NN,NT,TN,TT", "fold": "3 x 2000x1334",
-          "pack_elements": sum(d[4] * d[5] for d in pack), "ozaki": f"{n}^3 s=7"})
+          "pack_elements": sum(d[4] * d[5] for d in pack), "ozaki": f"{n}^3 s=7",
+          "ozaki_crt": f"{n}^3 moduli=15"})
 
 
 def yardstick(args):
@@ -334,5 +379,5 @@ def yardstick(args):
 
 
 def run(name, args):
-    {"yardstick": yardstick, "rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "grouped": grouped, "cholqr": cholqr,
+    {"yardstick": yardstick, "rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "ozaki-forms": ozaki_forms, "grouped": grouped, "cholqr": cholqr,
      "pmc-calib": pmc_calib}[name](args)
