@@ -723,7 +723,8 @@ constexpr int crt_pow2(int e, int p)                // 2^e mod p, balanced
 struct CrtTables {
     float p[kCrtMax], invp[kCrtMax];                 // modulus, 1/p (rounded)
     double pd[kCrtMax], invpd[kCrtMax];
-    float ginv[kCrtMax][kCrtMax];                    // p_j^-1 mod p_t, j < t
+    float gmul[kCrtMax][kCrtMax];                    // M_j mod p_t (balanced), M_j = p_0 ... p_{j-1}, j < t
+    float ginv[kCrtMax];                             // M_t^-1 mod p_t
     int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax];    // 2^21, 2^42, 2^63 mod p (balanced)
 };
 
@@ -739,7 +740,12 @@ constexpr CrtTables crt_tables()
         c.c21[t] = crt_pow2(21, p);
         c.c42[t] = crt_pow2(42, p);
         c.c63[t] = crt_pow2(63, p);
-        for (int j = 0; j < kCrtMax; ++j) c.ginv[t][j] = j < t ? (float) crt_inv(kCrtModuli[j] % p, p) : 0.0f;
+        int mj = 1;                                  // M_j mod p_t
+        for (int j = 0; j < kCrtMax; ++j) {
+            c.gmul[t][j] = j < t ? (float) (mj > p / 2 ? mj - p : mj) : 0.0f;
+            if (j < t) mj = (mj * (kCrtModuli[j] % p)) % p;
+        }
+        c.ginv[t] = (float) crt_inv(mj, p);
     }
     return c;
 }
@@ -759,14 +765,22 @@ __device__ __forceinline__ double crt_bal(double y, double p, double invp)
     return r;
 }
 
-// Same in FP32 for |y| < 2^17 (Garner steps).
+// Balanced residue modulo an odd p of an integer-valued float y, |y| < 2^19:
+// y / p is at least 1/(2p) from a half-integer and the rounded quotient
+// errs by less than 2^-11, so rint finds the nearest quotient and the
+// remainder is exactly the balanced one.
 __device__ __forceinline__ float crt_balf(float y, float p, float invp)
 {
-    float r = fmaf(-rintf(y * invp), p, y);
-    const float hi = floorf((p - 1.0f) * 0.5f);
-    r = r > hi ? r - p : r;
-    r = r < hi - p + 1.0f ? r + p : r;
-    return r;
+    return fmaf(-rintf(y * invp), p, y);
+}
+
+// Balanced residue modulo an odd p of an INT32 |x| < 2^30: a first quotient
+// from the rounded float leaves |r1| < 3p, the second step is crt_balf.
+__device__ __forceinline__ int crt_mod_i32(int x, float p, float invp)
+{
+    const int q1 = (int) rintf((float) x * invp);          // |q1| < 2^23: a 24-bit operand
+    const int r1 = x - __mul24(q1, (int) p);
+    return (int) crt_balf((float) r1, p, invp);
 }
 
 // Per line: S = sum over elements of ceil(m4^2 2^(2(ex - E) + 32)), with E
@@ -842,7 +856,9 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
 // Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| < 2^78)
 // is cut into signed 21-bit pieces a' = x3 2^63 + x2 2^42 + x1 2^21 + x0
 // (each step exact), and a' mod p = x3 c63 + x2 c42 + x1 c21 + x0 (mod p),
-// an INT32 sum below 2^30, reduced once.  Plane t: P[t * plane + line * Kp + kk].
+// an INT32 sum of 24-bit products below 2^30, reduced by crt_mod_i32; for
+// p_0 = 256 the residue is the low byte of x0.  Plane t:
+// P[t * plane + line * Kp + kk].
 __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__restrict__ X, int lines, int K,
                                                               int64_t sl, int64_t sk, const int *__restrict__ ex,
                                                               int nmod, int8_t *__restrict__ P, int64_t plane, int Kp)
@@ -881,9 +897,20 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
         x3[j] = (int) h3; x2[j] = (int) h2; x1[j] = (int) h1; x0[j] = (int) a;
     }
     int8_t *base = P + (int64_t) l * Kp + k0 + kq;
-    for (int m = 0; m < nmod; ++m) {
+    {
+        u32x4 v;                                    // p_0 = 256: 2^21 = 0 (mod 256)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) w |= ((uint32_t) x0[4 * q + b] & 255u) << (8 * b);
+            v[q] = w;
+        }
+        *reinterpret_cast<u32x4 *>(base) = v;
+    }
+    for (int m = 1; m < nmod; ++m) {
         const int c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
-        const double p = c_crt.pd[m], ip = c_crt.invpd[m];
+        const float p = c_crt.p[m], ip = c_crt.invp[m];
         u32x4 v;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -891,8 +918,8 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = 4 * q + b;
-                const int s = x3[j] * c63 + x2[j] * c42 + x1[j] * c21 + x0[j];
-                const int r = (int) crt_bal((double) s, p, ip);
+                const int s = __mul24(x3[j], c63) + __mul24(x2[j], c42) + __mul24(x1[j], c21) + x0[j];
+                const int r = crt_mod_i32(s, p, ip);
                 w |= ((uint32_t) r & 255u) << (8 * b);
             }
             v[q] = w;
@@ -930,10 +957,11 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
     }
 }
 
-// Per element (4 consecutive rows per thread): the N residues, Garner's
-// balanced mixed-radix digits v_0..v_{N-1} (FP32, exact: every operand is
-// below 2^17), x = v_0 + p_0 (v_1 + p_1 (...)) in FP64 from the top digit,
-// C = alpha 2^-eA 2^-eB x (+ beta C).
+// Per element (4 consecutive rows per thread): the N residues r_t, Garner's
+// balanced mixed-radix digits v_t = (r_t - sum_{j<t} v_j M_j) M_t^-1 mod p_t
+// (M_j = p_0 ... p_{j-1}; FP32 and exact: the sum stays below 2^19 and is
+// reduced before the product), x = v_0 + p_0 (v_1 + p_1 (...)) in FP64 from
+// the top digit, C = alpha 2^-eA 2^-eB x (+ beta C).
 template <int NM>
 __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restrict__ R, int64_t rplane, int Mp, int M,
                                                           int N, const int *__restrict__ eA,
@@ -952,13 +980,13 @@ __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restric
         }
 #pragma unroll
         for (int t = 1; t < NM; ++t) {
-            const float p = c_crt.p[t], ip = c_crt.invp[t];
+            const float p = c_crt.p[t], ip = c_crt.invp[t], gi = c_crt.ginv[t];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                float x = v[q][t];
+                float sum = v[q][0];                       // M_0 = 1
 #pragma unroll
-                for (int s = 0; s < t; ++s) x = crt_balf((x - v[q][s]) * c_crt.ginv[t][s], p, ip);
-                v[q][t] = x;
+                for (int s = 1; s < t; ++s) sum = fmaf(v[q][s], c_crt.gmul[t][s], sum);
+                v[q][t] = crt_balf((v[q][t] - crt_balf(sum, p, ip)) * gi, p, ip);
             }
         }
         const int eb = eB[j];
