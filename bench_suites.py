@@ -209,6 +209,11 @@ def ozaki_forms(args):
 
 # ---------------------------------------------------------------------------
 def grouped(args):
+    """The simulated 4x2x3 grid on 10007^3, whole sim_exec per call (both
+    redistributions, transposes, allgather, products, k-layer fold):
+    FP64 form per launch / grouped untuned / tuned / hipGraph, then the
+    Ozaki CRT form per rank / grouped / hipGraph, with the error of every
+    form on 512 sampled entries against double-double."""
     from oracle import oracle as O
     m = n = k = 10007
     p = 24
@@ -217,7 +222,14 @@ def grouped(args):
     A = rnd.uniform(-1, 1, (m, k))
     B = rnd.uniform(-1, 1, (k, n))
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    ii, jj = rnd.integers(0, m, 512), rnd.integers(0, n, 512)
+    ref, ab = O.dd_sample(0, 0, m, n, k, np.asfortranarray(A), m, np.asfortranarray(B), k, ii, jj)
     del A, B
+
+    def errs():
+        got = dC.sample(ii, jj)
+        e = np.abs(got - ref)
+        return {"max_err_over_absAabsB": float(np.max(e / ab)), "max_rel_err": float(np.max(e / np.abs(ref)))}
     dC = hip.DeviceArray.zeros((m, n))
     Ap = [dA.at(r[2] * m) for r in w["src_A"]]
     Bp = [dB.at(r[2] * k) for r in w["src_B"]]
@@ -227,13 +239,13 @@ def grouped(args):
     run = lambda: sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
     hip.set_sync(False)
     ms = timed(run, reps=3)
-    ref = dC.numpy()
-    emit({"suite": "grouped", "mode": "per-rank, per-round launches", "ms": round(ms, 3),
-          "tflops": round(flops / ms / 1e9, 2)})
+    ref_c = dC.numpy()
+    emit(dict({"suite": "grouped", "mode": "per-rank, per-round launches", "ms": round(ms, 3),
+               "tflops": round(flops / ms / 1e9, 2)}, **errs()))
     sim.set_mode(grouped=True, graph=False)
     sim.set_tuning(1, 1)
     ms = timed(run, reps=3)
-    d = float(np.max(np.abs(dC.numpy() - ref)))
+    d = float(np.max(np.abs(dC.numpy() - ref_c)))
     emit({"suite": "grouped", "mode": "grouped, untuned (rows=1, 1 WG/CU)", "ms": round(ms, 3),
           "tflops": round(flops / ms / 1e9, 2), "max_abs_diff_vs_per_launch": d})
     g, wg = sim.autotune(2)
@@ -242,8 +254,19 @@ def grouped(args):
           "tflops": round(flops / ms / 1e9, 2)})
     sim.set_mode(grouped=True, graph=True)
     ms = timed(run, reps=3)
-    emit({"suite": "grouped", "mode": "grouped, tuned, hipGraph replay", "ms": round(ms, 3),
-          "tflops": round(flops / ms / 1e9, 2), "max_abs_diff_vs_per_launch": float(np.max(np.abs(dC.numpy() - ref)))})
+    emit(dict({"suite": "grouped", "mode": "grouped, tuned, hipGraph replay", "ms": round(ms, 3),
+               "tflops": round(flops / ms / 1e9, 2),
+               "max_abs_diff_vs_per_launch": float(np.max(np.abs(dC.numpy() - ref_c)))}, **errs()))
+    for nmod, modes in ((15, ((False, False), (True, False), (True, True))), (14, ((True, True),)),
+                        (16, ((True, True),))):
+        sim.set_ozaki(nmod)
+        for grp, graph in modes:
+            sim.set_mode(grouped=grp, graph=graph)
+            ms = timed(run, reps=3)
+            mode = ("grouped" if grp else "per-rank product launches") + (", hipGraph replay" if graph else "")
+            emit(dict({"suite": "grouped", "form": f"ozaki-int8-crt moduli={nmod}", "mode": mode,
+                       "ms": round(ms, 3), "effective_tflops": round(flops / ms / 1e9, 2)}, **errs()))
+    sim.set_ozaki(0)
     hip.set_sync(True)
     sim.free()
 

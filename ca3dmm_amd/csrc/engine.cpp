@@ -1076,7 +1076,9 @@ struct ca3dmm_sim {
     std::vector<ca3dmm_engine_p> eng;
     int grouped = 0;              // Cannon rounds + k fold as one persistent launch
     int use_graph = 0;            // replay that launch from a captured hipGraph
+    int ozaki = 0;                // > 0: the local products through the CRT form with this many moduli
     ca3::grouped::Plan *plan = nullptr;
+    ca3::grouped::OzPlan *ozplan = nullptr;
 };
 
 namespace {
@@ -1092,12 +1094,11 @@ int cannon_member(const Problem &P, int r, int row, int col)
 // Problems and fold groups of the persistent grouped launch: one problem
 // per active rank (its q Cannon rounds as k-segments read in place), one
 // fold group per mn-position when kp > 1.
-int build_grouped(ca3dmm_sim *S)
+int collect_grouped(ca3dmm_sim *S, std::vector<ca3::grouped::Problem> &probs,
+                    std::vector<ca3::grouped::Fold> &folds)
 {
     const Problem &P = S->P;
     namespace G = ca3::grouped;
-    std::vector<G::Problem> probs;
-    std::vector<G::Fold> folds;
     const int mn = P.grid.mp * P.grid.np, kp = P.grid.kp;
     std::vector<int> fold_of(mn, -1);
     if (kp > 1) {
@@ -1148,7 +1149,23 @@ int build_grouped(ca3dmm_sim *S)
         pr.fold = (kp > 1) ? fold_of[r % mn] : -1;
         probs.push_back(pr);
     }
-    return G::build(probs, folds, &S->plan);
+    return 0;
+}
+
+int build_grouped(ca3dmm_sim *S)
+{
+    std::vector<ca3::grouped::Problem> probs;
+    std::vector<ca3::grouped::Fold> folds;
+    int rc = collect_grouped(S, probs, folds);
+    return rc ? rc : ca3::grouped::build(probs, folds, &S->plan);
+}
+
+int build_ozaki(ca3dmm_sim *S)
+{
+    std::vector<ca3::grouped::Problem> probs;
+    std::vector<ca3::grouped::Fold> folds;
+    int rc = collect_grouped(S, probs, folds);
+    return rc ? rc : ca3::grouped::build_ozaki(probs, folds, S->ozaki, &S->ozplan);
 }
 
 int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
@@ -1231,9 +1248,16 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
         if (rc) return rc;
     }
 
-    // 3.+4. as one persistent grouped launch (rounds accumulated in
-    //    registers, fold done by the last layer of each tile)
-    if (S->grouped) {
+    // 3.+4. through the CRT form of the Ozaki scheme: every rank's product
+    //    over all its rounds, the k-layer fold in layer order on the
+    //    reconstructed values (products in one persistent launch when grouped)
+    const bool fp64_rounds = !S->ozaki;
+    if (S->ozaki) {
+        if (!S->ozplan && (rc = build_ozaki(S))) return rc;
+        if ((rc = ca3::grouped::run_ozaki(S->ozplan, S->grouped != 0, S->use_graph != 0))) return rc;
+    } else if (S->grouped) {
+        // 3.+4. as one persistent grouped launch (rounds accumulated in
+        //    registers, fold done by the last layer of each tile)
         if (!S->plan && (rc = build_grouped(S))) return rc;
         if ((rc = ca3::grouped::run(S->plan, S->use_graph != 0))) return rc;
     }
@@ -1241,7 +1265,7 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     // 3. Cannon rounds: round s of rank (row, col) multiplies the A block of
     //    (row, (row+col+s) % q) by the B block of ((row+col+s) % q, col),
     //    beta = 0 in round 0 and 1 afterwards (cannon.c:319-433).
-    for (int r = 0; r < p && !rc && !S->grouped; ++r) {
+    for (int r = 0; r < p && !rc && !S->grouped && fp64_rounds; ++r) {
         ca3dmm_engine_p e = S->eng[r];
         if (!e->is_active) continue;
         const cannon_engine_p ce = e->cannon_engine;
@@ -1263,7 +1287,7 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     if (rc) return rc;
 
     // 4. k-layer reduction in rank (layer) order (ca3dmm.c:1287-1335)
-    for (int r = 0; r < p && !rc && !S->grouped; ++r) {
+    for (int r = 0; r < p && !rc && !S->grouped && fp64_rounds; ++r) {
         ca3dmm_engine_p e = S->eng[r];
         if (!e->is_active || e->task_k_num == 1 || !e->C_out_nrow || !e->C_out_ncol) continue;
         const int kp = e->task_k_num, mn = P.grid.mp * P.grid.np;
@@ -1373,6 +1397,18 @@ int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph)
     return HIP_PROXY_OK;
 }
 
+int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli)
+{
+    if (!sim || nmoduli < 0 || (nmoduli > 0 && (nmoduli < 4 || nmoduli > 20))) return HIP_PROXY_ERR_ARG;
+    if (nmoduli != sim->ozaki && sim->ozplan) {
+        (void) hip_stream_sync(nullptr);
+        ca3::grouped::destroy_ozaki(sim->ozplan);
+        sim->ozplan = nullptr;
+    }
+    sim->ozaki = nmoduli;
+    return HIP_PROXY_OK;
+}
+
 int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu)
 {
     if (!sim) return HIP_PROXY_ERR_ARG;
@@ -1403,6 +1439,10 @@ void ca3dmm_sim_free(ca3dmm_sim_p *sim)
 {
     if (!sim || !*sim) return;
     if ((*sim)->plan) ca3::grouped::destroy((*sim)->plan);
+    if ((*sim)->ozplan) {
+        (void) hip_stream_sync(nullptr);
+        ca3::grouped::destroy_ozaki((*sim)->ozplan);
+    }
     for (auto &e : (*sim)->eng) ca3dmm_engine_free(&e);
     delete *sim;
     *sim = nullptr;

@@ -54,5 +54,17 @@ const Tuning &tuning(const Plan *p);
 void set_tuning(Plan *p, const Tuning &t);
 void destroy(Plan *p);
 
+// The same problems and fold groups through the CRT form of the Ozaki
+// scheme with nmod moduli: each problem's product over all its k-segments
+// (one exact INT8 product per modulus), reconstructed in FP64 and, for a
+// fold group, added over the layers in layer order (ozaki.hip).
+struct OzPlan;
+int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &folds, int nmod, OzPlan **out);
+// grouped: the products of every problem in one persistent launch fed by a
+// device queue (else one launch per problem); use_graph replays the whole
+// sequence from a captured hipGraph.
+int run_ozaki(OzPlan *p, bool grouped, bool use_graph);
+void destroy_ozaki(OzPlan *p);
+
 }  // namespace grouped
 }  // namespace ca3

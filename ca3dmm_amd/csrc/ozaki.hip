@@ -30,11 +30,13 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <type_traits>
 #include <utility>
 
 #include "ca3dmm_ext.h"
+#include "grouped.h"
 #include "hip_common.h"
 
 namespace {
@@ -55,16 +57,16 @@ constexpr int kExpEmpty = INT_MIN;
 // Line l of the operand: element (l, kk) at X[l*sl + kk*sk].  mx[l] holds
 // the bits of the largest |x| seen (non-negative doubles order like their
 // bit patterns, so an integer atomicMax is exact); 0 means an all-zero line.
-__global__ __launch_bounds__(256) void ozaki_max_kernel(const double *__restrict__ X, int lines, int K,
-                                                         int64_t sl, int64_t sk, unsigned long long *mx)
+// Block (bx, by) of a gx x gy grid: 64 lines x (K / gy chunk); the thread
+// layout follows the contiguous direction so loads coalesce.
+__device__ __forceinline__ void oz_max_body(int bx, int by, int gy, const double *__restrict__ X, int lines, int K,
+                                            int64_t sl, int64_t sk, unsigned long long *mx)
 {
-    // block: 64 lines x (K / gridDim.y chunk); thread layout follows the
-    // contiguous direction so loads coalesce
     const bool line_contig = (sl == 1);
     const int t = threadIdx.x;
-    const int kchunk = (K + gridDim.y - 1) / gridDim.y;
-    const int k0 = blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
-    const int l0 = blockIdx.x * 64;
+    const int kchunk = (K + gy - 1) / gy;
+    const int k0 = by * kchunk, k1 = min(K, k0 + kchunk);
+    const int l0 = bx * 64;
     __shared__ double red[256];
     double m = 0.0;
     if (line_contig) {
@@ -88,6 +90,12 @@ __global__ __launch_bounds__(256) void ozaki_max_kernel(const double *__restrict
         }
         if (l < lines && v > 0.0) atomicMax(&mx[l], (unsigned long long) __double_as_longlong(v));
     }
+}
+
+__global__ __launch_bounds__(256) void ozaki_max_kernel(const double *__restrict__ X, int lines, int K,
+                                                         int64_t sl, int64_t sk, unsigned long long *mx)
+{
+    oz_max_body(blockIdx.x, blockIdx.y, gridDim.y, X, lines, K, sl, sk, mx);
 }
 
 // E[l] = kExpEmpty for a zero line, else e + 1 or e + 2 as above.
@@ -236,6 +244,12 @@ struct OzArgs {
     // CRT form: residue planes R_t (int8, Mp x Np, ld Mp), s = number of moduli
     int8_t *R8;
     int64_t rplane;
+    // grouped CRT launch (in the launch's own arguments): first queue item
+    // of each problem, the 8 per-XCD counters (the per-problem OzArgs come
+    // as a separate kernel argument)
+    const int *tile0;
+    int nprob, total;
+    int *queue;
 };
 
 // 16-byte chunk swizzle within a 64-byte LDS row.  An MFMA operand read
@@ -798,16 +812,15 @@ __device__ __forceinline__ unsigned long long crt_norm_term(double x, int E)
     return ((m2 - 1) >> min(-sh, 62)) + 1;
 }
 
-__global__ __launch_bounds__(256) void ozaki2_norm_kernel(const double *__restrict__ X, int lines, int K,
-                                                           int64_t sl, int64_t sk,
-                                                           const unsigned long long *__restrict__ mx,
-                                                           unsigned long long *S)
+__device__ __forceinline__ void oz2_norm_body(int bx, int by, int gy, const double *__restrict__ X, int lines,
+                                              int K, int64_t sl, int64_t sk,
+                                              const unsigned long long *__restrict__ mx, unsigned long long *S)
 {
     const bool line_contig = (sl == 1);
     const int t = threadIdx.x;
-    const int kchunk = (K + gridDim.y - 1) / gridDim.y;
-    const int k0 = blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
-    const int l0 = blockIdx.x * 64;
+    const int kchunk = (K + gy - 1) / gy;
+    const int k0 = by * kchunk, k1 = min(K, k0 + kchunk);
+    const int l0 = bx * 64;
     __shared__ unsigned long long red[256];
     unsigned long long acc = 0;
     const int myline = line_contig ? l0 + (t & 63) : l0 + (t >> 2);
@@ -832,6 +845,14 @@ __global__ __launch_bounds__(256) void ozaki2_norm_kernel(const double *__restri
         }
         if (l < lines && v != 0) atomicAdd(&S[l], v);
     }
+}
+
+__global__ __launch_bounds__(256) void ozaki2_norm_kernel(const double *__restrict__ X, int lines, int K,
+                                                           int64_t sl, int64_t sk,
+                                                           const unsigned long long *__restrict__ mx,
+                                                           unsigned long long *S)
+{
+    oz2_norm_body(blockIdx.x, blockIdx.y, gridDim.y, X, lines, K, sl, sk, mx, S);
 }
 
 // e = T + 19 - E - ceil(bitlen(S - 1) / 2): the largest power with
@@ -859,13 +880,13 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
 // an INT32 sum of 24-bit products below 2^30, reduced by crt_mod_i32; for
 // p_0 = 256 the residue is the low byte of x0.  Plane t:
 // P[t * plane + line * Kp + kk].
-__global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__restrict__ X, int lines, int K,
-                                                              int64_t sl, int64_t sk, const int *__restrict__ ex,
-                                                              int nmod, int8_t *__restrict__ P, int64_t plane, int Kp)
+__device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *__restrict__ X, int lines, int K,
+                                                 int64_t sl, int64_t sk, const int *__restrict__ ex, int nmod,
+                                                 int8_t *__restrict__ P, int64_t plane, int Kp)
 {
     __shared__ double tile[64][65];
     const int t = threadIdx.x;
-    const int l0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int l0 = bx * 64, k0 = by * 64;
     if (sl == 1) {
         for (int p = 0; p < 16; ++p) {
             const int li = t & 63, ki = (t >> 6) + 4 * p;
@@ -928,6 +949,13 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
     }
 }
 
+__global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__restrict__ X, int lines, int K,
+                                                              int64_t sl, int64_t sk, const int *__restrict__ ex,
+                                                              int nmod, int8_t *__restrict__ P, int64_t plane, int Kp)
+{
+    oz2_residue_body(blockIdx.x, blockIdx.y, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
+}
+
 // Epilogue of a CRT segment (modulus t, one k-chunk): R_t = (R_t + Z) mod p_t
 // (R_t = Z mod p_t on the first chunk), over the whole padded tile.
 template <int WTM, int WTN, int FI, int FJ>
@@ -957,11 +985,45 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
     }
 }
 
-// Per element (4 consecutive rows per thread): the N residues r_t, Garner's
-// balanced mixed-radix digits v_t = (r_t - sum_{j<t} v_j M_j) M_t^-1 mod p_t
-// (M_j = p_0 ... p_{j-1}; FP32 and exact: the sum stays below 2^19 and is
-// reduced before the product), x = v_0 + p_0 (v_1 + p_1 (...)) in FP64 from
-// the top digit, C = alpha 2^-eA 2^-eB x (+ beta C).
+// Per element (4 consecutive rows i0..i0+3 of column j): the N residues r_t,
+// Garner's balanced mixed-radix digits v_t = (r_t - sum_{j<t} v_j M_j) M_t^-1
+// mod p_t (M_j = p_0 ... p_{j-1}; FP32 and exact: the sum stays below 2^19
+// and is reduced before the product), then x = v_0 + p_0 (v_1 + p_1 (...))
+// in FP64 from the top digit and 2^-eA 2^-eB x (0 for an empty line).
+template <int NM>
+__device__ __forceinline__ void oz2_crt_column(const int8_t *__restrict__ R, int64_t rplane, int Mp, int i0, int j,
+                                               const int *__restrict__ eA, int eb, double (&out)[4])
+{
+    float v[4][NM];
+#pragma unroll
+    for (int t = 0; t < NM; ++t) {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(R + (int64_t) t * rplane + (int64_t) j * Mp + i0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q][t] = (float) (int8_t) (w >> (8 * q));
+    }
+#pragma unroll
+    for (int t = 1; t < NM; ++t) {
+        const float p = c_crt.p[t], ip = c_crt.invp[t], gi = c_crt.ginv[t];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float sum = v[q][0];                           // M_0 = 1
+#pragma unroll
+            for (int s = 1; s < t; ++s) sum = fmaf(v[q][s], c_crt.gmul[t][s], sum);
+            v[q][t] = crt_balf((v[q][t] - crt_balf(sum, p, ip)) * gi, p, ip);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double x = (double) v[q][NM - 1];
+#pragma unroll
+        for (int t = NM - 2; t >= 0; --t) x = ca3::add_rn(ca3::mul_rn(x, c_crt.pd[t]), (double) v[q][t]);
+        const int ea = eA[i0 + q];
+        out[q] = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(x, -ea), -eb) : 0.0;
+    }
+}
+
+// C = alpha * (the reconstructed product) (+ beta C); rows of the padded
+// planes past M are read (they hold zeros) but not written.
 template <int NM>
 __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restrict__ R, int64_t rplane, int Mp, int M,
                                                           int N, const int *__restrict__ eA,
@@ -971,40 +1033,137 @@ __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restric
     const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
     if (i0 >= M) return;
     for (int j = blockIdx.y; j < N; j += gridDim.y) {
-        float v[4][NM];
-#pragma unroll
-        for (int t = 0; t < NM; ++t) {
-            const uint32_t w = *reinterpret_cast<const uint32_t *>(R + (int64_t) t * rplane + (int64_t) j * Mp + i0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[q][t] = (float) (int8_t) (w >> (8 * q));
-        }
-#pragma unroll
-        for (int t = 1; t < NM; ++t) {
-            const float p = c_crt.p[t], ip = c_crt.invp[t], gi = c_crt.ginv[t];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float sum = v[q][0];                       // M_0 = 1
-#pragma unroll
-                for (int s = 1; s < t; ++s) sum = fmaf(v[q][s], c_crt.gmul[t][s], sum);
-                v[q][t] = crt_balf((v[q][t] - crt_balf(sum, p, ip)) * gi, p, ip);
-            }
-        }
-        const int eb = eB[j];
+        double x[4];
+        oz2_crt_column<NM>(R, rplane, Mp, i0, j, eA, eB[j], x);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = i0 + q;
             if (i >= M) break;
-            double x = (double) v[q][NM - 1];
-#pragma unroll
-            for (int t = NM - 2; t >= 0; --t) x = ca3::add_rn(ca3::mul_rn(x, c_crt.pd[t]), (double) v[q][t]);
-            const int ea = eA[i];
-            x = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(x, -ea), -eb) : 0.0;
             double *cp = C + (int64_t) j * ldc + i;
-            x = ca3::mul_rn(alpha, x);
-            if (beta != 0.0) x = ca3::add_rn(x, ca3::mul_rn(beta, *cp));
-            *cp = x;
+            double v = ca3::mul_rn(alpha, x[q]);
+            if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
+            *cp = v;
         }
     }
+}
+
+// Grouped reconstruction: one job per fold group of the simulated grid (kp
+// layer products of the same C block, added in layer order, each column
+// written to its owner's C_out slice) or per problem (kp = 1).
+constexpr int kGrpMaxKp = 8;
+struct CrtJob {
+    int kp, M, N, Mp, rblocks;            // rblocks: ceil(M / 1024)
+    const int8_t *R[kGrpMaxKp];           // the layers' residue planes (rplane apart)
+    int64_t rplane;
+    const int *eA[kGrpMaxKp], *eB[kGrpMaxKp];
+    int cs[kGrpMaxKp + 1];                // column slice t = [cs[t], cs[t+1]) goes to Cout[t]
+    double *Cout[kGrpMaxKp];
+    int64_t ldo;
+};
+
+template <int NM>
+__global__ __launch_bounds__(256) void ozaki2_crt_grouped_kernel(const CrtJob *__restrict__ jobs,
+                                                                  const int *__restrict__ blk0, int njobs)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const CrtJob &J = jobs[lo];
+    const int local = b - blk0[lo];
+    const int j = local / J.rblocks, i0 = (local - j * J.rblocks) * 1024 + threadIdx.x * 4;
+    if (i0 >= J.M) return;
+    double sum[4];
+    for (int l = 0; l < J.kp; ++l) {
+        double x[4];
+        oz2_crt_column<NM>(J.R[l], J.rplane, J.Mp, i0, j, J.eA[l], J.eB[l][j], x);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum[q] = (l == 0) ? x[q] : ca3::add_rn(sum[q], x[q]);
+    }
+    int sl = 0;
+    while (j >= J.cs[sl + 1]) ++sl;
+    double *out = J.Cout[sl] + (int64_t) (j - J.cs[sl]) * J.ldo;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (i0 + q < J.M) out[i0 + q] = sum[q];
+}
+
+// Grouped preparation: one job per (problem, operand, k-segment); kind 0
+// line maxima, 1 norm sums, 2 residue planes.  A block finds its job in the
+// prefix of block counts.
+struct PrepJob {
+    const double *X;
+    int64_t sl, sk;                        // element (line, kk) at X[line * sl + kk * sk]
+    int lines, K;
+    int rbx, rby;                          // grid of the max / norm passes
+    int sbx, sby;                          // grid of the residue pass
+    unsigned long long *mx, *S;            // this operand's lines
+    int *ex;
+    int8_t *P;                             // planes, already offset to the segment's k
+    int64_t plane;
+    int Kp;
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void ozaki2_prep_grouped_kernel(const PrepJob *__restrict__ jobs,
+                                                                   const int *__restrict__ blk0, int njobs,
+                                                                   int nmod)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const PrepJob &J = jobs[lo];
+    const int local = b - blk0[lo];
+    if (KIND == 0) {
+        oz_max_body(local % J.rbx, local / J.rbx, J.rby, J.X, J.lines, J.K, J.sl, J.sk, J.mx);
+    } else if (KIND == 1) {
+        oz2_norm_body(local % J.rbx, local / J.rbx, J.rby, J.X, J.lines, J.K, J.sl, J.sk, J.mx, J.S);
+    } else {
+        oz2_residue_body(local % J.sbx, local / J.sbx, J.X, J.lines, J.K, J.sl, J.sk, J.ex, nmod, J.P, J.plane,
+                         J.Kp);
+    }
+}
+
+// The fields a CRT segment loop reads, copied through readfirstlane so they
+// live in scalar registers (as kernel arguments do) and the loop's address
+// arithmetic stays scalar.
+__device__ __forceinline__ int64_t oz_rfl64(int64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) ((uint64_t) v >> 32));
+    return (int64_t) (((uint64_t) hi << 32) | lo);
+}
+
+template <class T>
+__device__ __forceinline__ T *oz_rflp(T *p)
+{
+    return (T *) (uintptr_t) oz_rfl64((int64_t) (uintptr_t) p);
+}
+
+__device__ __forceinline__ OzArgs oz_uniform_crt_args(const OzArgs &a)
+{
+    OzArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = __builtin_amdgcn_readfirstlane(a.M);
+    g.N = __builtin_amdgcn_readfirstlane(a.N);
+    g.Mp = __builtin_amdgcn_readfirstlane(a.Mp);
+    g.Np = __builtin_amdgcn_readfirstlane(a.Np);
+    g.Kp = __builtin_amdgcn_readfirstlane(a.Kp);
+    g.s = __builtin_amdgcn_readfirstlane(a.s);
+    g.nchunks = __builtin_amdgcn_readfirstlane(a.nchunks);
+    g.kcb = __builtin_amdgcn_readfirstlane(a.kcb);
+    g.PA = oz_rflp(a.PA);
+    g.PB = oz_rflp(a.PB);
+    g.planeA = oz_rfl64(a.planeA);
+    g.planeB = oz_rfl64(a.planeB);
+    g.R8 = oz_rflp(a.R8);
+    g.rplane = oz_rfl64(a.rplane);
+    return g;
 }
 
 // Software-pipelined form (the default).  Direct-to-LDS loads fill a ring
@@ -1045,11 +1204,12 @@ __device__ __forceinline__ void oz_mfmas(const i32x4 (&af)[FI], const i32x4 (&bf
 // NW waves per workgroup in a 2 x NW/2 grid (wave w: rows (w & 1) * WTM,
 // columns (w >> 1) * WTN); with NW = 8 two waves share each SIMD and cover
 // each other's waits.
-template <bool SPLIT, int WTM, int WTN, bool FUSED, int NW, bool CRT = false>
-__global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
+template <bool SPLIT, int WTM, int WTN, bool FUSED, int NW, bool CRT = false, bool GROUPED = false>
+__global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const OzArgs *table = nullptr)
 {
     static_assert(!(SPLIT && FUSED), "the split form has its own fold");
     static_assert(!CRT || (FUSED && !SPLIT), "the CRT form runs fused");
+    static_assert(!GROUPED || CRT, "the grouped launch runs the CRT form");
     constexpr int NBUF = 4;
     constexpr int kBMo = 2 * WTM, kBNo = (NW / 2) * WTN;
     constexpr int FI = WTM / 16, FJ = WTN / 16;
@@ -1068,9 +1228,9 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);     // uniform: LDS targets stay scalar
     const int wm = wave & 1, wn = wave >> 1;
-    int tm, tn, d, kb0, nkb;
-    int64_t part_off;
-    if (!oz_item<SPLIT, kBMo, kBNo>(g, tm, tn, d, kb0, nkb, part_off)) return;
+    // one output tile (tm, tn) of the product described by g
+    auto body = [&](const OzArgs &g, const int tm, const int tn, int d, int kb0, int nkb, const int64_t part_off)
+        __attribute__((always_inline)) {
     const int m0 = tm * kBMo, n0 = tn * kBNo;
 
     const int lr = lane >> 2;
@@ -1219,6 +1379,53 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g)
     } else {
         oz_epilogue<SPLIT, WTM, WTN>(g, acc, m0, n0, d, part_off, first, last);
     }
+    }
+    };
+
+    if constexpr (GROUPED) {
+        // persistent: items (problem, tile), problem p owning items
+        // [tile0[p], tile0[p + 1]).  The items are cut into 8 contiguous
+        // ranges, one per XCD (workgroup b runs on XCD b % 8), each with its
+        // own counter in queue[0..7], so neighbouring tiles share an L2; a
+        // workgroup whose range is used up takes items from the next ranges.
+        // Every workgroup leaves once all 8 ranges are used up.
+        __shared__ int s_item;
+        const int xcd = blockIdx.x & 7;
+        for (;;) {
+            if (tid == 0) {
+                int got = -1;
+                for (int v = 0; v < 8 && got < 0; ++v) {
+                    const int x = (xcd + v) & 7;
+                    const int lo = (int) ((int64_t) g0.total * x / 8), hi = (int) ((int64_t) g0.total * (x + 1) / 8);
+                    if (lo >= hi) continue;
+                    const int my = lo + atomicAdd(&g0.queue[x], 1);
+                    if (my < hi) got = my;
+                }
+                s_item = got;
+            }
+            __syncthreads();
+            // uniform values in scalar registers: the problem's fields then
+            // come by scalar loads and its address arithmetic stays scalar
+            const int item = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+            if (item < 0) break;
+            int lo = 0, hi = g0.nprob - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (g0.tile0[mid] <= item) lo = mid; else hi = mid - 1;
+            }
+            const OzArgs g = oz_uniform_crt_args(table[__builtin_amdgcn_readfirstlane(lo)]);
+            const int t = item - g0.tile0[lo];
+            const int tiles_m = g.Mp / kBMo;
+            const int G = 4, per = G * (g.Np / kBNo), grp = t / per, first = grp * G;
+            const int gsz = min(tiles_m - first, G), rr = t - grp * per;
+            body(g, first + rr % gsz, rr / gsz, 0, 0, 0, 0);
+        }
+    } else {
+        int tm, tn, d, kb0, nkb;
+        int64_t part_off;
+        if (!oz_item<SPLIT, kBMo, kBNo>(g0, tm, tn, d, kb0, nkb, part_off)) return;
+        body(g0, tm, tn, d, kb0, nkb, part_off);
     }
 }
 
@@ -1533,6 +1740,313 @@ int ozaki2_core(int ta, int tb, int m, int n, int k, double alpha, const double 
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------
+// Grouped CRT form for the simulated grid (declared in grouped.h)
+// ---------------------------------------------------------------------
+namespace ca3 {
+namespace grouped {
+
+struct OzPlan {
+    int nmod = 0, nprob = 0, total = 0, nlines = 0;
+    int njobs = 0, rblocks = 0, sblocks = 0, ncrt = 0, crtblocks = 0;
+    void *planes = nullptr;                  // operand residue planes and product residue planes
+    unsigned long long *words = nullptr;     // [2 * nlines]: line maxima, then norm sums
+    int *e = nullptr;                        // [nlines] line exponents
+    int *queue = nullptr;
+    PrepJob *d_jobs = nullptr;
+    int *d_blk_r = nullptr, *d_blk_s = nullptr;
+    OzArgs *d_args = nullptr;
+    int *d_tile0 = nullptr;
+    CrtJob *d_crt = nullptr;
+    int *d_crt_blk = nullptr;
+    std::vector<OzArgs> h_args;              // the per-problem launches
+    std::vector<int> h_tiles;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_grouped = false;
+};
+
+namespace {
+
+void drop_oz_graph(OzPlan *p)
+{
+    if (p->exec) { (void) hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+    if (p->graph) { (void) hipGraphDestroy(p->graph); p->graph = nullptr; }
+}
+
+template <int... Ns>
+void launch_crt_grouped_n(int nmod, std::integer_sequence<int, Ns...>, int blocks, hipStream_t st,
+                          const CrtJob *jobs, const int *blk0, int njobs)
+{
+    ((nmod == Ns + kCrtMin ? (void) (ozaki2_crt_grouped_kernel<Ns + kCrtMin><<<blocks, 256, 0, st>>>(jobs, blk0,
+                                                                                                   njobs))
+                           : void()), ...);
+}
+
+int oz_sequence(OzPlan *p, bool grouped, hipStream_t st)
+{
+    fill_int_kernel<<<std::max(1, std::min(4096, (4 * p->nlines + 255) / 256)), 256, 0, st>>>((int *) p->words,
+                                                                                               4 * p->nlines, 0);
+    fill_int_kernel<<<1, 64, 0, st>>>(p->queue, 8, 0);
+    if (p->rblocks) {
+        ozaki2_prep_grouped_kernel<0><<<p->rblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_r, p->njobs, p->nmod);
+        ozaki2_prep_grouped_kernel<1><<<p->rblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_r, p->njobs, p->nmod);
+    }
+    ozaki2_scale_kernel<<<(p->nlines + 255) / 256, 256, 0, st>>>(p->words, p->words + p->nlines, p->nlines,
+                                                                  kCrtT[p->nmod], p->e);
+    if (p->sblocks)
+        ozaki2_prep_grouped_kernel<2><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+    if (grouped) {
+        OzArgs g0;
+        memset(&g0, 0, sizeof(g0));
+        g0.tile0 = p->d_tile0;
+        g0.nprob = p->nprob;
+        g0.total = p->total;
+        g0.queue = p->queue;
+        // one 512-thread workgroup per CU (128 KB of LDS each); every one
+        // leaves when the queue passes the last item
+        const int grid = std::max(1, std::min(p->total, ca3::cu_count()));
+        if (p->total) ozaki_pipe_kernel<false, 128, 64, true, 8, true, true><<<grid, 512, 0, st>>>(g0, p->d_args);
+    } else {
+        for (int r = 0; r < p->nprob; ++r)
+            if (p->h_tiles[r])
+                ozaki_pipe_kernel<false, 128, 64, true, 8, true, false><<<p->h_tiles[r], 512, 0, st>>>(p->h_args[r]);
+    }
+    if (p->crtblocks)
+        launch_crt_grouped_n(p->nmod, std::make_integer_sequence<int, kCrtMax - kCrtMin + 1>{}, p->crtblocks, st,
+                             p->d_crt, p->d_crt_blk, p->ncrt);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ca3::fail(e, "grouped ozaki launch");
+}
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &v, hipStream_t st)
+{
+    hipError_t e = hipMalloc((void **) dst, sizeof(T) * std::max<size_t>(1, v.size()));
+    if (e == hipSuccess && !v.empty()) e = hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st);
+    return e;
+}
+
+}  // namespace
+
+int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &folds, int nmod, OzPlan **out)
+{
+    *out = nullptr;
+    if (nmod < kCrtMin || nmod > kCrtMax)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "grouped ozaki: nmoduli %d outside %d..%d", nmod, kCrtMin, kCrtMax);
+    auto *p = new OzPlan;
+    p->nmod = nmod;
+    p->nprob = (int) probs.size();
+    struct Lay {
+        int Mp, Np, Kp, lineA, lineB;
+        size_t pa, pb, r;
+    };
+    std::vector<Lay> lay(p->nprob);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t at = off; off += (bytes + 255) / 256 * 256; return at; };
+    std::vector<std::vector<int>> kofs(p->nprob);
+    for (int i = 0; i < p->nprob; ++i) {
+        const Problem &q = probs[i];
+        if (q.nseg < 0 || q.nseg > kMaxSeg || q.M < 0 || q.N < 0) { delete p; return HIP_PROXY_ERR_ARG; }
+        Lay &L = lay[i];
+        L.Mp = (int) round_up(std::max(q.M, 1), kTile);
+        L.Np = (int) round_up(std::max(q.N, 1), kTile);
+        int64_t kk = 0;
+        for (int s2 = 0; s2 < q.nseg; ++s2) {
+            kofs[i].push_back((int) kk);
+            kk += round_up(q.seg[s2].k, 64);
+        }
+        if (kk > kCrtKMax) { delete p; return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "grouped ozaki: k above 2^23"); }
+        L.Kp = (int) round_up(std::max<int64_t>(kk, 1), kKp);
+        L.pa = take((size_t) nmod * L.Mp * L.Kp);
+        L.pb = take((size_t) nmod * L.Np * L.Kp);
+        L.r = take((size_t) nmod * L.Mp * L.Np);
+        L.lineA = p->nlines;
+        p->nlines += L.Mp;
+        L.lineB = p->nlines;
+        p->nlines += L.Np;
+    }
+    hipStream_t st = ca3::stream();
+    hipError_t e = hipMalloc(&p->planes, std::max<size_t>(off, 256));
+    if (e == hipSuccess) e = hipMalloc((void **) &p->words, sizeof(unsigned long long) * 2 * std::max(1, p->nlines));
+    if (e == hipSuccess) e = hipMalloc((void **) &p->e, sizeof(int) * std::max(1, p->nlines));
+    if (e == hipSuccess) e = hipMalloc((void **) &p->queue, 8 * sizeof(int));
+    if (e != hipSuccess) { destroy_ozaki(p); return ca3::fail(e, "grouped ozaki workspace"); }
+    int8_t *base = (int8_t *) p->planes;
+    unsigned long long *mx = p->words, *S = p->words + p->nlines;
+
+    std::vector<PrepJob> jobs;
+    std::vector<int> blk_r(1, 0), blk_s(1, 0);
+    std::vector<int> tile0(1, 0);
+    std::vector<OzArgs> args;
+    std::map<const double *, int> by_c;
+    for (int i = 0; i < p->nprob; ++i) {
+        const Problem &q = probs[i];
+        const Lay &L = lay[i];
+        by_c[q.C] = i;
+        const int nseg = std::max(q.nseg, 1);              // no segment: one zero-filling job per operand
+        for (int s2 = 0; s2 < nseg; ++s2) {
+            const bool has = s2 < q.nseg;
+            const int k = has ? q.seg[s2].k : 0;
+            const int ko = has ? kofs[i][s2] : 0;
+            const bool lastseg = (s2 == nseg - 1);
+            const int kext = lastseg ? L.Kp - ko : (int) round_up(k, 64);
+            for (int op = 0; op < 2; ++op) {
+                PrepJob J;
+                memset(&J, 0, sizeof(J));
+                J.K = k;
+                J.lines = op ? q.N : q.M;
+                if (op == 0) { J.X = has ? q.seg[s2].A : nullptr; J.sl = 1; J.sk = has ? q.seg[s2].lda : 1; }
+                else { J.X = has ? q.seg[s2].B : nullptr; J.sl = has ? q.seg[s2].ldb : 1; J.sk = 1; }
+                const int line0 = op ? L.lineB : L.lineA, Lp = op ? L.Np : L.Mp;
+                J.rbx = (J.lines + 63) / 64;
+                J.rby = (k > 0 && J.lines > 0) ? std::min(64, std::max(1, k / 512)) : 0;
+                J.sbx = Lp / 64;
+                J.sby = kext / 64;
+                J.mx = mx + line0;
+                J.S = S + line0;
+                J.ex = p->e + line0;
+                J.P = base + (op ? L.pb : L.pa) + ko;
+                J.plane = (int64_t) Lp * L.Kp;
+                J.Kp = L.Kp;
+                jobs.push_back(J);
+                blk_r.push_back(blk_r.back() + J.rbx * J.rby);
+                blk_s.push_back(blk_s.back() + J.sbx * J.sby);
+            }
+        }
+        OzArgs g;
+        memset(&g, 0, sizeof(g));
+        g.M = q.M; g.N = q.N; g.Mp = L.Mp; g.Np = L.Np; g.Kp = L.Kp;
+        g.PA = base + L.pa; g.PB = base + L.pb;
+        g.planeA = (int64_t) L.Mp * L.Kp; g.planeB = (int64_t) L.Np * L.Kp;
+        g.eA = p->e + L.lineA; g.eB = p->e + L.lineB;
+        g.s = nmod;
+        g.nchunks = (L.Kp + kCrtChunk - 1) / kCrtChunk;
+        g.kcb = kCrtChunk / kKb;
+        g.R8 = base + L.r;
+        g.rplane = (int64_t) L.Mp * L.Np;
+        args.push_back(g);
+        const int tiles = (q.M > 0 && q.N > 0) ? (L.Mp / 256) * (L.Np / 256) : 0;
+        p->h_tiles.push_back(tiles);
+        tile0.push_back(tile0.back() + tiles);
+    }
+    p->h_args = args;
+    p->total = tile0.back();
+    p->njobs = (int) jobs.size();
+    p->rblocks = blk_r.back();
+    p->sblocks = blk_s.back();
+    blk_r.pop_back();
+    blk_s.pop_back();
+    tile0.pop_back();
+
+    // reconstruction jobs: fold groups, then problems outside any fold group
+    std::vector<CrtJob> crt;
+    std::vector<int> cblk(1, 0);
+    std::vector<char> folded(p->nprob, 0);
+    auto add_job = [&](CrtJob &J) {
+        J.rblocks = (J.M + 1023) / 1024;
+        crt.push_back(J);
+        cblk.push_back(cblk.back() + J.rblocks * J.N);
+    };
+    for (const Fold &f : folds) {
+        if (f.kp < 1 || f.kp > kGrpMaxKp || f.kp > kMaxKp) { destroy_ozaki(p); return HIP_PROXY_ERR_ARG; }
+        CrtJob J;
+        memset(&J, 0, sizeof(J));
+        J.kp = f.kp; J.M = f.M; J.N = f.N;
+        for (int l = 0; l < f.kp; ++l) {
+            auto it = by_c.find(f.Cl[l]);
+            if (it == by_c.end() || probs[it->second].M != f.M || probs[it->second].N != f.N) {
+                destroy_ozaki(p);
+                return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "grouped ozaki: fold layer without its product");
+            }
+            const int i = it->second;
+            folded[i] = 1;
+            J.Mp = lay[i].Mp;
+            J.rplane = (int64_t) lay[i].Mp * lay[i].Np;
+            J.R[l] = base + lay[i].r;
+            J.eA[l] = p->e + lay[i].lineA;
+            J.eB[l] = p->e + lay[i].lineB;
+            J.Cout[l] = f.Cout[l];
+            J.cs[l] = f.cs[l];
+        }
+        J.cs[f.kp] = f.cs[f.kp];
+        J.ldo = f.ldo;
+        if (J.M > 0 && J.N > 0) add_job(J);
+    }
+    for (int i = 0; i < p->nprob; ++i) {
+        const Problem &q = probs[i];
+        if (folded[i] || q.M == 0 || q.N == 0) continue;
+        CrtJob J;
+        memset(&J, 0, sizeof(J));
+        J.kp = 1; J.M = q.M; J.N = q.N; J.Mp = lay[i].Mp;
+        J.rplane = (int64_t) lay[i].Mp * lay[i].Np;
+        J.R[0] = base + lay[i].r;
+        J.eA[0] = p->e + lay[i].lineA;
+        J.eB[0] = p->e + lay[i].lineB;
+        J.Cout[0] = q.C;
+        J.cs[0] = 0;
+        J.cs[1] = q.N;
+        J.ldo = q.ldc;
+        add_job(J);
+    }
+    p->ncrt = (int) crt.size();
+    p->crtblocks = cblk.back();
+    cblk.pop_back();
+
+    e = upload(&p->d_jobs, jobs, st);
+    if (e == hipSuccess) e = upload(&p->d_blk_r, blk_r, st);
+    if (e == hipSuccess) e = upload(&p->d_blk_s, blk_s, st);
+    if (e == hipSuccess) e = upload(&p->d_args, args, st);
+    if (e == hipSuccess) e = upload(&p->d_tile0, tile0, st);
+    if (e == hipSuccess) e = upload(&p->d_crt, crt, st);
+    if (e == hipSuccess) e = upload(&p->d_crt_blk, cblk, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { destroy_ozaki(p); return ca3::fail(e, "grouped ozaki plan"); }
+    *out = p;
+    return 0;
+}
+
+int run_ozaki(OzPlan *p, bool grouped, bool use_graph)
+{
+    hipStream_t st = ca3::stream();
+    if (!use_graph) return oz_sequence(p, grouped, st);
+    if (p->exec && p->graph_grouped != grouped) drop_oz_graph(p);
+    if (!p->exec) {
+        CA3_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = oz_sequence(p, grouped, st);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) { if (g) (void) hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return ca3::fail(e, "hipStreamEndCapture");
+        p->graph = g;
+        CA3_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+        p->graph_grouped = grouped;
+    }
+    CA3_CHECK(hipGraphLaunch(p->exec, st));
+    return 0;
+}
+
+void destroy_ozaki(OzPlan *p)
+{
+    if (!p) return;
+    drop_oz_graph(p);
+    (void) hipFree(p->planes);
+    (void) hipFree(p->words);
+    (void) hipFree(p->e);
+    (void) hipFree(p->queue);
+    (void) hipFree(p->d_jobs);
+    (void) hipFree(p->d_blk_r);
+    (void) hipFree(p->d_blk_s);
+    (void) hipFree(p->d_args);
+    (void) hipFree(p->d_tile0);
+    (void) hipFree(p->d_crt);
+    (void) hipFree(p->d_crt_blk);
+    delete p;
+}
+
+}  // namespace grouped
+}  // namespace ca3
 
 extern "C" {
 

@@ -116,6 +116,7 @@ _SIGS = {
     "ca3dmm_sim_set_mode": (I, [P, I, I]),
     "ca3dmm_sim_autotune": (I, [P, I, IP, IP]),
     "ca3dmm_sim_set_tuning": (I, [P, I, I]),
+    "ca3dmm_sim_set_ozaki": (I, [P, I]),
 }
 for _n, (_r, _a) in _SIGS.items():
     _f = getattr(lib, _n)
@@ -168,6 +169,10 @@ class SimGrid:
         g, w = C.c_int(), C.c_int()
         check(lib.ca3dmm_sim_autotune(self.h, reps, C.byref(g), C.byref(w)), "ca3dmm_sim_autotune")
         return g.value, w.value
+
+    def set_ozaki(self, nmoduli):
+        """nmoduli > 0: the local products through the CRT form (0: FP64 MFMA)."""
+        return check(lib.ca3dmm_sim_set_ozaki(self.h, nmoduli), "ca3dmm_sim_set_ozaki")
 
     def set_tuning(self, group_rows, wg_per_cu):
         return check(lib.ca3dmm_sim_set_tuning(self.h, group_rows, wg_per_cu), "ca3dmm_sim_set_tuning")

@@ -68,6 +68,17 @@ int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per
 /* Fix the grouped launch's tuning by hand (e.g. the untuned baseline). */
 int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu);
 
+/* nmoduli > 0 (4..20): every rank's local product -- all its Cannon rounds
+ * -- runs through the CRT form of the Ozaki scheme (hip_ozaki2_dgemm's
+ * arithmetic on the concatenated k-segments), and the k-layer fold adds the
+ * reconstructed layer products in layer order.  With set_mode(grouped = 1)
+ * the products of all ranks run as one persistent launch fed by a device
+ * queue (else one launch per rank); use_graph replays the whole sequence
+ * (line scales, residues, products, reconstruction and fold) from a
+ * captured hipGraph.  0 returns to the FP64 MFMA form.  No reference
+ * counterpart (the reference's local GEMM is src/cannon.c:405-427). */
+int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli);
+
 void ca3dmm_sim_free(ca3dmm_sim_p *sim);
 
 #ifdef __cplusplus

@@ -16,7 +16,7 @@ def linear_fill(nrow, ncol, a, b):
 
 
 def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False, grouped=False,
-            graph=False, autotune=False):
+            graph=False, autotune=False, ozaki=0, repeat=1):
     """Run the simulated p-rank grid with the example_AB 1D column layout.
     A, B: full matrices as stored (numpy, any order).  Returns the full C
     gathered from the ranks' dst_C blocks.  keep: also returns the SimGrid,
@@ -35,11 +35,14 @@ def run_sim(p, m, n, k, ta, tb, A, B, proc_grid=None, is_btb=False, keep=False, 
     ldB = [B.shape[0]] * p
     Cp = [dC.at(r_[2] * mm) for r_ in w["dst_C"]]
     ldC = [mm] * p
-    if grouped:
-        sim.set_mode(grouped=True, graph=graph)
+    if ozaki:
+        sim.set_ozaki(ozaki)
+    if grouped or graph:
+        sim.set_mode(grouped=grouped, graph=graph)
         if autotune:
             sim.autotune(1)
-    sim.exec(Ap, ldA, Bp, ldB, Cp, ldC)
+    for _ in range(repeat):
+        sim.exec(Ap, ldA, Bp, ldB, Cp, ldC)
     out = dC.numpy()
     if keep:
         return out, sim, w, (Ap, ldA, Bp, ldB, Cp, ldC, dC, dA, dB)
@@ -60,3 +63,24 @@ def single_engine(m, n, k, ta, tb, dst=None, btb=False):
                                  None, E.comm_self(), E.DEV_TYPE_HIP, C.byref(eng), None)
     assert eng, "engine init failed"
     return eng
+
+
+def ozaki_grid_reference(m, n, k, p, proc_grid, opA, opB, nmod):
+    """The Ozaki form of the simulated grid, restated: every C block of the
+    2D grid is the sum over the k layers, in layer order, of the CRT-form
+    product (oracle/ozaki.py, ozaki2_dgemm) of the layer's operand blocks."""
+    from oracle import ozaki as Z
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=proc_grid)
+    out = np.full((m, n), np.nan)
+    for pl in w["plans"]:
+        if not pl.is_active or pl.rank_k != 0 or not pl.C_2dmm_nrow or not pl.C_2dmm_ncol:
+            continue
+        rs = slice(pl.C_2dmm_srow, pl.C_2dmm_srow + pl.C_2dmm_nrow)
+        cs = slice(pl.C_2dmm_scol, pl.C_2dmm_scol + pl.C_2dmm_ncol)
+        acc = None
+        for layer in range(pl.task_k_num):
+            k0, kn = O.block_spos_size(k, pl.task_k_num, layer)
+            v = Z.ozaki2_dgemm(opA[rs, k0:k0 + kn], opB[k0:k0 + kn, cs], nmod)
+            acc = v if acc is None else acc + v
+        out[rs, cs] = acc
+    return out

@@ -62,3 +62,56 @@ def test_grouped_graph_replays(hip):
         sim.exec(Ap, ldA, Bp, ldB, Cp, ldC)
         assert np.array_equal(dC.numpy(), eager), rep
     sim.free()
+
+
+def _phi(rnd, shape, phi):
+    from oracle import ozaki as Z
+    return np.asfortranarray(Z.phi_matrix(rnd, shape, phi))
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_grouped_ozaki_24_ranks_bit_exact(hip, ta, tb):
+    """The Ozaki (CRT) form of the 4x2x3 grid on 1001^3: every rank's
+    product over its two Cannon rounds in one persistent launch, the kp = 3
+    fold on the reconstructed values, bit-exact against the restated grid."""
+    from tests.simutil import ozaki_grid_reference
+    rnd = np.random.default_rng(40 + ta)
+    m = n = k = 1001
+    A = _phi(rnd, (k, m) if ta else (m, k), 2.0)
+    B = _phi(rnd, (n, k) if tb else (k, n), 2.0)
+    opA, opB = (A.T if ta else A), (B.T if tb else B)
+    want = ozaki_grid_reference(m, n, k, 24, [4, 2, 3], opA, opB, 15)
+    got = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3], grouped=True, ozaki=15)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("grid,p,N", [([2, 2, 3], 12, 8), ([2, 3, 1], 6, 16), ([1, 1, 1], 1, 20)])
+def test_grouped_ozaki_per_rank_graph_and_grids(hip, grid, p, N):
+    """Per-rank product launches, the grouped launch and its hipGraph replay
+    (replayed three times) give the same bits, on grids with and without a
+    k-layer fold."""
+    from tests.simutil import ozaki_grid_reference
+    rnd = np.random.default_rng(41 + p)
+    m, n, k = 700, 530, 900
+    A, B = _phi(rnd, (m, k), 0.5), _phi(rnd, (k, n), 0.5)
+    want = ozaki_grid_reference(m, n, k, p, grid, A, B, N)
+    a = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=grid, ozaki=N)
+    b = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=grid, grouped=True, ozaki=N)
+    c = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=grid, grouped=True, graph=True, ozaki=N, repeat=3)
+    assert np.array_equal(a, want) and np.array_equal(b, want) and np.array_equal(c, want)
+
+
+def test_grouped_ozaki_accuracy_vs_fp64_form(hip):
+    """16 moduli on the 24-rank grid: the error against the exact product
+    is no larger than the FP64 MFMA form's (integer-free inputs, phi = 0.5)."""
+    from oracle import oracle as O
+    rnd = np.random.default_rng(44)
+    m, n, k = 640, 520, 1500
+    A, B = _phi(rnd, (m, k), 0.5), _phi(rnd, (k, n), 0.5)
+    ii, jj = rnd.integers(0, m, 2000), rnd.integers(0, n, 2000)
+    ref, ab = O.dd_sample(0, 0, m, n, k, A, m, B, k, ii, jj)
+    fp = run_sim(24, m, n, k, 0, 0, A, B, proc_grid=[4, 2, 3], grouped=True)
+    oz = run_sim(24, m, n, k, 0, 0, A, B, proc_grid=[4, 2, 3], grouped=True, ozaki=16)
+    e_fp = np.max(np.abs(fp[ii, jj] - ref) / ab)
+    e_oz = np.max(np.abs(oz[ii, jj] - ref) / ab)
+    assert e_oz <= e_fp, (e_oz, e_fp)
