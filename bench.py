@@ -100,7 +100,7 @@ def ozaki_at_equal_error(hip, n, rnd):
 
     slices = search("s", range(6, 12), lambda s: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, s),
                     lambda s: s * (s + 1) // 2)
-    crt = search("moduli", range(13, 21), lambda N: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, N),
+    crt = search("moduli", range(13, 26), lambda N: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, N),
                  lambda N: N)
     hip.set_sync(True)
     out = {"inputs": "uniform[-0.5,0.5]*exp(0.5*N(0,1)), phi=0.5", "m": n, "n": n, "k": n, "samples": 4096,
@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--k", type=int, default=None, help="k of the ozaki-forms suite (default n)")
     ap.add_argument("--cases", default="NN,NT,TN,TT")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

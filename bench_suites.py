@@ -8,7 +8,7 @@ Each suite prints JSON lines (one per measurement) on stdout:
                0 alone), with achieved HBM GB/s on algorithmic bytes;
   ozaki-sweep  native FP64-MFMA DGEMM vs the two Ozaki INT8 forms at 8192^3
                and 65536 x 65536 x 1024 for phi in {0.5, 2, 4}, s in 3..11
-               slices and 10..20 CRT moduli, error against double-double;
+               slices and 10..25 CRT moduli, error against double-double;
   ozaki-forms  native DGEMM, slices s = 8 and CRT with 15 moduli at n^3
                (phi = 0.5), three timed calls each: the kernel-trace workload;
   grouped      the simulated 4x2x3 grid on 10007^3: one launch per rank and
@@ -132,7 +132,7 @@ def rank_share(args):
 # ---------------------------------------------------------------------------
 def ozaki_sweep(args):
     """Error on 4096 sampled entries against double-double, per (shape, phi,
-    form, count): the slice form for s = 3..11 and the CRT form for 10..20
+    form, count): the slice form for s = 3..11 and the CRT form for 10..25
     moduli; `no_worse_than_native` needs both error measures at or below the
     native product's."""
     from oracle import oracle as O
@@ -140,7 +140,7 @@ def ozaki_sweep(args):
     rnd = np.random.default_rng(7)
     shapes = [(args.n, args.n, args.n), (65536, 65536, 1024)]
     forms = [("ozaki-int8", "s", range(3, 12), hip.ozaki_dgemm, Z.products),
-             ("ozaki-int8-crt", "moduli", range(10, 21), hip.ozaki2_dgemm, lambda c: c)]
+             ("ozaki-int8-crt", "moduli", range(10, 26), hip.ozaki2_dgemm, lambda c: c)]
     for m, n, k in shapes:
         flops = 2.0 * m * n * k
         for phi in (0.5, 2.0, 4.0):
@@ -177,28 +177,30 @@ def ozaki_sweep(args):
 def ozaki_forms(args):
     """One shape, the three FP64-accurate products side by side (for kernel
     traces): native FP64-MFMA DGEMM, the slice form at s = 8 and the CRT
-    form at 15 moduli, n^3 with phi = 0.5 inputs; one warm-up and three timed
-    calls each, error on 1024 sampled entries against double-double."""
+    form at 15 moduli, n x n x k (k = --k, default n) with phi = 0.5 inputs;
+    one warm-up and three timed calls each, error on 1024 sampled entries
+    against double-double."""
     from oracle import oracle as O
     from oracle import ozaki as Z
     n = args.n
+    k = args.k or n
     rnd = np.random.default_rng(1)
-    A, B = Z.phi_matrix(rnd, (n, n), 0.5), Z.phi_matrix(rnd, (n, n), 0.5)
+    A, B = Z.phi_matrix(rnd, (n, k), 0.5), Z.phi_matrix(rnd, (k, n), 0.5)
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
     dC = hip.DeviceArray.zeros((n, n))
     ii, jj = rnd.integers(0, n, 1024), rnd.integers(0, n, 1024)
-    ref, ab = O.dd_sample(0, 0, n, n, n, np.asfortranarray(A), n, np.asfortranarray(B), n, ii, jj)
+    ref, ab = O.dd_sample(0, 0, n, n, k, np.asfortranarray(A), n, np.asfortranarray(B), k, ii, jj)
     del A, B
-    flops = 2.0 * n ** 3
+    flops = 2.0 * n * n * k
     hip.set_sync(False)
-    for name, f, prods in (("dgemm-mfma", lambda: hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC), None),
-                           ("ozaki-int8 s=8", lambda: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 8), 36),
+    for name, f, prods in (("dgemm-mfma", lambda: hip.dgemm(0, 0, n, n, k, 1.0, dA, dB, 0.0, dC), None),
+                           ("ozaki-int8 s=8", lambda: hip.ozaki_dgemm(0, 0, n, n, k, 1.0, dA, dB, 0.0, dC, 8), 36),
                            ("ozaki-int8-crt moduli=15",
-                            lambda: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 15), 15)):
+                            lambda: hip.ozaki2_dgemm(0, 0, n, n, k, 1.0, dA, dB, 0.0, dC, 15), 15)):
         ms = timed(f, reps=3)
         got = dC.sample(ii, jj)
         e = np.abs(got - ref)
-        rec = {"suite": "ozaki-forms", "n": n, "phi": 0.5, "method": name, "ms": round(ms, 3),
+        rec = {"suite": "ozaki-forms", "n": n, "k": k, "phi": 0.5, "method": name, "ms": round(ms, 3),
                "effective_tflops": round(flops / ms / 1e9, 2), "max_err_over_absAabsB": float(np.max(e / ab)),
                "max_rel_err": float(np.max(e / np.abs(ref)))}
         if prods:

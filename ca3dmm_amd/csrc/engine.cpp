@@ -1399,7 +1399,7 @@ int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph)
 
 int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli)
 {
-    if (!sim || nmoduli < 0 || (nmoduli > 0 && (nmoduli < 4 || nmoduli > 20))) return HIP_PROXY_ERR_ARG;
+    if (!sim || nmoduli < 0 || (nmoduli > 0 && (nmoduli < 4 || nmoduli > 25))) return HIP_PROXY_ERR_ARG;
     if (nmoduli != sim->ozaki && sim->ozplan) {
         (void) hip_stream_sync(nullptr);
         ca3::grouped::destroy_ozaki(sim->ozplan);

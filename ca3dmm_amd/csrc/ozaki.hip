@@ -705,17 +705,18 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(Oz
 // planes R_t (ld Mp); ozaki2_crt_kernel turns the N residues of an element
 // into balanced mixed-radix digits (Garner) and evaluates them in FP64 by
 // Horner from the top digit (rounded multiply, then rounded add).
-constexpr int kCrtMax = 20;
+constexpr int kCrtMax = 25;
 constexpr int kCrtMin = 4;
 constexpr int kCrtKMax = 1 << 23;   // the integer norm sums stay below 2^63
-// INT32-exact k-chunk: kchunk * 128 * 128 + 128 < 2^31, a multiple of kKp
-constexpr int kCrtChunk = (int) ((((1u << 31) - 1 - 128) / 16384u) / kKp * kKp);
+// k-chunk: kchunk * 128 * 128 + 128 < 2^30 (the INT32 sum plus the previous
+// residue stays in crt_mod_i32's range), a multiple of kKp
+constexpr int kCrtChunk = (int) ((((1u << 30) - 1 - 128) / 16384u) / kKp * kKp);
 
-constexpr int kCrtModuli[kCrtMax] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
-                                     223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
-// T(N) = (bitlen(P) - 2) / 2 for the first N moduli (N = 0 .. 20)
-constexpr int kCrtT[kCrtMax + 1] = {0, 3, 7, 11, 15, 19, 23, 27, 31, 35, 39,
-                                    43, 46, 50, 54, 58, 62, 65, 69, 73, 77};
+constexpr int kCrtModuli[kCrtMax] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211,
+                                     199, 197, 193, 191, 181, 179, 173, 167, 163, 157, 151, 149};
+// T(N) = (bitlen(P) - 2) / 2 for the first N moduli (N = 0 .. 25)
+constexpr int kCrtT[kCrtMax + 1] = {0, 3, 7, 11, 15, 19, 23, 27, 31, 35, 39, 43, 46, 50,
+                                    54, 58, 62, 65, 69, 73, 77, 80, 84, 88, 91, 95};
 
 constexpr int crt_inv(int a, int p)                 // a^-1 mod p (extended Euclid)
 {
@@ -739,7 +740,7 @@ struct CrtTables {
     double pd[kCrtMax], invpd[kCrtMax];
     float gmul[kCrtMax][kCrtMax];                    // M_j mod p_t (balanced), M_j = p_0 ... p_{j-1}, j < t
     float ginv[kCrtMax];                             // M_t^-1 mod p_t
-    int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax];    // 2^21, 2^42, 2^63 mod p (balanced)
+    int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax], c84[kCrtMax];  // 2^21 .. 2^84 mod p (balanced)
 };
 
 constexpr CrtTables crt_tables()
@@ -754,6 +755,7 @@ constexpr CrtTables crt_tables()
         c.c21[t] = crt_pow2(21, p);
         c.c42[t] = crt_pow2(42, p);
         c.c63[t] = crt_pow2(63, p);
+        c.c84[t] = crt_pow2(84, p);
         int mj = 1;                                  // M_j mod p_t
         for (int j = 0; j < kCrtMax; ++j) {
             c.gmul[t][j] = j < t ? (float) (mj > p / 2 ? mj - p : mj) : 0.0f;
@@ -874,11 +876,11 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
     e[l] = T + 19 - E - (b + 1) / 2;
 }
 
-// Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| < 2^78)
-// is cut into signed 21-bit pieces a' = x3 2^63 + x2 2^42 + x1 2^21 + x0
-// (each step exact), and a' mod p = x3 c63 + x2 c42 + x1 c21 + x0 (mod p),
-// an INT32 sum of 24-bit products below 2^30, reduced by crt_mod_i32; for
-// p_0 = 256 the residue is the low byte of x0.  Plane t:
+// Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| < 2^95)
+// is cut into signed pieces a' = x4 2^84 + x3 2^63 + x2 2^42 + x1 2^21 + x0
+// (|x_i| < 2^21, each step exact), and a' mod p = x4 c84 + x3 c63 + x2 c42 +
+// x1 c21 + x0 (mod p), an INT32 sum of 24-bit products below 2^30, reduced by
+// crt_mod_i32; for p_0 = 256 the residue is the low byte of x0.  Plane t:
 // P[t * plane + line * Kp + kk].
 __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *__restrict__ X, int lines, int K,
                                                  int64_t sl, int64_t sk, const int *__restrict__ ex, int nmod,
@@ -905,17 +907,19 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
     const int l = l0 + li;
     const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
     const int e = zero_line ? 0 : ex[l];
-    int x3[16], x2[16], x1[16], x0[16];
+    int x4[16], x3[16], x2[16], x1[16], x0[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         double a = zero_line ? 0.0 : rint(ldexp(tile[li][kq + j], e));
+        const double h4 = trunc(ldexp(a, -84));
+        a -= ldexp(h4, 84);
         const double h3 = trunc(ldexp(a, -63));
         a -= ldexp(h3, 63);
         const double h2 = trunc(ldexp(a, -42));
         a -= ldexp(h2, 42);
         const double h1 = trunc(ldexp(a, -21));
         a -= ldexp(h1, 21);
-        x3[j] = (int) h3; x2[j] = (int) h2; x1[j] = (int) h1; x0[j] = (int) a;
+        x4[j] = (int) h4; x3[j] = (int) h3; x2[j] = (int) h2; x1[j] = (int) h1; x0[j] = (int) a;
     }
     int8_t *base = P + (int64_t) l * Kp + k0 + kq;
     {
@@ -930,7 +934,7 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
         *reinterpret_cast<u32x4 *>(base) = v;
     }
     for (int m = 1; m < nmod; ++m) {
-        const int c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
+        const int c84 = c_crt.c84[m], c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
         const float p = c_crt.p[m], ip = c_crt.invp[m];
         u32x4 v;
 #pragma unroll
@@ -939,7 +943,8 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = 4 * q + b;
-                const int s = __mul24(x3[j], c63) + __mul24(x2[j], c42) + __mul24(x1[j], c21) + x0[j];
+                const int s = __mul24(x4[j], c84) + __mul24(x3[j], c63) + __mul24(x2[j], c42) +
+                              __mul24(x1[j], c21) + x0[j];
                 const int r = crt_mod_i32(s, p, ip);
                 w |= ((uint32_t) r & 255u) << (8 * b);
             }
@@ -956,48 +961,64 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
     oz2_residue_body(blockIdx.x, blockIdx.y, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
 }
 
+// Residue planes are stored in the MFMA's own order: a plane is Mp x Np
+// int8 in 16 x 16 blocks of 256 contiguous bytes, block (rb, cb) at
+// (cb * Mp / 16 + rb) * 256; inside a block, lane l's 4-byte word (at 4 l)
+// holds rows rb * 16 + l % 16, columns cb * 16 + 4 (l / 16) + q for byte q,
+// which is the accumulator layout of v_mfma_i32_16x16x64_i8 with operands
+// fed (B, A).  A wave then stores and loads whole 256-byte runs.
+__device__ __forceinline__ int64_t crt_block_word(int Mp, int rb, int cb)
+{
+    return ((int64_t) cb * (Mp >> 4) + rb) * 64;
+}
+
 // Epilogue of a CRT segment (modulus t, one k-chunk): R_t = (R_t + Z) mod p_t
-// (R_t = Z mod p_t on the first chunk), over the whole padded tile.
+// (R_t = Z mod p_t on the first chunk), over the whole padded tile.  For
+// p_0 = 256 the balanced residue is the low byte.
 template <int WTM, int WTN, int FI, int FJ>
 __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&acc)[FI][FJ], int m0, int n0, int t,
                                                 bool first)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int li = lane & 15, kq = lane >> 4;
-    const double p = c_crt.pd[t], ip = c_crt.invpd[t];
-    int8_t *R = g.R8 + (int64_t) t * g.rplane;
+    const float p = c_crt.p[t], ip = c_crt.invp[t];
+    uint32_t *R = reinterpret_cast<uint32_t *>(g.R8 + (int64_t) t * g.rplane) + lane;
 #pragma unroll
     for (int i = 0; i < FI; ++i) {
-        const int row = m0 + wm * WTM + i * 16 + li;
+        const int rb = (m0 + wm * WTM + i * 16) >> 4;
 #pragma unroll
         for (int j = 0; j < FJ; ++j) {
+            uint32_t *wp = R + crt_block_word(g.Mp, rb, (n0 + wn * WTN + j * 16) >> 4);
+            const uint32_t prev = first ? 0u : *wp;
+            uint32_t w = 0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int col = n0 + wn * WTN + j * 16 + 4 * kq + r;
-                int8_t *rp = R + (int64_t) col * g.Mp + row;
-                double v = (double) acc[i][j][r];
-                if (!first) v += (double) *rp;
-                *rp = (int8_t) (int) crt_bal(v, p, ip);
+                const int v = acc[i][j][r] + (int) (int8_t) (prev >> (8 * r));
+                const int res = (t == 0) ? v : crt_mod_i32(v, p, ip);
+                w |= ((uint32_t) res & 255u) << (8 * r);
             }
+            *wp = w;
         }
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-// Per element (4 consecutive rows i0..i0+3 of column j): the N residues r_t,
-// Garner's balanced mixed-radix digits v_t = (r_t - sum_{j<t} v_j M_j) M_t^-1
-// mod p_t (M_j = p_0 ... p_{j-1}; FP32 and exact: the sum stays below 2^19
-// and is reduced before the product), then x = v_0 + p_0 (v_1 + p_1 (...))
-// in FP64 from the top digit and 2^-eA 2^-eB x (0 for an empty line).
+// One lane's 4 elements of block (rb, cb): row rb * 16 + lane % 16,
+// columns cb * 16 + 4 (lane / 16) + q.  From the N residues r_t, Garner's
+// balanced mixed-radix digits v_t = (r_t - sum_{j<t} v_j M_j) M_t^-1 mod p_t
+// (M_j = p_0 ... p_{j-1}; FP32 and exact: the sum stays below 2^19 and is
+// reduced before the product), then x = v_0 + p_0 (v_1 + p_1 (...)) in FP64
+// from the top digit, and 2^-eA 2^-eB x (0 for an empty line).
 template <int NM>
-__device__ __forceinline__ void oz2_crt_column(const int8_t *__restrict__ R, int64_t rplane, int Mp, int i0, int j,
-                                               const int *__restrict__ eA, int eb, double (&out)[4])
+__device__ __forceinline__ void oz2_crt_block(const int8_t *__restrict__ R, int64_t rplane, int Mp, int rb, int cb,
+                                              int lane, const int *__restrict__ eA, const int *__restrict__ eB,
+                                              double (&out)[4])
 {
+    const uint32_t *W = reinterpret_cast<const uint32_t *>(R) + crt_block_word(Mp, rb, cb) + lane;
     float v[4][NM];
 #pragma unroll
     for (int t = 0; t < NM; ++t) {
-        const uint32_t w = *reinterpret_cast<const uint32_t *>(R + (int64_t) t * rplane + (int64_t) j * Mp + i0);
+        const uint32_t w = W[(t * rplane) >> 2];
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q][t] = (float) (int8_t) (w >> (8 * q));
     }
@@ -1012,47 +1033,52 @@ __device__ __forceinline__ void oz2_crt_column(const int8_t *__restrict__ R, int
             v[q][t] = crt_balf((v[q][t] - crt_balf(sum, p, ip)) * gi, p, ip);
         }
     }
+    const int ea = eA[rb * 16 + (lane & 15)];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         double x = (double) v[q][NM - 1];
 #pragma unroll
         for (int t = NM - 2; t >= 0; --t) x = ca3::add_rn(ca3::mul_rn(x, c_crt.pd[t]), (double) v[q][t]);
-        const int ea = eA[i0 + q];
+        const int eb = eB[cb * 16 + 4 * (lane >> 4) + q];
         out[q] = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(x, -ea), -eb) : 0.0;
     }
 }
 
-// C = alpha * (the reconstructed product) (+ beta C); rows of the padded
-// planes past M are read (they hold zeros) but not written.
+// C = alpha * (the reconstructed product) (+ beta C).  A workgroup takes 4
+// blocks (one per wave), blocks ordered column of blocks by column of blocks.
 template <int NM>
 __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restrict__ R, int64_t rplane, int Mp, int M,
                                                           int N, const int *__restrict__ eA,
                                                           const int *__restrict__ eB, double alpha, double beta,
                                                           double *C, int64_t ldc)
 {
-    const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
-    if (i0 >= M) return;
-    for (int j = blockIdx.y; j < N; j += gridDim.y) {
-        double x[4];
-        oz2_crt_column<NM>(R, rplane, Mp, i0, j, eA, eB[j], x);
+    const int nbm = (M + 15) >> 4, nbn = (N + 15) >> 4;
+    const int64_t b = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= (int64_t) nbm * nbn) return;
+    const int lane = threadIdx.x & 63;
+    const int cb = (int) (b / nbm), rb = (int) (b - (int64_t) cb * nbm);
+    const int row = rb * 16 + (lane & 15);
+    if (row >= M) return;
+    double x[4];
+    oz2_crt_block<NM>(R, rplane, Mp, rb, cb, lane, eA, eB, x);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = i0 + q;
-            if (i >= M) break;
-            double *cp = C + (int64_t) j * ldc + i;
-            double v = ca3::mul_rn(alpha, x[q]);
-            if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
-            *cp = v;
-        }
+    for (int q = 0; q < 4; ++q) {
+        const int col = cb * 16 + 4 * (lane >> 4) + q;
+        if (col >= N) break;
+        double *cp = C + (int64_t) col * ldc + row;
+        double v = ca3::mul_rn(alpha, x[q]);
+        if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
+        *cp = v;
     }
 }
 
 // Grouped reconstruction: one job per fold group of the simulated grid (kp
 // layer products of the same C block, added in layer order, each column
-// written to its owner's C_out slice) or per problem (kp = 1).
+// written to its owner's C_out slice) or per problem (kp = 1).  A job's
+// workgroups take 4 blocks each, as ozaki2_crt_kernel.
 constexpr int kGrpMaxKp = 8;
 struct CrtJob {
-    int kp, M, N, Mp, rblocks;            // rblocks: ceil(M / 1024)
+    int kp, M, N, Mp, nbm, nwg;           // nbm: ceil(M / 16); nwg: workgroups of the job
     const int8_t *R[kGrpMaxKp];           // the layers' residue planes (rplane apart)
     int64_t rplane;
     const int *eA[kGrpMaxKp], *eB[kGrpMaxKp];
@@ -1065,29 +1091,35 @@ template <int NM>
 __global__ __launch_bounds__(256) void ozaki2_crt_grouped_kernel(const CrtJob *__restrict__ jobs,
                                                                   const int *__restrict__ blk0, int njobs)
 {
-    const int b = blockIdx.x;
+    const int wg = blockIdx.x;
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (blk0[mid] <= b) lo = mid; else hi = mid - 1;
+        if (blk0[mid] <= wg) lo = mid; else hi = mid - 1;
     }
     const CrtJob &J = jobs[lo];
-    const int local = b - blk0[lo];
-    const int j = local / J.rblocks, i0 = (local - j * J.rblocks) * 1024 + threadIdx.x * 4;
-    if (i0 >= J.M) return;
+    const int64_t b = (int64_t) (wg - blk0[lo]) * 4 + (threadIdx.x >> 6);
+    const int nbn = (J.N + 15) >> 4;
+    if (b >= (int64_t) J.nbm * nbn) return;
+    const int lane = threadIdx.x & 63;
+    const int cb = (int) (b / J.nbm), rb = (int) (b - (int64_t) cb * J.nbm);
+    const int row = rb * 16 + (lane & 15);
+    if (row >= J.M) return;
     double sum[4];
     for (int l = 0; l < J.kp; ++l) {
         double x[4];
-        oz2_crt_column<NM>(J.R[l], J.rplane, J.Mp, i0, j, J.eA[l], J.eB[l][j], x);
+        oz2_crt_block<NM>(J.R[l], J.rplane, J.Mp, rb, cb, lane, J.eA[l], J.eB[l], x);
 #pragma unroll
         for (int q = 0; q < 4; ++q) sum[q] = (l == 0) ? x[q] : ca3::add_rn(sum[q], x[q]);
     }
-    int sl = 0;
-    while (j >= J.cs[sl + 1]) ++sl;
-    double *out = J.Cout[sl] + (int64_t) (j - J.cs[sl]) * J.ldo;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (i0 + q < J.M) out[i0 + q] = sum[q];
+    for (int q = 0; q < 4; ++q) {
+        const int col = cb * 16 + 4 * (lane >> 4) + q;
+        if (col >= J.N) break;
+        int sl = 0;
+        while (col >= J.cs[sl + 1]) ++sl;
+        J.Cout[sl][(int64_t) (col - J.cs[sl]) * J.ldo + row] = sum[q];
+    }
 }
 
 // Grouped preparation: one job per (problem, operand, k-segment); kind 0
@@ -1733,7 +1765,7 @@ int ozaki2_core(int ta, int tb, int m, int n, int k, double alpha, const double 
     g.R8 = R;
     g.rplane = (int64_t) Mp * Np;
     ozaki_pipe_kernel<false, 128, 64, true, 8, true><<<(Mp / 256) * (Np / 256), 512, 0, st>>>(g);
-    const dim3 grid((m + 1023) / 1024, std::min(n, 65535));
+    const dim3 grid((unsigned) ((((int64_t) (m + 15) / 16) * ((n + 15) / 16) + 3) / 4));
     launch_crt_n(nmod, std::make_integer_sequence<int, kCrtMax - kCrtMin + 1>{}, grid, st, R, g.rplane, Mp, m, n,
                  eA, eB, alpha, beta, C, ldC);
     return ca3::finish();
@@ -1945,9 +1977,10 @@ int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &fold
     std::vector<int> cblk(1, 0);
     std::vector<char> folded(p->nprob, 0);
     auto add_job = [&](CrtJob &J) {
-        J.rblocks = (J.M + 1023) / 1024;
+        J.nbm = (J.M + 15) / 16;
+        J.nwg = (int) (((int64_t) J.nbm * ((J.N + 15) / 16) + 3) / 4);
         crt.push_back(J);
-        cblk.push_back(cblk.back() + J.rblocks * J.N);
+        cblk.push_back(cblk.back() + J.nwg);
     };
     for (const Fold &f : folds) {
         if (f.kp < 1 || f.kp > kGrpMaxKp || f.kp > kMaxKp) { destroy_ozaki(p); return HIP_PROXY_ERR_ARG; }

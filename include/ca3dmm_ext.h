@@ -71,7 +71,7 @@ size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices);
 /* C := alpha * op(A) * op(B) + beta * C through the integer-modular (CRT)
  * form of the Ozaki scheme (Ozaki scheme II): lines of op(A) and op(B) are
  * scaled by powers of two to integers whose 2-norms fit the product of the
- * first nmoduli moduli (4 <= nmoduli <= 20), one exact INT8 product per
+ * first nmoduli moduli (4 <= nmoduli <= 25), one exact INT8 product per
  * modulus (v_mfma_i32_16x16x64_i8), then Garner's mixed-radix digits are
  * evaluated in FP64 in a fixed order, so results are reproducible bit for
  * bit (oracle: ozaki2_dgemm).  k <= 2^23.  No reference counterpart: an

@@ -68,7 +68,7 @@ int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per
 /* Fix the grouped launch's tuning by hand (e.g. the untuned baseline). */
 int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu);
 
-/* nmoduli > 0 (4..20): every rank's local product -- all its Cannon rounds
+/* nmoduli > 0 (4..25): every rank's local product -- all its Cannon rounds
  * -- runs through the CRT form of the Ozaki scheme (hip_ozaki2_dgemm's
  * arithmetic on the concatenated k-segments), and the k-layer fold adds the
  * reconstructed layer products in layer order.  With set_mode(grouped = 1)

@@ -239,7 +239,8 @@ def diagonals8(SA, SB, s):
 # sqrt(k)/2 <= 2^(T-1) to the norm (k <= 4^T), hence ||a'|| <= 2^T.
 # ---------------------------------------------------------------------------
 CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
-              223, 217, 211, 199, 197, 193, 191, 181, 179, 173)
+              223, 217, 211, 199, 197, 193, 191, 181, 179, 173,
+              167, 163, 157, 151, 149)
 CRT_MIN, CRT_MAX = 4, len(CRT_MODULI)
 CRT_KMAX = 1 << 23          # S <= k 2^40 stays below 2^63
 
@@ -296,16 +297,17 @@ def balanced_mod(x, p):
 
 def crt_residues(Aint, N):
     """Residue planes (int8) of an array of integer-valued doubles, exact:
-    |a'| < 2^78 is split into three signed 26-bit pieces first."""
-    x2 = np.trunc(np.ldexp(Aint, -52))
-    rem = Aint - np.ldexp(x2, 52)
-    x1 = np.trunc(np.ldexp(rem, -26))
-    x0 = rem - np.ldexp(x1, 26)
-    parts = [v.astype(np.int64) for v in (x2, x1, x0)]
+    |a'| < 2^104 is split into four signed 26-bit pieces first."""
+    parts = []
+    rem = Aint
+    for sh in (78, 52, 26):
+        x = np.trunc(np.ldexp(rem, -sh))
+        rem = rem - np.ldexp(x, sh)
+        parts.append(x.astype(np.int64))
+    parts.append(rem.astype(np.int64))
     out = []
     for p in CRT_MODULI[:N]:
-        c26, c52 = pow(2, 26, p), pow(2, 52, p)
-        r = (parts[0] % p) * c52 + (parts[1] % p) * c26 + (parts[2] % p)
+        r = sum((x % p) * pow(2, sh, p) for x, sh in zip(parts, (78, 52, 26, 0)))
         out.append(balanced_mod(r, p).astype(np.int8))
     return out
 

@@ -37,7 +37,7 @@ def test_ozaki2_bit_exact_vs_spec(hip, ta, tb, m, n, k, N):
     assert np.array_equal(_run(hip, ta, tb, opA, opB, N), want)
 
 
-@pytest.mark.parametrize("N", [4, 5, 7, 8, 10, 11, 12, 13, 15, 17, 19])
+@pytest.mark.parametrize("N", [4, 5, 7, 8, 10, 11, 12, 13, 15, 17, 19, 21, 22, 23, 24, 25])
 def test_ozaki2_every_modulus_count(hip, N):
     rnd = np.random.default_rng(500 + N)
     opA, opB = Z.phi_matrix(rnd, (70, 300), 4.0), Z.phi_matrix(rnd, (300, 90), 4.0)
@@ -82,7 +82,7 @@ def test_ozaki2_k_chunks(hip):
 
 def test_ozaki2_rejects_bad_arguments(hip):
     dA = hip.DeviceArray.zeros((8, 8))
-    for N in (3, 21):
+    for N in (3, 26):
         with pytest.raises(RuntimeError):
             hip.ozaki2_dgemm(0, 0, 8, 8, 8, 1.0, dA, dA, 0.0, dA, N)
 

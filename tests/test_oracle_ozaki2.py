@@ -31,7 +31,7 @@ def test_moduli_and_budget():
         assert 4 ** T >= Z.CRT_KMAX                 # rounding slack sqrt(k)/2 <= 2^(T-1)
 
 
-@pytest.mark.parametrize("N", [4, 9, 16, 20])
+@pytest.mark.parametrize("N", [4, 9, 16, 20, 25])
 def test_residues_equal_integer_remainders(N):
     rnd = np.random.default_rng(N)
     T = Z.crt_T(N)
@@ -50,7 +50,7 @@ def test_residues_equal_integer_remainders(N):
         assert [int(r) for r in R[t]] == [_bal(v, p) for v in vals], p
 
 
-@pytest.mark.parametrize("N", [4, 7, 14, 20])
+@pytest.mark.parametrize("N", [4, 7, 14, 20, 25])
 def test_garner_digits_rebuild_the_integer(N):
     rnd = np.random.default_rng(100 + N)
     P = Z.crt_product(N)
@@ -67,7 +67,7 @@ def test_garner_digits_rebuild_the_integer(N):
         assert acc == x
 
 
-@pytest.mark.parametrize("N", [4, 12, 16, 20])
+@pytest.mark.parametrize("N", [4, 12, 16, 20, 25])
 @pytest.mark.parametrize("phi", [0.5, 4.0])
 def test_line_scale_keeps_the_norm_bound(N, phi):
     rnd = np.random.default_rng(int(phi * 10) + N)
@@ -126,7 +126,7 @@ def _bruteforce(opA, opB, N, alpha, beta, Cin):
     return C
 
 
-@pytest.mark.parametrize("N", [4, 9, 16, 20])
+@pytest.mark.parametrize("N", [4, 9, 16, 20, 25])
 def test_matches_bruteforce_integers(N):
     rnd = np.random.default_rng(7 * N)
     opA = Z.phi_matrix(rnd, (5, 9), 2.0)
