@@ -259,15 +259,26 @@ def grouped(args):
     emit(dict({"suite": "grouped", "mode": "grouped, tuned, hipGraph replay", "ms": round(ms, 3),
                "tflops": round(flops / ms / 1e9, 2),
                "max_abs_diff_vs_per_launch": float(np.max(np.abs(dC.numpy() - ref_c)))}, **errs()))
-    for nmod, modes in ((15, ((False, False), (True, False), (True, True))), (14, ((True, True),)),
-                        (16, ((True, True),))):
+    for nmod in (15, 14, 16):
         sim.set_ozaki(nmod)
-        for grp, graph in modes:
-            sim.set_mode(grouped=grp, graph=graph)
+
+        def rec(mode):
             ms = timed(run, reps=3)
-            mode = ("grouped" if grp else "per-rank product launches") + (", hipGraph replay" if graph else "")
             emit(dict({"suite": "grouped", "form": f"ozaki-int8-crt moduli={nmod}", "mode": mode,
                        "ms": round(ms, 3), "effective_tflops": round(flops / ms / 1e9, 2)}, **errs()))
+        if nmod == 15:
+            sim.set_tuning_ozaki(256, 0)
+            sim.set_mode(grouped=False, graph=False)
+            rec("per-rank product launches, tile 256x256")
+            sim.set_mode(grouped=True, graph=False)
+            rec("grouped, fixed tile 256x256, single queue")
+        sim.set_mode(grouped=True, graph=False)
+        t, q = sim.autotune_ozaki(2)
+        tuned = f"tile 256x{t}, {'per-XCD' if q else 'single'} queue"
+        if nmod == 15:
+            rec(f"grouped, tuned ({tuned})")
+        sim.set_mode(grouped=True, graph=True)
+        rec(f"grouped, tuned ({tuned}), hipGraph replay")
     sim.set_ozaki(0)
     hip.set_sync(True)
     sim.free()

@@ -1409,6 +1409,32 @@ int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli)
     return HIP_PROXY_OK;
 }
 
+int ca3dmm_sim_autotune_ozaki(ca3dmm_sim_p sim, int reps, int *tile_n, int *xcd_queue)
+{
+    if (!sim || !sim->ozaki) return HIP_PROXY_ERR_ARG;
+    for (auto e : sim->eng)
+        if (!e->workbuf_d && total_bytes(e)) return HIP_PROXY_ERR_ARG;
+    int rc = 0;
+    if (!sim->ozplan && (rc = build_ozaki(sim))) return rc;
+    if ((rc = ca3::grouped::autotune_ozaki(sim->ozplan, std::max(1, reps)))) return rc;
+    const ca3::grouped::OzTuning &t = ca3::grouped::tuning_ozaki(sim->ozplan);
+    if (tile_n) *tile_n = t.tile_n;
+    if (xcd_queue) *xcd_queue = t.xcd_queue;
+    return hip_stream_sync(nullptr);
+}
+
+int ca3dmm_sim_set_tuning_ozaki(ca3dmm_sim_p sim, int tile_n, int xcd_queue)
+{
+    if (!sim || !sim->ozaki || (tile_n != 256 && tile_n != 128)) return HIP_PROXY_ERR_ARG;
+    int rc = 0;
+    if (!sim->ozplan && (rc = build_ozaki(sim))) return rc;
+    ca3::grouped::OzTuning t;
+    t.tile_n = tile_n;
+    t.xcd_queue = xcd_queue ? 1 : 0;
+    ca3::grouped::set_tuning_ozaki(sim->ozplan, t);
+    return HIP_PROXY_OK;
+}
+
 int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu)
 {
     if (!sim) return HIP_PROXY_ERR_ARG;

@@ -58,8 +58,18 @@ void destroy(Plan *p);
 // scheme with nmod moduli: each problem's product over all its k-segments
 // (one exact INT8 product per modulus), reconstructed in FP64 and, for a
 // fold group, added over the layers in layer order (ozaki.hip).
+struct OzTuning {
+    int tile_n = 256;         // workgroup tile 256 x tile_n: 256 (8 waves) or 128 (4 waves)
+    int xcd_queue = 1;        // 1: items in 8 per-XCD ranges; 0: one counter over all items
+};
+
 struct OzPlan;
 int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &folds, int nmod, OzPlan **out);
+// Time the candidate tunings of the grouped launch (tile 256 / 128 x queue
+// order) and keep the fastest for this grid's shape signature.
+int autotune_ozaki(OzPlan *p, int reps);
+const OzTuning &tuning_ozaki(const OzPlan *p);
+void set_tuning_ozaki(OzPlan *p, const OzTuning &t);
 // grouped: the products of every problem in one persistent launch fed by a
 // device queue (else one launch per problem); use_graph replays the whole
 // sequence from a captured hipGraph.

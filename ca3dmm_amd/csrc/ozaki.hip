@@ -32,6 +32,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <type_traits>
 #include <utility>
 
@@ -250,6 +251,7 @@ struct OzArgs {
     const int *tile0;
     int nprob, total;
     int *queue;
+    int flatq;              // 1: one counter over all items; 0: 8 per-XCD ranges
 };
 
 // 16-byte chunk swizzle within a 64-byte LDS row.  An MFMA operand read
@@ -1426,12 +1428,18 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
         for (;;) {
             if (tid == 0) {
                 int got = -1;
-                for (int v = 0; v < 8 && got < 0; ++v) {
-                    const int x = (xcd + v) & 7;
-                    const int lo = (int) ((int64_t) g0.total * x / 8), hi = (int) ((int64_t) g0.total * (x + 1) / 8);
-                    if (lo >= hi) continue;
-                    const int my = lo + atomicAdd(&g0.queue[x], 1);
-                    if (my < hi) got = my;
+                if (g0.flatq) {
+                    const int my = atomicAdd(&g0.queue[0], 1);
+                    got = my < g0.total ? my : -1;
+                } else {
+                    for (int v = 0; v < 8 && got < 0; ++v) {
+                        const int x = (xcd + v) & 7;
+                        const int lo = (int) ((int64_t) g0.total * x / 8);
+                        const int hi = (int) ((int64_t) g0.total * (x + 1) / 8);
+                        if (lo >= hi) continue;
+                        const int my = lo + atomicAdd(&g0.queue[x], 1);
+                        if (my < hi) got = my;
+                    }
                 }
                 s_item = got;
             }
@@ -1780,7 +1788,8 @@ namespace ca3 {
 namespace grouped {
 
 struct OzPlan {
-    int nmod = 0, nprob = 0, total = 0, nlines = 0;
+    int nmod = 0, nprob = 0, nlines = 0;
+    int total[2] = {0, 0};                   // queue items for 256 x 256 / 256 x 128 tiles
     int njobs = 0, rblocks = 0, sblocks = 0, ncrt = 0, crtblocks = 0;
     void *planes = nullptr;                  // operand residue planes and product residue planes
     unsigned long long *words = nullptr;     // [2 * nlines]: line maxima, then norm sums
@@ -1789,14 +1798,17 @@ struct OzPlan {
     PrepJob *d_jobs = nullptr;
     int *d_blk_r = nullptr, *d_blk_s = nullptr;
     OzArgs *d_args = nullptr;
-    int *d_tile0 = nullptr;
+    int *d_tile0[2] = {nullptr, nullptr};
     CrtJob *d_crt = nullptr;
     int *d_crt_blk = nullptr;
     std::vector<OzArgs> h_args;              // the per-problem launches
-    std::vector<int> h_tiles;
+    std::vector<int> h_tiles[2];
+    OzTuning tune;
+    std::string signature;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_grouped = false;
+    OzTuning graph_tune;
 };
 
 namespace {
@@ -1816,8 +1828,12 @@ void launch_crt_grouped_n(int nmod, std::integer_sequence<int, Ns...>, int block
                            : void()), ...);
 }
 
-int oz_sequence(OzPlan *p, bool grouped, hipStream_t st)
+std::map<std::string, OzTuning> g_oz_cache;
+std::mutex g_oz_cache_mu;
+
+int oz_sequence(OzPlan *p, const OzTuning &tn, bool grouped, hipStream_t st)
 {
+    const int v = (tn.tile_n == 128) ? 1 : 0;
     fill_int_kernel<<<std::max(1, std::min(4096, (4 * p->nlines + 255) / 256)), 256, 0, st>>>((int *) p->words,
                                                                                                4 * p->nlines, 0);
     fill_int_kernel<<<1, 64, 0, st>>>(p->queue, 8, 0);
@@ -1832,18 +1848,27 @@ int oz_sequence(OzPlan *p, bool grouped, hipStream_t st)
     if (grouped) {
         OzArgs g0;
         memset(&g0, 0, sizeof(g0));
-        g0.tile0 = p->d_tile0;
+        g0.tile0 = p->d_tile0[v];
         g0.nprob = p->nprob;
-        g0.total = p->total;
+        g0.total = p->total[v];
         g0.queue = p->queue;
-        // one 512-thread workgroup per CU (128 KB of LDS each); every one
-        // leaves when the queue passes the last item
-        const int grid = std::max(1, std::min(p->total, ca3::cu_count()));
-        if (p->total) ozaki_pipe_kernel<false, 128, 64, true, 8, true, true><<<grid, 512, 0, st>>>(g0, p->d_args);
+        g0.flatq = tn.xcd_queue ? 0 : 1;
+        // one workgroup per CU (128 KB of LDS for 256 x 256, 64 KB for
+        // 256 x 128 at one per CU too: the queue keeps every CU busy); every
+        // one leaves when the queue passes the last item
+        const int grid = std::max(1, std::min(g0.total, ca3::cu_count()));
+        if (g0.total) {
+            if (v == 0) ozaki_pipe_kernel<false, 128, 64, true, 8, true, true><<<grid, 512, 0, st>>>(g0, p->d_args);
+            else ozaki_pipe_kernel<false, 128, 64, true, 4, true, true><<<grid, 256, 0, st>>>(g0, p->d_args);
+        }
     } else {
-        for (int r = 0; r < p->nprob; ++r)
-            if (p->h_tiles[r])
-                ozaki_pipe_kernel<false, 128, 64, true, 8, true, false><<<p->h_tiles[r], 512, 0, st>>>(p->h_args[r]);
+        for (int r = 0; r < p->nprob; ++r) {
+            if (!p->h_tiles[v][r]) continue;
+            if (v == 0)
+                ozaki_pipe_kernel<false, 128, 64, true, 8, true, false><<<p->h_tiles[0][r], 512, 0, st>>>(p->h_args[r]);
+            else
+                ozaki_pipe_kernel<false, 128, 64, true, 4, true, false><<<p->h_tiles[1][r], 256, 0, st>>>(p->h_args[r]);
+        }
     }
     if (p->crtblocks)
         launch_crt_grouped_n(p->nmod, std::make_integer_sequence<int, kCrtMax - kCrtMin + 1>{}, p->crtblocks, st,
@@ -1910,7 +1935,7 @@ int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &fold
 
     std::vector<PrepJob> jobs;
     std::vector<int> blk_r(1, 0), blk_s(1, 0);
-    std::vector<int> tile0(1, 0);
+    std::vector<int> tile0[2] = {std::vector<int>(1, 0), std::vector<int>(1, 0)};
     std::vector<OzArgs> args;
     std::map<const double *, int> by_c;
     for (int i = 0; i < p->nprob; ++i) {
@@ -1959,18 +1984,24 @@ int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &fold
         g.R8 = base + L.r;
         g.rplane = (int64_t) L.Mp * L.Np;
         args.push_back(g);
-        const int tiles = (q.M > 0 && q.N > 0) ? (L.Mp / 256) * (L.Np / 256) : 0;
-        p->h_tiles.push_back(tiles);
-        tile0.push_back(tile0.back() + tiles);
+        for (int v = 0; v < 2; ++v) {
+            const int tiles = (q.M > 0 && q.N > 0) ? (L.Mp / 256) * (L.Np / (v ? 128 : 256)) : 0;
+            p->h_tiles[v].push_back(tiles);
+            tile0[v].push_back(tile0[v].back() + tiles);
+        }
+        p->signature += std::to_string(q.M) + "x" + std::to_string(q.N) + "x" + std::to_string(L.Kp) + ";";
     }
+    p->signature += "m" + std::to_string(nmod);
     p->h_args = args;
-    p->total = tile0.back();
+    p->total[0] = tile0[0].back();
+    p->total[1] = tile0[1].back();
     p->njobs = (int) jobs.size();
     p->rblocks = blk_r.back();
     p->sblocks = blk_s.back();
     blk_r.pop_back();
     blk_s.pop_back();
-    tile0.pop_back();
+    tile0[0].pop_back();
+    tile0[1].pop_back();
 
     // reconstruction jobs: fold groups, then problems outside any fold group
     std::vector<CrtJob> crt;
@@ -2031,11 +2062,17 @@ int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &fold
     if (e == hipSuccess) e = upload(&p->d_blk_r, blk_r, st);
     if (e == hipSuccess) e = upload(&p->d_blk_s, blk_s, st);
     if (e == hipSuccess) e = upload(&p->d_args, args, st);
-    if (e == hipSuccess) e = upload(&p->d_tile0, tile0, st);
+    if (e == hipSuccess) e = upload(&p->d_tile0[0], tile0[0], st);
+    if (e == hipSuccess) e = upload(&p->d_tile0[1], tile0[1], st);
     if (e == hipSuccess) e = upload(&p->d_crt, crt, st);
     if (e == hipSuccess) e = upload(&p->d_crt_blk, cblk, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { destroy_ozaki(p); return ca3::fail(e, "grouped ozaki plan"); }
+    {
+        std::lock_guard<std::mutex> lk(g_oz_cache_mu);
+        auto it = g_oz_cache.find(p->signature);
+        if (it != g_oz_cache.end()) p->tune = it->second;
+    }
     *out = p;
     return 0;
 }
@@ -2043,11 +2080,13 @@ int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &fold
 int run_ozaki(OzPlan *p, bool grouped, bool use_graph)
 {
     hipStream_t st = ca3::stream();
-    if (!use_graph) return oz_sequence(p, grouped, st);
-    if (p->exec && p->graph_grouped != grouped) drop_oz_graph(p);
+    if (!use_graph) return oz_sequence(p, p->tune, grouped, st);
+    if (p->exec && (p->graph_grouped != grouped || p->graph_tune.tile_n != p->tune.tile_n ||
+                    p->graph_tune.xcd_queue != p->tune.xcd_queue))
+        drop_oz_graph(p);
     if (!p->exec) {
         CA3_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int rc = oz_sequence(p, grouped, st);
+        int rc = oz_sequence(p, p->tune, grouped, st);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
         if (rc) { if (g) (void) hipGraphDestroy(g); return rc; }
@@ -2055,10 +2094,51 @@ int run_ozaki(OzPlan *p, bool grouped, bool use_graph)
         p->graph = g;
         CA3_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
         p->graph_grouped = grouped;
+        p->graph_tune = p->tune;
     }
     CA3_CHECK(hipGraphLaunch(p->exec, st));
     return 0;
 }
+
+int autotune_ozaki(OzPlan *p, int reps)
+{
+    static const int tiles[] = {256, 128};
+    static const int queues[] = {1, 0};
+    hipStream_t st = ca3::stream();
+    hipEvent_t e0, e1;
+    CA3_CHECK(hipEventCreate(&e0));
+    CA3_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    OzTuning bt = p->tune;
+    int rc = 0;
+    for (int tn : tiles) {
+        for (int q : queues) {
+            OzTuning t;
+            t.tile_n = tn;
+            t.xcd_queue = q;
+            if ((rc = oz_sequence(p, t, true, st))) break;               // warm-up
+            (void) hipEventRecord(e0, st);
+            for (int r = 0; r < reps && !rc; ++r) rc = oz_sequence(p, t, true, st);
+            (void) hipEventRecord(e1, st);
+            (void) hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void) hipEventElapsedTime(&ms, e0, e1);
+            if (!rc && ms < best) { best = ms; bt = t; }
+        }
+        if (rc) break;
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    if (rc) return rc;
+    p->tune = bt;
+    std::lock_guard<std::mutex> lk(g_oz_cache_mu);
+    g_oz_cache[p->signature] = bt;
+    return 0;
+}
+
+const OzTuning &tuning_ozaki(const OzPlan *p) { return p->tune; }
+
+void set_tuning_ozaki(OzPlan *p, const OzTuning &t) { p->tune = t; }
 
 void destroy_ozaki(OzPlan *p)
 {
@@ -2072,7 +2152,8 @@ void destroy_ozaki(OzPlan *p)
     (void) hipFree(p->d_blk_r);
     (void) hipFree(p->d_blk_s);
     (void) hipFree(p->d_args);
-    (void) hipFree(p->d_tile0);
+    (void) hipFree(p->d_tile0[0]);
+    (void) hipFree(p->d_tile0[1]);
     (void) hipFree(p->d_crt);
     (void) hipFree(p->d_crt_blk);
     delete p;

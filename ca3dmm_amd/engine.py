@@ -117,6 +117,8 @@ _SIGS = {
     "ca3dmm_sim_autotune": (I, [P, I, IP, IP]),
     "ca3dmm_sim_set_tuning": (I, [P, I, I]),
     "ca3dmm_sim_set_ozaki": (I, [P, I]),
+    "ca3dmm_sim_autotune_ozaki": (I, [P, I, IP, IP]),
+    "ca3dmm_sim_set_tuning_ozaki": (I, [P, I, I]),
 }
 for _n, (_r, _a) in _SIGS.items():
     _f = getattr(lib, _n)
@@ -173,6 +175,16 @@ class SimGrid:
     def set_ozaki(self, nmoduli):
         """nmoduli > 0: the local products through the CRT form (0: FP64 MFMA)."""
         return check(lib.ca3dmm_sim_set_ozaki(self.h, nmoduli), "ca3dmm_sim_set_ozaki")
+
+    def autotune_ozaki(self, reps=3):
+        """Tune the Ozaki form's grouped launch; returns (tile_n, xcd_queue)."""
+        t, q = C.c_int(), C.c_int()
+        check(lib.ca3dmm_sim_autotune_ozaki(self.h, reps, C.byref(t), C.byref(q)), "ca3dmm_sim_autotune_ozaki")
+        return t.value, q.value
+
+    def set_tuning_ozaki(self, tile_n, xcd_queue):
+        return check(lib.ca3dmm_sim_set_tuning_ozaki(self.h, tile_n, 1 if xcd_queue else 0),
+                     "ca3dmm_sim_set_tuning_ozaki")
 
     def set_tuning(self, group_rows, wg_per_cu):
         return check(lib.ca3dmm_sim_set_tuning(self.h, group_rows, wg_per_cu), "ca3dmm_sim_set_tuning")

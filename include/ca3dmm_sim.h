@@ -79,6 +79,16 @@ int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu);
  * counterpart (the reference's local GEMM is src/cannon.c:405-427). */
 int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli);
 
+/* The Ozaki form's grouped launch: time its tunings (workgroup tile 256 x
+ * 256 with 8 waves or 256 x 128 with 4 waves; queue items in 8 per-XCD
+ * ranges or from one counter), keep the fastest for this grid's shape
+ * signature and report it.  Needs set_ozaki(nmoduli > 0) first.  No
+ * reference counterpart. */
+int ca3dmm_sim_autotune_ozaki(ca3dmm_sim_p sim, int reps, int *tile_n, int *xcd_queue);
+
+/* Fix the Ozaki form's tuning by hand (tile_n 256 or 128). */
+int ca3dmm_sim_set_tuning_ozaki(ca3dmm_sim_p sim, int tile_n, int xcd_queue);
+
 void ca3dmm_sim_free(ca3dmm_sim_p *sim);
 
 #ifdef __cplusplus

@@ -115,3 +115,38 @@ def test_grouped_ozaki_accuracy_vs_fp64_form(hip):
     e_fp = np.max(np.abs(fp[ii, jj] - ref) / ab)
     e_oz = np.max(np.abs(oz[ii, jj] - ref) / ab)
     assert e_oz <= e_fp, (e_oz, e_fp)
+
+
+def test_grouped_ozaki_every_tuning_bit_exact(hip):
+    """The Ozaki form's grouped launch under each tuning (256 x 256 / 256 x
+    128 tiles, per-XCD / single queue) and after autotuning, per-rank and
+    grouped, against the restated grid."""
+    from ca3dmm_amd import engine as E
+    from oracle import oracle as O
+    from tests.simutil import ozaki_grid_reference
+    rnd = np.random.default_rng(45)
+    m, n, k, p, grid, N = 650, 590, 700, 12, [2, 2, 3], 15
+    A, B = _phi(rnd, (m, k), 1.0), _phi(rnd, (k, n), 1.0)
+    want = ozaki_grid_reference(m, n, k, p, grid, A, B, N)
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=grid)
+    dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+    dC = hip.DeviceArray.filled((m, n), 0xFF)
+    sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=grid, allocate=True)
+    Ap = [dA.at(r[2] * m) for r in w["src_A"]]
+    Bp = [dB.at(r[2] * k) for r in w["src_B"]]
+    Cp = [dC.at(r[2] * m) for r in w["dst_C"]]
+    sim.set_ozaki(N)
+    for tile_n in (256, 128):
+        for xq in (1, 0):
+            sim.set_tuning_ozaki(tile_n, xq)
+            for grouped in (False, True):
+                sim.set_mode(grouped=grouped, graph=False)
+                sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
+                assert np.array_equal(dC.numpy(), want), (tile_n, xq, grouped)
+    sim.set_mode(grouped=True, graph=True)
+    t, q = sim.autotune_ozaki(1)
+    assert t in (256, 128) and q in (0, 1)
+    for _ in range(2):
+        sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
+        assert np.array_equal(dC.numpy(), want)
+    sim.free()
