@@ -884,6 +884,8 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
 // x1 c21 + x0 (mod p), an INT32 sum of 24-bit products below 2^30, reduced by
 // crt_mod_i32; for p_0 = 256 the residue is the low byte of x0.  Plane t:
 // P[t * plane + line * Kp + kk].
+// NP: pieces needed for |a'| < 2^(T-1): 3 below 2^63, 4 below 2^84, else 5.
+template <int NP>
 __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *__restrict__ X, int lines, int K,
                                                  int64_t sl, int64_t sk, const int *__restrict__ ex, int nmod,
                                                  int8_t *__restrict__ P, int64_t plane, int Kp)
@@ -913,9 +915,9 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         double a = zero_line ? 0.0 : rint(ldexp(tile[li][kq + j], e));
-        const double h4 = trunc(ldexp(a, -84));
+        const double h4 = NP > 4 ? trunc(ldexp(a, -84)) : 0.0;
         a -= ldexp(h4, 84);
-        const double h3 = trunc(ldexp(a, -63));
+        const double h3 = NP > 3 ? trunc(ldexp(a, -63)) : 0.0;
         a -= ldexp(h3, 63);
         const double h2 = trunc(ldexp(a, -42));
         a -= ldexp(h2, 42);
@@ -945,8 +947,8 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = 4 * q + b;
-                const int s = __mul24(x4[j], c84) + __mul24(x3[j], c63) + __mul24(x2[j], c42) +
-                              __mul24(x1[j], c21) + x0[j];
+                const int s = (NP > 4 ? __mul24(x4[j], c84) : 0) + (NP > 3 ? __mul24(x3[j], c63) : 0) +
+                              __mul24(x2[j], c42) + __mul24(x1[j], c21) + x0[j];
                 const int r = crt_mod_i32(s, p, ip);
                 w |= ((uint32_t) r & 255u) << (8 * b);
             }
@@ -956,12 +958,16 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
     }
 }
 
+template <int NP>
 __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__restrict__ X, int lines, int K,
                                                               int64_t sl, int64_t sk, const int *__restrict__ ex,
                                                               int nmod, int8_t *__restrict__ P, int64_t plane, int Kp)
 {
-    oz2_residue_body(blockIdx.x, blockIdx.y, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
+    oz2_residue_body<NP>(blockIdx.x, blockIdx.y, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
 }
+
+// Pieces of a' = rint(x 2^e), |a'| <= 2^(T-1): 3 for T <= 63, 4 for T <= 84.
+constexpr int crt_pieces(int nmod) { return kCrtT[nmod] <= 63 ? 3 : (kCrtT[nmod] <= 84 ? 4 : 5); }
 
 // Residue planes are stored in the MFMA's own order: a plane is Mp x Np
 // int8 in 16 x 16 blocks of 256 contiguous bytes, block (rb, cb) at
@@ -1140,7 +1146,7 @@ struct PrepJob {
     int Kp;
 };
 
-template <int KIND>
+template <int KIND, int NP = 5>
 __global__ __launch_bounds__(256) void ozaki2_prep_grouped_kernel(const PrepJob *__restrict__ jobs,
                                                                    const int *__restrict__ blk0, int njobs,
                                                                    int nmod)
@@ -1158,8 +1164,8 @@ __global__ __launch_bounds__(256) void ozaki2_prep_grouped_kernel(const PrepJob 
     } else if (KIND == 1) {
         oz2_norm_body(local % J.rbx, local / J.rbx, J.rby, J.X, J.lines, J.K, J.sl, J.sk, J.mx, J.S);
     } else {
-        oz2_residue_body(local % J.sbx, local / J.sbx, J.X, J.lines, J.K, J.sl, J.sk, J.ex, nmod, J.P, J.plane,
-                         J.Kp);
+        oz2_residue_body<NP>(local % J.sbx, local / J.sbx, J.X, J.lines, J.K, J.sl, J.sk, J.ex, nmod, J.P,
+                             J.plane, J.Kp);
     }
 }
 
@@ -1755,10 +1761,16 @@ int ozaki2_core(int ta, int tb, int m, int n, int k, double alpha, const double 
         ozaki2_norm_kernel<<<dim3((n + 63) / 64, ky), 256, 0, st>>>(B, n, k, b_sl, b_sk, mx + Mp, S + Mp);
     }
     ozaki2_scale_kernel<<<(nl + 255) / 256, 256, 0, st>>>(mx, S, nl, kCrtT[nmod], eA);
-    ozaki2_residue_kernel<<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(A, m, k, a_sl, a_sk, eA, nmod, PA,
-                                                                  (int64_t) Mp * Kp, Kp);
-    ozaki2_residue_kernel<<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(B, n, k, b_sl, b_sk, eB, nmod, PB,
-                                                                  (int64_t) Np * Kp, Kp);
+    auto residues = [&](auto NP) {
+        ozaki2_residue_kernel<decltype(NP)::value><<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(
+            A, m, k, a_sl, a_sk, eA, nmod, PA, (int64_t) Mp * Kp, Kp);
+        ozaki2_residue_kernel<decltype(NP)::value><<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(
+            B, n, k, b_sl, b_sk, eB, nmod, PB, (int64_t) Np * Kp, Kp);
+    };
+    const int np = crt_pieces(nmod);
+    if (np == 3) residues(std::integral_constant<int, 3>{});
+    else if (np == 4) residues(std::integral_constant<int, 4>{});
+    else residues(std::integral_constant<int, 5>{});
     OzArgs g;
     memset(&g, 0, sizeof(g));
     g.M = m; g.N = n; g.Mp = Mp; g.Np = Np; g.Kp = Kp;
@@ -1843,8 +1855,15 @@ int oz_sequence(OzPlan *p, const OzTuning &tn, bool grouped, hipStream_t st)
     }
     ozaki2_scale_kernel<<<(p->nlines + 255) / 256, 256, 0, st>>>(p->words, p->words + p->nlines, p->nlines,
                                                                   kCrtT[p->nmod], p->e);
-    if (p->sblocks)
-        ozaki2_prep_grouped_kernel<2><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+    if (p->sblocks) {
+        const int np = crt_pieces(p->nmod);
+        if (np == 3)
+            ozaki2_prep_grouped_kernel<2, 3><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+        else if (np == 4)
+            ozaki2_prep_grouped_kernel<2, 4><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+        else
+            ozaki2_prep_grouped_kernel<2, 5><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+    }
     if (grouped) {
         OzArgs g0;
         memset(&g0, 0, sizeof(g0));
