@@ -254,6 +254,20 @@ struct OzArgs {
     int flatq;              // 1: one counter over all items; 0: 8 per-XCD ranges
 };
 
+// Uniform values kept in scalar registers (readfirstlane of each 32-bit half).
+__device__ __forceinline__ int64_t oz_rfl64(int64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) ((uint64_t) v >> 32));
+    return (int64_t) (((uint64_t) hi << 32) | lo);
+}
+
+template <class T>
+__device__ __forceinline__ T *oz_rflp(T *p)
+{
+    return (T *) (uintptr_t) oz_rfl64((int64_t) (uintptr_t) p);
+}
+
 // 16-byte chunk swizzle within a 64-byte LDS row.  An MFMA operand read
 // (ds_read_b128) has lanes 16 q .. 16 q + 15 on rows r..r+15 (r a multiple of
 // 16) at logical chunk q.  Measured on gfx950 with SQ_LDS_BANK_CONFLICT over
@@ -982,32 +996,49 @@ __device__ __forceinline__ int64_t crt_block_word(int Mp, int rb, int cb)
 
 // Epilogue of a CRT segment (modulus t, one k-chunk): R_t = (R_t + Z) mod p_t
 // (R_t = Z mod p_t on the first chunk), over the whole padded tile.  For
-// p_0 = 256 the balanced residue is the low byte.
+// p_0 = 256 the balanced residue is the low byte.  The plane and block
+// addresses are wave-uniform (scalar registers) and each lane adds only its
+// 4-byte slot, so no per-block 64-bit address is kept live across the
+// segment loop and the 32 stores of a lane stream without waits.
 template <int WTM, int WTN, int FI, int FJ>
 __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&acc)[FI][FJ], int m0, int n0, int t,
                                                 bool first)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave & 1, wn = wave >> 1;
     const float p = c_crt.p[t], ip = c_crt.invp[t];
-    uint32_t *R = reinterpret_cast<uint32_t *>(g.R8 + (int64_t) t * g.rplane) + lane;
+    typedef __attribute__((address_space(1))) uint32_t gu32;
+    gu32 *Rt = (gu32 *) oz_rflp(reinterpret_cast<uint32_t *>(g.R8 + (int64_t) t * g.rplane));
+    const int mp16 = __builtin_amdgcn_readfirstlane(g.Mp >> 4);
+    const int rb0 = __builtin_amdgcn_readfirstlane((m0 + wm * WTM) >> 4);
+    const int cb0 = __builtin_amdgcn_readfirstlane((n0 + wn * WTN) >> 4);
+    // the first chunk loads nothing, so its stores never wait (vmcnt also
+    // counts stores); later chunks add the stored residue back in
+    auto run = [&](auto ODD, auto FIRST) __attribute__((always_inline)) {
 #pragma unroll
-    for (int i = 0; i < FI; ++i) {
-        const int rb = (m0 + wm * WTM + i * 16) >> 4;
+        for (int i = 0; i < FI; ++i) {
 #pragma unroll
-        for (int j = 0; j < FJ; ++j) {
-            uint32_t *wp = R + crt_block_word(g.Mp, rb, (n0 + wn * WTN + j * 16) >> 4);
-            const uint32_t prev = first ? 0u : *wp;
-            uint32_t w = 0;
+            for (int j = 0; j < FJ; ++j) {
+                gu32 *wp = Rt + (int64_t) ((cb0 + j) * mp16 + rb0 + i) * 64;
+                const uint32_t prev = decltype(FIRST)::value ? 0u : wp[lane];
+                uint32_t w = 0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int v = acc[i][j][r] + (int) (int8_t) (prev >> (8 * r));
-                const int res = (t == 0) ? v : crt_mod_i32(v, p, ip);
-                w |= ((uint32_t) res & 255u) << (8 * r);
+                for (int r = 0; r < 4; ++r) {
+                    const int v = acc[i][j][r] + (int) (int8_t) (prev >> (8 * r));
+                    const int res = decltype(ODD)::value ? crt_mod_i32(v, p, ip) : v;
+                    w |= ((uint32_t) res & 255u) << (8 * r);
+                }
+                wp[lane] = w;
             }
-            *wp = w;
         }
-        __builtin_amdgcn_sched_barrier(0);
+    };
+    if (first) {
+        if (t == 0) run(std::false_type{}, std::true_type{});
+        else run(std::true_type{}, std::true_type{});
+    } else {
+        if (t == 0) run(std::false_type{}, std::false_type{});
+        else run(std::true_type{}, std::false_type{});
     }
 }
 
@@ -1172,19 +1203,6 @@ __global__ __launch_bounds__(256) void ozaki2_prep_grouped_kernel(const PrepJob 
 // The fields a CRT segment loop reads, copied through readfirstlane so they
 // live in scalar registers (as kernel arguments do) and the loop's address
 // arithmetic stays scalar.
-__device__ __forceinline__ int64_t oz_rfl64(int64_t v)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) ((uint64_t) v >> 32));
-    return (int64_t) (((uint64_t) hi << 32) | lo);
-}
-
-template <class T>
-__device__ __forceinline__ T *oz_rflp(T *p)
-{
-    return (T *) (uintptr_t) oz_rfl64((int64_t) (uintptr_t) p);
-}
-
 __device__ __forceinline__ OzArgs oz_uniform_crt_args(const OzArgs &a)
 {
     OzArgs g;
@@ -1259,6 +1277,11 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
     // 8 waves: no room for it (128 VGPRs per wave), the other wave of the
     // SIMD covers the fragment reads
     constexpr bool PREFETCH = (NW == 4);
+    // CRT form with 8 waves: the load stream runs on across segments (the
+    // next modulus' first stages are loaded during the last stages of the
+    // current one and land while its epilogue runs).  Not in the grouped
+    // kernel, where the extra state makes the compiler shuffle accumulators.
+    constexpr bool XSEG = CRT && !PREFETCH && !GROUPED;
     constexpr int L = WA + WB;
     static_assert(kBMo % kBNo == 0 && kTile % kBMo == 0 && kTile % kBNo == 0, "tile shapes");
     static_assert((kKp / kKb) % NBUF == 0 && L <= 63, "ring turns per chunk / vmcnt range");
@@ -1280,6 +1303,7 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
     const int64_t blk16 = (int64_t) 16 * g.Kp;
     const int nseg = FUSED ? g.s * g.nchunks : 1;
     int it = 0, ib = 0, inext = 0;
+    int idd = 0, icc = 0;                     // XSEG: modulus and chunk of the next stage to load
     for (int sg = 0; sg < nseg; ++sg) {
     bool first = g.first != 0, last = g.last != 0;
     if (FUSED) {
@@ -1291,9 +1315,11 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
         first = CRT ? (c == 0) : (sg == 0);
     }
     const int total = (CRT ? 1 : d + 1) * nkb;  // 64-byte stages; a multiple of NBUF
-    it = 0;
-    ib = 0;
-    inext = 0;
+    if (!XSEG) {
+        it = 0;
+        ib = 0;
+        inext = 0;
+    }
 
     auto slotA = [&](auto J) -> int8_t * {
         if constexpr (J.value == 0) return A0;
@@ -1310,15 +1336,34 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
     // issue pointer: (pair it, block ib) of the next stage to load, advanced
     // by one stage per issue and held at the last stage past the end
     auto issue = [&](int /*z*/, auto J) __attribute__((always_inline)) {
-        const int t = it, b = ib;
-        const bool more = (inext + 1 < total);
-        inext = more ? inext + 1 : inext;
-        const int nb = ib + 1;
-        ib = more ? (nb == nkb ? 0 : nb) : ib;
-        it = more ? (nb == nkb ? it + 1 : it) : it;
-        const int64_t koff = (int64_t) (kb0 + b) * kKb;
-        const int8_t *pa = gA + (int64_t) (CRT ? d : t) * g.planeA + koff;
-        const int8_t *pb = gB + (int64_t) (d - t) * g.planeB + koff;
+        const int8_t *pa, *pb;
+        if constexpr (XSEG) {
+            // (modulus idd, chunk icc, block ib): on to the next segment at
+            // the end of one, held at the last stage past the end of the last
+            const int kb0i = icc * g.kcb;
+            const int nkbi = min(g.kcb, g.Kp / kKb - kb0i);
+            const int b = ib, nb = ib + 1;
+            const bool wrap = (nb == nkbi);
+            const bool last_chunk = (icc + 1 == g.nchunks);
+            const bool more_seg = !last_chunk || (idd + 1 < g.s);
+            const int64_t koff = (int64_t) (kb0i + b) * kKb;
+            pa = gA + (int64_t) idd * g.planeA + koff;
+            pb = gB + (int64_t) idd * g.planeB + koff;
+            ib = wrap ? (more_seg ? 0 : ib) : nb;
+            const bool step_seg = wrap && more_seg;
+            idd = (step_seg && last_chunk) ? idd + 1 : idd;
+            icc = step_seg ? (last_chunk ? 0 : icc + 1) : icc;
+        } else {
+            const int t = it, b = ib;
+            const bool more = (inext + 1 < total);
+            inext = more ? inext + 1 : inext;
+            const int nb = ib + 1;
+            ib = more ? (nb == nkb ? 0 : nb) : ib;
+            it = more ? (nb == nkb ? it + 1 : it) : it;
+            const int64_t koff = (int64_t) (kb0 + b) * kKb;
+            pa = gA + (int64_t) (CRT ? d : t) * g.planeA + koff;
+            pb = gB + (int64_t) (d - t) * g.planeB + koff;
+        }
         int8_t *sa = slotA(J), *sb = slotB(J);
 #pragma unroll
         for (int i = 0; i < WA; ++i)
@@ -1349,10 +1394,13 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
     using I3 = std::integral_constant<int, 3>;
-    // prologue: stages 0, 1, 2 in flight; fragments of stage 0
-    issue(0, I0{});
-    issue(min(1, total - 1), I1{});
-    issue(min(2, total - 1), I2{});
+    // prologue: stages 0, 1, 2 in flight; fragments of stage 0 (XSEG: only
+    // before the first segment, later ones were loaded by the previous one)
+    if (!XSEG || sg == 0) {
+        issue(0, I0{});
+        issue(min(1, total - 1), I1{});
+        issue(min(2, total - 1), I2{});
+    }
     if constexpr (PREFETCH) {
         __builtin_amdgcn_s_waitcnt(kWaitL2);  // own loads of stage 0 done
         sync(0);
@@ -1408,10 +1456,13 @@ __global__ __launch_bounds__(64 * NW, 1) void ozaki_pipe_kernel(OzArgs g0, const
         step(z + 3, I3{}, af1, bf1, af0, bf0);
     }
     // the loads and fragment reads past the last stage were repeats of it;
-    // drain the loads before the workgroup can end (or reuse the ring)
-    __builtin_amdgcn_s_waitcnt(0);
+    // drain the loads before the workgroup can end (or reuse the ring).
+    // XSEG: the loads in flight are the next segment's first stages; they
+    // are drained after the epilogue (vmcnt also counts its stores)
+    if (!XSEG) __builtin_amdgcn_s_waitcnt(0);
     if (CRT) {
         oz_epilogue_crt<WTM, WTN>(g, acc, m0, n0, d, first);
+        if (XSEG) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     } else if (FUSED) {
         oz_epilogue_w<WTM, WTN>(g, acc, m0, n0, d, first);
