@@ -756,7 +756,7 @@ struct CrtTables {
     double pd[kCrtMax], invpd[kCrtMax];
     float gmul[kCrtMax][kCrtMax];                    // M_j mod p_t (balanced), M_j = p_0 ... p_{j-1}, j < t
     float ginv[kCrtMax];                             // M_t^-1 mod p_t
-    int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax], c84[kCrtMax];  // 2^21 .. 2^84 mod p (balanced)
+    int c15[kCrtMax], c21[kCrtMax], c42[kCrtMax], c63[kCrtMax], c84[kCrtMax];  // 2^15 .. 2^84 mod p (balanced)
 };
 
 constexpr CrtTables crt_tables()
@@ -768,6 +768,7 @@ constexpr CrtTables crt_tables()
         c.invp[t] = 1.0f / (float) p;
         c.pd[t] = (double) p;
         c.invpd[t] = 1.0 / (double) p;
+        c.c15[t] = crt_pow2(15, p);
         c.c21[t] = crt_pow2(21, p);
         c.c42[t] = crt_pow2(42, p);
         c.c63[t] = crt_pow2(63, p);
@@ -806,13 +807,15 @@ __device__ __forceinline__ float crt_balf(float y, float p, float invp)
     return fmaf(-rintf(y * invp), p, y);
 }
 
-// Balanced residue modulo an odd p of an INT32 |x| < 2^30: a first quotient
-// from the rounded float leaves |r1| < 3p, the second step is crt_balf.
-__device__ __forceinline__ int crt_mod_i32(int x, float p, float invp)
+// Balanced residue modulo an odd p (149 <= p <= 255) of an INT32 |x| < 2^30:
+// x = h 2^15 + l folds to y = h c15 + l = x (mod p) with c15 = 2^15 mod p
+// (balanced), |y| < 2^23 so y is exact in FP32; y / p is at least 1/(2p)
+// from a half-integer while the rounded quotient errs by less than 2^-9,
+// so one rint gives the balanced remainder exactly.
+__device__ __forceinline__ int crt_mod_i32(int x, float p, float invp, int c15)
 {
-    const int q1 = (int) rintf((float) x * invp);          // |q1| < 2^23: a 24-bit operand
-    const int r1 = x - __mul24(q1, (int) p);
-    return (int) crt_balf((float) r1, p, invp);
+    const float y = (float) (__mul24(x >> 15, c15) + (x & 0x7fff));
+    return (int) fmaf(-rintf(y * invp), p, y);
 }
 
 // Per line: S = sum over elements of ceil(m4^2 2^(2(ex - E) + 32)), with E
@@ -953,6 +956,7 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
     }
     for (int m = 1; m < nmod; ++m) {
         const int c84 = c_crt.c84[m], c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
+        const int c15 = c_crt.c15[m];
         const float p = c_crt.p[m], ip = c_crt.invp[m];
         u32x4 v;
 #pragma unroll
@@ -963,7 +967,7 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
                 const int j = 4 * q + b;
                 const int s = (NP > 4 ? __mul24(x4[j], c84) : 0) + (NP > 3 ? __mul24(x3[j], c63) : 0) +
                               __mul24(x2[j], c42) + __mul24(x1[j], c21) + x0[j];
-                const int r = crt_mod_i32(s, p, ip);
+                const int r = crt_mod_i32(s, p, ip, c15);
                 w |= ((uint32_t) r & 255u) << (8 * b);
             }
             v[q] = w;
@@ -1008,6 +1012,7 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave & 1, wn = wave >> 1;
     const float p = c_crt.p[t], ip = c_crt.invp[t];
+    const int c15 = c_crt.c15[t];
     typedef __attribute__((address_space(1))) uint32_t gu32;
     gu32 *Rt = (gu32 *) oz_rflp(reinterpret_cast<uint32_t *>(g.R8 + (int64_t) t * g.rplane));
     const int mp16 = __builtin_amdgcn_readfirstlane(g.Mp >> 4);
@@ -1026,7 +1031,7 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int v = acc[i][j][r] + (int) (int8_t) (prev >> (8 * r));
-                    const int res = decltype(ODD)::value ? crt_mod_i32(v, p, ip) : v;
+                    const int res = decltype(ODD)::value ? crt_mod_i32(v, p, ip, c15) : v;
                     w |= ((uint32_t) res & 255u) << (8 * r);
                 }
                 wp[lane] = w;
