@@ -719,13 +719,14 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(Oz
 // Residue planes (balanced int8) go through the same pipelined INT8 kernel,
 // one segment per (modulus, k-chunk), whose epilogue keeps Z mod p as int8
 // planes R_t (ld Mp); ozaki2_crt_kernel turns the N residues of an element
-// into balanced mixed-radix digits (Garner) and evaluates them in FP64 by
-// Horner from the top digit (rounded multiply, then rounded add).
+// into balanced mixed-radix digits over the pair products Q_i = p_2i p_2i+1
+// (Garner) and evaluates them in FP64 by Horner from the top digit (rounded
+// multiply, then rounded add).
 constexpr int kCrtMax = 25;
 constexpr int kCrtMin = 4;
 constexpr int kCrtKMax = 1 << 23;   // the integer norm sums stay below 2^63
 // k-chunk: kchunk * 128 * 128 + 128 < 2^30 (the INT32 sum plus the previous
-// residue stays in crt_mod_i32's range), a multiple of kKp
+// residue stays within INT32), a multiple of kKp
 constexpr int kCrtChunk = (int) ((((1u << 30) - 1 - 128) / 16384u) / kKp * kKp);
 
 constexpr int kCrtModuli[kCrtMax] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211,
@@ -751,12 +752,33 @@ constexpr int crt_pow2(int e, int p)                // 2^e mod p, balanced
     return r > p / 2 ? r - p : r;
 }
 
+constexpr int64_t crt_bal64(int64_t x, int64_t q)  // balanced residue: [-(q-1)/2, (q-1)/2], [-q/2, q/2) for even q
+{
+    int64_t r = x % q;
+    if (r < 0) r += q;
+    return r > ((q & 1) ? (q - 1) / 2 : q / 2 - 1) ? r - q : r;
+}
+
+// Reconstruction over composite moduli: Q_i = p_2i p_2i+1 (i < N / 2) and,
+// for odd N, the single p_N-1 as the last one; kCrtC composites at most.
+constexpr int kCrtC = (kCrtMax + 1) / 2;
+
 struct CrtTables {
-    float p[kCrtMax], invp[kCrtMax];                 // modulus, 1/p (rounded)
-    double pd[kCrtMax], invpd[kCrtMax];
-    float gmul[kCrtMax][kCrtMax];                    // M_j mod p_t (balanced), M_j = p_0 ... p_{j-1}, j < t
-    float ginv[kCrtMax];                             // M_t^-1 mod p_t
-    int c15[kCrtMax], c21[kCrtMax], c42[kCrtMax], c63[kCrtMax], c84[kCrtMax];  // 2^15 .. 2^84 mod p (balanced)
+    double pd[kCrtMax], invpd[kCrtMax];              // modulus, 1/p (rounded)
+    int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax], c84[kCrtMax];  // 2^21 .. 2^84 mod p (balanced)
+    // Garner over the pairs: with M_i = Q_0 ... Q_i-1, g_i = M_i^-1 mod Q_i
+    // and the idempotents E_a = 1 (mod p_2i), 0 (mod p_2i+1) and E_b the
+    // other way round, digit i is V_i = (r_2i ga_i + r_2i+1 gb_i +
+    // sum_j<i V_j h_ij) mod Q_i with ga_i = E_a g_i, gb_i = E_b g_i and
+    // h_ij = -M_j g_i, all balanced modulo Q_i.
+    double q[kCrtC], invq[kCrtC];                    // Q_i, 1/Q_i (rounded)
+    float ga[kCrtC], gb[kCrtC];
+    double h[kCrtC][kCrtC];
+    // the same for a single last modulus p_2i (odd N): g = M_i^-1 mod p_2i
+    double qs[kCrtC], invqs[kCrtC];
+    float gs[kCrtC];
+    double hs[kCrtC][kCrtC];
+    double bias0;                                    // 1/2 + 1/(2 Q_0): rounding offset of the even Q_0
 };
 
 constexpr CrtTables crt_tables()
@@ -764,58 +786,72 @@ constexpr CrtTables crt_tables()
     CrtTables c{};
     for (int t = 0; t < kCrtMax; ++t) {
         const int p = kCrtModuli[t];
-        c.p[t] = (float) p;
-        c.invp[t] = 1.0f / (float) p;
         c.pd[t] = (double) p;
         c.invpd[t] = 1.0 / (double) p;
-        c.c15[t] = crt_pow2(15, p);
         c.c21[t] = crt_pow2(21, p);
         c.c42[t] = crt_pow2(42, p);
         c.c63[t] = crt_pow2(63, p);
         c.c84[t] = crt_pow2(84, p);
-        int mj = 1;                                  // M_j mod p_t
-        for (int j = 0; j < kCrtMax; ++j) {
-            c.gmul[t][j] = j < t ? (float) (mj > p / 2 ? mj - p : mj) : 0.0f;
-            if (j < t) mj = (mj * (kCrtModuli[j] % p)) % p;
-        }
-        c.ginv[t] = (float) crt_inv(mj, p);
     }
+    int64_t Q[kCrtC] = {};
+    for (int i = 0; i < kCrtC; ++i)
+        Q[i] = 2 * i + 1 < kCrtMax ? (int64_t) kCrtModuli[2 * i] * kCrtModuli[2 * i + 1] : kCrtModuli[2 * i];
+    for (int i = 0; i < kCrtC; ++i) {
+        const int64_t pa = kCrtModuli[2 * i];
+        for (int single = 0; single < 2; ++single) {
+            if (!single && 2 * i + 1 >= kCrtMax) continue;
+            const int64_t qi = single ? pa : Q[i];
+            int64_t mi = 1;                          // M_i mod qi
+            for (int j = 0; j < i; ++j) mi = mi * (Q[j] % qi) % qi;
+            const int64_t gi = crt_inv((int) mi, (int) qi);
+            int64_t mj = 1;                          // M_j mod qi
+            for (int j = 0; j < kCrtC; ++j) {
+                const double hv = j < i ? (double) crt_bal64(-mj * gi, qi) : 0.0;
+                if (single) c.hs[i][j] = hv;
+                else c.h[i][j] = hv;
+                if (j < i) mj = mj * (Q[j] % qi) % qi;
+            }
+            if (single) {
+                c.qs[i] = (double) qi;
+                c.invqs[i] = 1.0 / (double) qi;
+                c.gs[i] = (float) crt_bal64(gi, qi);
+            } else {
+                const int64_t pb = kCrtModuli[2 * i + 1];
+                const int64_t ea = pb * crt_inv((int) (pb % pa), (int) pa) % qi;
+                const int64_t eb = pa * crt_inv((int) (pa % pb), (int) pb) % qi;
+                c.q[i] = (double) qi;
+                c.invq[i] = 1.0 / (double) qi;
+                c.ga[i] = (float) crt_bal64(ea * gi, qi);
+                c.gb[i] = (float) crt_bal64(eb * gi, qi);
+            }
+        }
+    }
+    c.bias0 = 0.5 + 0.5 / (double) Q[0];
     return c;
 }
 
 __constant__ CrtTables c_crt = crt_tables();
+// Q_0 = 256 * 255: E_a = -255, E_b = 256; Q_1 = 253 * 251: g_1 = 65280^-1
+static_assert(crt_tables().ga[0] == -255.0f && crt_tables().gb[0] == 256.0f && crt_tables().ga[1] == 5271.0f &&
+                  crt_tables().gb[1] == -17457.0f && crt_tables().h[1][0] == 12186.0,
+              "composite Garner constants");
 
-// Balanced residue of an integer-valued y (|y| < 2^45) modulo p: the
-// quotient rint(y / p) from the rounded reciprocal is exact or off by one,
-// and the two corrections bring r into [-(p-1)/2, (p-1)/2] ([-128, 127]
-// for p = 256).
-__device__ __forceinline__ double crt_bal(double y, double p, double invp)
+// Balanced residue modulo an odd p (149 <= p <= 255) of an INT32 x: FP64
+// holds x exactly, |x / p| < 2^24 so the rounded quotient x (1/p) errs by
+// less than 2^-27, while x / p is at least 1/(2p) from a half-integer; rint
+// finds the nearest quotient and the remainder is exactly the balanced one.
+__device__ __forceinline__ int crt_mod_i32(int x, double p, double invp)
 {
-    double r = fma(-rint(y * invp), p, y);
-    const double hi = floor((p - 1.0) * 0.5);
-    r = r > hi ? r - p : r;
-    r = r < hi - p + 1.0 ? r + p : r;
-    return r;
+    const double y = (double) x;
+    return (int) fma(-rint(y * invp), p, y);
 }
 
-// Balanced residue modulo an odd p of an integer-valued float y, |y| < 2^19:
-// y / p is at least 1/(2p) from a half-integer and the rounded quotient
-// errs by less than 2^-11, so rint finds the nearest quotient and the
-// remainder is exactly the balanced one.
-__device__ __forceinline__ float crt_balf(float y, float p, float invp)
+// Low bytes of r0 .. r3 as one word (byte q = r_q), three byte permutes.
+__device__ __forceinline__ uint32_t crt_pack4(int r0, int r1, int r2, int r3)
 {
-    return fmaf(-rintf(y * invp), p, y);
-}
-
-// Balanced residue modulo an odd p (149 <= p <= 255) of an INT32 |x| < 2^30:
-// x = h 2^15 + l folds to y = h c15 + l = x (mod p) with c15 = 2^15 mod p
-// (balanced), |y| < 2^23 so y is exact in FP32; y / p is at least 1/(2p)
-// from a half-integer while the rounded quotient errs by less than 2^-9,
-// so one rint gives the balanced remainder exactly.
-__device__ __forceinline__ int crt_mod_i32(int x, float p, float invp, int c15)
-{
-    const float y = (float) (__mul24(x >> 15, c15) + (x & 0x7fff));
-    return (int) fmaf(-rintf(y * invp), p, y);
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t) r1, (uint32_t) r0, 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t) r3, (uint32_t) r2, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
 // Per line: S = sum over elements of ceil(m4^2 2^(2(ex - E) + 32)), with E
@@ -946,31 +982,24 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
     {
         u32x4 v;                                    // p_0 = 256: 2^21 = 0 (mod 256)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t w = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) w |= ((uint32_t) x0[4 * q + b] & 255u) << (8 * b);
-            v[q] = w;
-        }
+        for (int q = 0; q < 4; ++q) v[q] = crt_pack4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
         *reinterpret_cast<u32x4 *>(base) = v;
     }
     for (int m = 1; m < nmod; ++m) {
         const int c84 = c_crt.c84[m], c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
-        const int c15 = c_crt.c15[m];
-        const float p = c_crt.p[m], ip = c_crt.invp[m];
+        const double p = c_crt.pd[m], ip = c_crt.invpd[m];
         u32x4 v;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            uint32_t w = 0;
+            int r[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = 4 * q + b;
                 const int s = (NP > 4 ? __mul24(x4[j], c84) : 0) + (NP > 3 ? __mul24(x3[j], c63) : 0) +
                               __mul24(x2[j], c42) + __mul24(x1[j], c21) + x0[j];
-                const int r = crt_mod_i32(s, p, ip, c15);
-                w |= ((uint32_t) r & 255u) << (8 * b);
+                r[b] = crt_mod_i32(s, p, ip);
             }
-            v[q] = w;
+            v[q] = crt_pack4(r[0], r[1], r[2], r[3]);
         }
         *reinterpret_cast<u32x4 *>(base + (int64_t) m * plane) = v;
     }
@@ -1011,8 +1040,7 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave & 1, wn = wave >> 1;
-    const float p = c_crt.p[t], ip = c_crt.invp[t];
-    const int c15 = c_crt.c15[t];
+    const double p = c_crt.pd[t], ip = c_crt.invpd[t];
     typedef __attribute__((address_space(1))) uint32_t gu32;
     gu32 *Rt = (gu32 *) oz_rflp(reinterpret_cast<uint32_t *>(g.R8 + (int64_t) t * g.rplane));
     const int mp16 = __builtin_amdgcn_readfirstlane(g.Mp >> 4);
@@ -1027,14 +1055,13 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
             for (int j = 0; j < FJ; ++j) {
                 gu32 *wp = Rt + (int64_t) ((cb0 + j) * mp16 + rb0 + i) * 64;
                 const uint32_t prev = decltype(FIRST)::value ? 0u : wp[lane];
-                uint32_t w = 0;
+                int res[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int v = acc[i][j][r] + (int) (int8_t) (prev >> (8 * r));
-                    const int res = decltype(ODD)::value ? crt_mod_i32(v, p, ip, c15) : v;
-                    w |= ((uint32_t) res & 255u) << (8 * r);
+                    res[r] = decltype(ODD)::value ? crt_mod_i32(v, p, ip) : v;
                 }
-                wp[lane] = w;
+                wp[lane] = crt_pack4(res[0], res[1], res[2], res[3]);
             }
         }
     };
@@ -1048,41 +1075,49 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
 }
 
 // One lane's 4 elements of block (rb, cb): row rb * 16 + lane % 16,
-// columns cb * 16 + 4 (lane / 16) + q.  From the N residues r_t, Garner's
-// balanced mixed-radix digits v_t = (r_t - sum_{j<t} v_j M_j) M_t^-1 mod p_t
-// (M_j = p_0 ... p_{j-1}; FP32 and exact: the sum stays below 2^19 and is
-// reduced before the product), then x = v_0 + p_0 (v_1 + p_1 (...)) in FP64
-// from the top digit, and 2^-eA 2^-eB x (0 for an empty line).
+// columns cb * 16 + 4 (lane / 16) + q.  Garner's balanced digits V_i over
+// the composite moduli: the member residues enter in FP32 (|r ga + r' gb|
+// < 2^24, exact), the digit sum and its reduction run in FP64 (|sum| <
+// 2^35 and the rounded quotient errs by far less than 1/(2 Q_i), so rint
+// finds the balanced remainder; floor with the offset 1/2 + 1/(2 Q_0) gives
+// [-Q_0/2, Q_0/2) for the even Q_0).  Then x = V_0 + Q_0 (V_1 + Q_1 (...))
+// in FP64 from the top digit, and 2^-eA 2^-eB x (0 for an empty line).
 template <int NM>
 __device__ __forceinline__ void oz2_crt_block(const int8_t *__restrict__ R, int64_t rplane, int Mp, int rb, int cb,
                                               int lane, const int *__restrict__ eA, const int *__restrict__ eB,
                                               double (&out)[4])
 {
-    const uint32_t *W = reinterpret_cast<const uint32_t *>(R) + crt_block_word(Mp, rb, cb) + lane;
-    float v[4][NM];
+    constexpr int NC = (NM + 1) / 2;
+    // rb and cb are wave-uniform: the block's plane addresses stay in scalar
+    // registers and a lane adds only its 4-byte slot
+    typedef __attribute__((address_space(1))) const uint32_t gcu32;
+    const uint32_t *W = reinterpret_cast<const uint32_t *>(R) + crt_block_word(Mp, rb, cb);
+    uint32_t w[NM];
 #pragma unroll
-    for (int t = 0; t < NM; ++t) {
-        const uint32_t w = W[(t * rplane) >> 2];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q][t] = (float) (int8_t) (w >> (8 * q));
-    }
-#pragma unroll
-    for (int t = 1; t < NM; ++t) {
-        const float p = c_crt.p[t], ip = c_crt.invp[t], gi = c_crt.ginv[t];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float sum = v[q][0];                           // M_0 = 1
-#pragma unroll
-            for (int s = 1; s < t; ++s) sum = fmaf(v[q][s], c_crt.gmul[t][s], sum);
-            v[q][t] = crt_balf((v[q][t] - crt_balf(sum, p, ip)) * gi, p, ip);
-        }
-    }
+    for (int t = 0; t < NM; ++t) w[t] = ((gcu32 *) oz_rflp(W + t * (rplane >> 2)))[lane];
     const int ea = eA[rb * 16 + (lane & 15)];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        double x = (double) v[q][NM - 1];
+        double V[NC];
 #pragma unroll
-        for (int t = NM - 2; t >= 0; --t) x = ca3::add_rn(ca3::mul_rn(x, c_crt.pd[t]), (double) v[q][t]);
+        for (int i = 0; i < NC; ++i) {
+            const bool pair = 2 * i + 1 < NM;
+            const float ra = (float) (int8_t) (w[2 * i] >> (8 * q));
+            const float s32 = pair ? fmaf((float) (int8_t) (w[2 * i + 1] >> (8 * q)), c_crt.gb[i], ra * c_crt.ga[i])
+                                   : ra * c_crt.gs[i];
+            double s = (double) s32;
+#pragma unroll
+            for (int j = 0; j < i; ++j) s = fma(V[j], pair ? c_crt.h[i][j] : c_crt.hs[i][j], s);
+            if (i == 0) {
+                V[i] = fma(-floor(fma(s, c_crt.invq[0], c_crt.bias0)), c_crt.q[0], s);
+            } else {
+                const double qi = pair ? c_crt.q[i] : c_crt.qs[i], iq = pair ? c_crt.invq[i] : c_crt.invqs[i];
+                V[i] = fma(-rint(s * iq), qi, s);
+            }
+        }
+        double x = V[NC - 1];
+#pragma unroll
+        for (int i = NC - 2; i >= 0; --i) x = ca3::add_rn(ca3::mul_rn(x, c_crt.q[i]), V[i]);
         const int eb = eB[cb * 16 + 4 * (lane >> 4) + q];
         out[q] = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(x, -ea), -eb) : 0.0;
     }
@@ -1097,7 +1132,7 @@ __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restric
                                                           double *C, int64_t ldc)
 {
     const int nbm = (M + 15) >> 4, nbn = (N + 15) >> 4;
-    const int64_t b = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t b = (int64_t) blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int) threadIdx.x >> 6);
     if (b >= (int64_t) nbm * nbn) return;
     const int lane = threadIdx.x & 63;
     const int cb = (int) (b / nbm), rb = (int) (b - (int64_t) cb * nbm);
@@ -1142,7 +1177,7 @@ __global__ __launch_bounds__(256) void ozaki2_crt_grouped_kernel(const CrtJob *_
         if (blk0[mid] <= wg) lo = mid; else hi = mid - 1;
     }
     const CrtJob &J = jobs[lo];
-    const int64_t b = (int64_t) (wg - blk0[lo]) * 4 + (threadIdx.x >> 6);
+    const int64_t b = (int64_t) (wg - blk0[lo]) * 4 + __builtin_amdgcn_readfirstlane((int) threadIdx.x >> 6);
     const int nbn = (J.N + 15) >> 4;
     if (b >= (int64_t) J.nbm * nbn) return;
     const int lane = threadIdx.x & 63;

@@ -221,11 +221,14 @@ def diagonals8(SA, SB, s):
 #   * residue planes r_t(a') in [-(p_t-1)/2, (p_t-1)/2] ([-128, 127] for 256)
 #     are int8; Z_t = R_t(A') R_t(B') is exact in INT32 over k-chunks, and
 #     z_t = Z_t mod p_t (balanced);
-#   * Garner's algorithm turns (z_t) into the balanced mixed-radix digits v_t
-#     of the unique x in [-P/2, P/2) with x = C' (mod P), i.e. x = C';
-#   * x is evaluated in FP64 by Horner from the top digit, one rounded
-#     multiply and one rounded add per step (no fused multiply-add), and
-#     C_ij = alpha 2^-e_i 2^-f_j x (+ beta C_in).
+#   * the moduli are taken in pairs, Q_0 = p_0 p_1, Q_1 = p_2 p_3, ... (for
+#     odd N the last modulus stays single), and Garner's algorithm over the
+#     Q_i turns (z_t) into the balanced mixed-radix digits V_i (in
+#     [-Q_i/2, Q_i/2) for the even Q_0) of the unique x in [-P/2, P/2) with
+#     x = C' (mod P), i.e. x = C';
+#   * x = V_0 + Q_0 (V_1 + Q_1 (V_2 + ...)) is evaluated in FP64 by Horner
+#     from the top digit, one rounded multiply and one rounded add per step
+#     (no fused multiply-add), and C_ij = alpha 2^-e_i 2^-f_j x (+ beta C_in).
 # Everything up to the digits is exact integer arithmetic, so any correct
 # implementation agrees; the Horner sum fixes the rounding, and the device
 # kernel follows the same order bit for bit.
@@ -312,27 +315,42 @@ def crt_residues(Aint, N):
     return out
 
 
+def crt_composites(N):
+    """Composite moduli of the reconstruction: Q_i = p_2i p_2i+1, and p_N-1
+    alone as the last one for odd N."""
+    ps = CRT_MODULI[:N]
+    return [ps[i] * (ps[i + 1] if i + 1 < N else 1) for i in range(0, N, 2)]
+
+
 def crt_garner(Z, N):
-    """Balanced mixed-radix digits v_0..v_{N-1} (int64 arrays) of the x in
-    [-P/2, P/2) with x = Z_t (mod p_t) for every t; Z: list of residues."""
+    """Balanced mixed-radix digits V_0..V_{C-1} (int64 arrays) over the
+    composite moduli of the x in [-P/2, P/2) with x = Z_t (mod p_t) for
+    every t; Z: list of residues.  Exact integer arithmetic: the residue
+    modulo Q_i comes from its members by the two-modulus CRT, then
+    Garner's step V_i = (x - V_0 - V_1 Q_0 - ...) / (Q_0 ... Q_i-1) mod Q_i."""
     ps = CRT_MODULI[:N]
-    v = []
-    for t in range(N):
-        x = Z[t].astype(np.int64)
-        for j in range(t):
-            x = ((x - v[j]) * pow(ps[j], -1, ps[t])) % ps[t]
-        v.append(balanced_mod(x, ps[t]))
-    return v
+    Qs = crt_composites(N)
+    V = []
+    for i, Q in enumerate(Qs):
+        a = 2 * i
+        x = Z[a].astype(np.int64) % ps[a]
+        if a + 1 < N:
+            pa, pb = ps[a], ps[a + 1]
+            x = x + pa * (((Z[a + 1].astype(np.int64) - x) * pow(pa, -1, pb)) % pb)   # in [0, Q)
+        for j in range(i):
+            x = ((x - V[j]) * pow(Qs[j], -1, Q)) % Q
+        V.append(balanced_mod(x, Q))
+    return V
 
 
-def crt_horner(v, N):
-    """x = v_0 + p_0 (v_1 + p_1 (v_2 + ...)) in FP64, from the top digit,
+def crt_horner(V, N):
+    """x = V_0 + Q_0 (V_1 + Q_1 (V_2 + ...)) in FP64, from the top digit,
     a rounded multiply then a rounded add per step."""
-    ps = CRT_MODULI[:N]
-    x = v[N - 1].astype(np.float64)
-    for t in range(N - 2, -1, -1):
-        x = x * float(ps[t])
-        x = x + v[t].astype(np.float64)
+    Qs = crt_composites(N)
+    x = V[-1].astype(np.float64)
+    for i in range(len(Qs) - 2, -1, -1):
+        x = x * float(Qs[i])
+        x = x + V[i].astype(np.float64)
     return x
 
 

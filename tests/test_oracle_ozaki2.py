@@ -59,11 +59,18 @@ def test_garner_digits_rebuild_the_integer(N):
     xs += [-(P // 2), P // 2 - 1, 0, 1, -1]
     Zs = [np.array([_bal(x, p) for x in xs], dtype=np.int64) for p in Z.CRT_MODULI[:N]]
     v = Z.crt_garner(Zs, N)
+    Qs = Z.crt_composites(N)
+    prodQ = 1
+    for Q in Qs:
+        prodQ *= Q
+    assert len(v) == (N + 1) // 2 and prodQ == P
     for idx, x in enumerate(xs):
         acc, M = 0, 1
-        for t, p in enumerate(Z.CRT_MODULI[:N]):
-            acc += int(v[t][idx]) * M
-            M *= p
+        for i, Q in enumerate(Qs):
+            d = int(v[i][idx])
+            assert -(Q // 2) <= d <= (Q - 1) // 2
+            acc += d * M
+            M *= Q
         assert acc == x
 
 
@@ -97,26 +104,28 @@ def test_line_scale_keeps_the_norm_bound(N, phi):
 
 def _bruteforce(opA, opB, N, alpha, beta, Cin):
     """No residues: exact Python-integer dot products, then the balanced
-    mixed-radix digits of C' itself, then the same Horner sum."""
+    mixed-radix digits of C' itself over the pair products p_0 p_1,
+    p_2 p_3, ... (p_N-1 alone last for odd N), then the same Horner sum."""
     Ai, eA, zA = Z.crt_scaled(opA, 1, N)
     Bi, eB, zB = Z.crt_scaled(opB, 0, N)
     m, k = opA.shape
     n = opB.shape[1]
     ps = Z.CRT_MODULI[:N]
+    Qs = [ps[t] * (ps[t + 1] if t + 1 < N else 1) for t in range(0, N, 2)]
     C = np.zeros((m, n))
     for i in range(m):
         for j in range(n):
             x = sum(int(Ai[i, l]) * int(Bi[l, j]) for l in range(k))
             assert abs(x) < Z.crt_product(N) // 2
             digits = []
-            for p in ps:
-                d = _bal(x, p)
+            for Q in Qs:
+                d = _bal(x, Q)
                 digits.append(d)
-                x = (x - d) // p
+                x = (x - d) // Q
             assert x == 0
             acc = float(digits[-1])
-            for t in range(N - 2, -1, -1):
-                acc = acc * float(ps[t])
+            for t in range(len(Qs) - 2, -1, -1):
+                acc = acc * float(Qs[t])
                 acc = acc + float(digits[t])
             v = 0.0 if (zA[i] or zB[j]) else np.ldexp(np.ldexp(acc, -int(eA[i])), -int(eB[j]))
             v = alpha * v
