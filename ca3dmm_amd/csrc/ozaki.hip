@@ -68,27 +68,34 @@ __device__ __forceinline__ void oz_max_body(int bx, int by, int gy, const double
     const int kchunk = (K + gy - 1) / gy;
     const int k0 = by * kchunk, k1 = min(K, k0 + kchunk);
     const int l0 = bx * 64;
+    if (!line_contig) {
+        // k-contiguous lines: each wave walks 16 of the block's lines, its
+        // lanes reading consecutive elements (512-byte loads), then reduces
+        // across the wave
+        const int lane = t & 63, w = t >> 6;
+        for (int r = 0; r < 16; ++r) {
+            const int l = l0 + w * 16 + r;
+            if (l >= lines) break;
+            const double *xl = X + (int64_t) l * sl;
+            double m = 0.0;
+            for (int kk = k0 + lane; kk < k1; kk += 64) m = fmax(m, fabs(xl[kk]));
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+            if (lane == 0 && m > 0.0) atomicMax(&mx[l], (unsigned long long) __double_as_longlong(m));
+        }
+        return;
+    }
     __shared__ double red[256];
     double m = 0.0;
-    if (line_contig) {
-        const int myline = l0 + (t & 63);
-        for (int kk = k0 + (t >> 6); kk < k1; kk += 4)
-            if (myline < lines) m = fmax(m, fabs(X[myline + (int64_t) kk * sk]));
-    } else {
-        const int myline = l0 + (t >> 2);
-        for (int kk = k0 + (t & 3); kk < k1; kk += 4)
-            if (myline < lines) m = fmax(m, fabs(X[(int64_t) myline * sl + kk]));
-    }
+    const int myline = l0 + (t & 63);
+    for (int kk = k0 + (t >> 6); kk < k1; kk += 4)
+        if (myline < lines) m = fmax(m, fabs(X[myline + (int64_t) kk * sk]));
     red[t] = m;
     __syncthreads();
     if (t < 64) {
         const int l = l0 + t;
         double v = 0.0;
-        if (line_contig) {
-            for (int j = 0; j < 4; ++j) v = fmax(v, red[t + 64 * j]);
-        } else {
-            for (int j = 0; j < 4; ++j) v = fmax(v, red[4 * t + j]);
-        }
+        for (int j = 0; j < 4; ++j) v = fmax(v, red[t + 64 * j]);
         if (l < lines && v > 0.0) atomicMax(&mx[l], (unsigned long long) __double_as_longlong(v));
     }
 }
@@ -846,6 +853,15 @@ __device__ __forceinline__ int crt_mod_i32(int x, double p, double invp)
     return (int) fma(-rint(y * invp), p, y);
 }
 
+// a b + c for 24-bit signed a, b (b wave-uniform): one v_mad_i32_i24 (the
+// compiler otherwise forms two 24-bit products and a three-way add).
+__device__ __forceinline__ int crt_mad24(int a, int b, int c)
+{
+    int d;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+
 // Low bytes of r0 .. r3 as one word (byte q = r_q), three byte permutes.
 __device__ __forceinline__ uint32_t crt_pack4(int r0, int r1, int r2, int r3)
 {
@@ -878,28 +894,37 @@ __device__ __forceinline__ void oz2_norm_body(int bx, int by, int gy, const doub
     const int kchunk = (K + gy - 1) / gy;
     const int k0 = by * kchunk, k1 = min(K, k0 + kchunk);
     const int l0 = bx * 64;
+    if (!line_contig) {
+        // k-contiguous lines: a wave per line at a time, as in oz_max_body
+        const int lane = t & 63, w = t >> 6;
+        for (int r = 0; r < 16; ++r) {
+            const int l = l0 + w * 16 + r;
+            if (l >= lines) break;
+            int E;
+            (void) frexp(__longlong_as_double((long long) mx[l]), &E);
+            const double *xl = X + (int64_t) l * sl;
+            unsigned long long acc = 0;
+            for (int kk = k0 + lane; kk < k1; kk += 64) acc += crt_norm_term(xl[kk], E);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+            if (lane == 0 && acc != 0) atomicAdd(&S[l], acc);
+        }
+        return;
+    }
     __shared__ unsigned long long red[256];
     unsigned long long acc = 0;
-    const int myline = line_contig ? l0 + (t & 63) : l0 + (t >> 2);
+    const int myline = l0 + (t & 63);
     if (myline < lines) {
         int E;
         (void) frexp(__longlong_as_double((long long) mx[myline]), &E);
-        if (line_contig) {
-            for (int kk = k0 + (t >> 6); kk < k1; kk += 4) acc += crt_norm_term(X[myline + (int64_t) kk * sk], E);
-        } else {
-            for (int kk = k0 + (t & 3); kk < k1; kk += 4) acc += crt_norm_term(X[(int64_t) myline * sl + kk], E);
-        }
+        for (int kk = k0 + (t >> 6); kk < k1; kk += 4) acc += crt_norm_term(X[myline + (int64_t) kk * sk], E);
     }
     red[t] = acc;
     __syncthreads();
     if (t < 64) {
         const int l = l0 + t;
         unsigned long long v = 0;
-        if (line_contig) {
-            for (int j = 0; j < 4; ++j) v += red[t + 64 * j];
-        } else {
-            for (int j = 0; j < 4; ++j) v += red[4 * t + j];
-        }
+        for (int j = 0; j < 4; ++j) v += red[t + 64 * j];
         if (l < lines && v != 0) atomicAdd(&S[l], v);
     }
 }
@@ -983,7 +1008,7 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
         u32x4 v;                                    // p_0 = 256: 2^21 = 0 (mod 256)
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = crt_pack4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
-        *reinterpret_cast<u32x4 *>(base) = v;
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(base));
     }
     for (int m = 1; m < nmod; ++m) {
         const int c84 = c_crt.c84[m], c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
@@ -995,13 +1020,15 @@ __device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *_
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = 4 * q + b;
-                const int s = (NP > 4 ? __mul24(x4[j], c84) : 0) + (NP > 3 ? __mul24(x3[j], c63) : 0) +
-                              __mul24(x2[j], c42) + __mul24(x1[j], c21) + x0[j];
+                int s = crt_mad24(x1[j], c21, x0[j]);
+                s = crt_mad24(x2[j], c42, s);
+                if (NP > 3) s = crt_mad24(x3[j], c63, s);
+                if (NP > 4) s = crt_mad24(x4[j], c84, s);
                 r[b] = crt_mod_i32(s, p, ip);
             }
             v[q] = crt_pack4(r[0], r[1], r[2], r[3]);
         }
-        *reinterpret_cast<u32x4 *>(base + (int64_t) m * plane) = v;
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(base + (int64_t) m * plane));
     }
 }
 
@@ -1094,7 +1121,7 @@ __device__ __forceinline__ void oz2_crt_block(const int8_t *__restrict__ R, int6
     const uint32_t *W = reinterpret_cast<const uint32_t *>(R) + crt_block_word(Mp, rb, cb);
     uint32_t w[NM];
 #pragma unroll
-    for (int t = 0; t < NM; ++t) w[t] = ((gcu32 *) oz_rflp(W + t * (rplane >> 2)))[lane];
+    for (int t = 0; t < NM; ++t) w[t] = __builtin_nontemporal_load((gcu32 *) oz_rflp(W + t * (rplane >> 2)) + lane);
     const int ea = eA[rb * 16 + (lane & 15)];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1147,7 +1174,7 @@ __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restric
         double *cp = C + (int64_t) col * ldc + row;
         double v = ca3::mul_rn(alpha, x[q]);
         if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
-        *cp = v;
+        __builtin_nontemporal_store(v, cp);
     }
 }
 
@@ -1197,7 +1224,7 @@ __global__ __launch_bounds__(256) void ozaki2_crt_grouped_kernel(const CrtJob *_
         if (col >= J.N) break;
         int sl = 0;
         while (col >= J.cs[sl + 1]) ++sl;
-        J.Cout[sl][(int64_t) (col - J.cs[sl]) * J.ldo + row] = sum[q];
+        __builtin_nontemporal_store(sum[q], &J.Cout[sl][(int64_t) (col - J.cs[sl]) * J.ldo + row]);
     }
 }
 
