@@ -27,6 +27,7 @@
 // (d+1) * Kchunk * 128 * 128 < 2^31; for each diagonal the chunks are added
 // in k order (the oracle follows the same order).
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -772,7 +773,7 @@ constexpr int kCrtC = (kCrtMax + 1) / 2;
 
 struct CrtTables {
     double pd[kCrtMax], invpd[kCrtMax];              // modulus, 1/p (rounded)
-    int c21[kCrtMax], c42[kCrtMax], c63[kCrtMax], c84[kCrtMax];  // 2^21 .. 2^84 mod p (balanced)
+    double c27[kCrtMax], c32[kCrtMax], c64[kCrtMax]; // 2^27, 2^32, 2^64 mod p (balanced)
     // Garner over the pairs: with M_i = Q_0 ... Q_i-1, g_i = M_i^-1 mod Q_i
     // and the idempotents E_a = 1 (mod p_2i), 0 (mod p_2i+1) and E_b the
     // other way round, digit i is V_i = (r_2i ga_i + r_2i+1 gb_i +
@@ -795,10 +796,9 @@ constexpr CrtTables crt_tables()
         const int p = kCrtModuli[t];
         c.pd[t] = (double) p;
         c.invpd[t] = 1.0 / (double) p;
-        c.c21[t] = crt_pow2(21, p);
-        c.c42[t] = crt_pow2(42, p);
-        c.c63[t] = crt_pow2(63, p);
-        c.c84[t] = crt_pow2(84, p);
+        c.c27[t] = crt_pow2(27, p);
+        c.c32[t] = crt_pow2(32, p);
+        c.c64[t] = crt_pow2(64, p);
     }
     int64_t Q[kCrtC] = {};
     for (int i = 0; i < kCrtC; ++i)
@@ -851,15 +851,6 @@ __device__ __forceinline__ int crt_mod_i32(int x, double p, double invp)
 {
     const double y = (double) x;
     return (int) fma(-rint(y * invp), p, y);
-}
-
-// a b + c for 24-bit signed a, b (b wave-uniform): one v_mad_i32_i24 (the
-// compiler otherwise forms two 24-bit products and a three-way add).
-__device__ __forceinline__ int crt_mad24(int a, int b, int c)
-{
-    int d;
-    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
-    return d;
 }
 
 // Low bytes of r0 .. r3 as one word (byte q = r_q), three byte permutes.
@@ -956,92 +947,114 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
     e[l] = T + 19 - E - (b + 1) / 2;
 }
 
-// Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| < 2^95)
-// is cut into signed pieces a' = x4 2^84 + x3 2^63 + x2 2^42 + x1 2^21 + x0
-// (|x_i| < 2^21, each step exact), and a' mod p = x4 c84 + x3 c63 + x2 c42 +
-// x1 c21 + x0 (mod p), an INT32 sum of 24-bit products below 2^30, reduced by
-// crt_mod_i32; for p_0 = 256 the residue is the low byte of x0.  Plane t:
+// Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| <= 2^T)
+// is cut exactly into FP64 pieces, NP = 2: a' = h 2^27 + l (T <= 69, |h| <=
+// 2^42, |l| < 2^27); NP = 3: a' = h2 2^64 + h1 2^32 + l (|h1|, |l| < 2^32).
+// Then s = h c27 + l (or h2 c64 + h1 c32 + l) = a' (mod p) with c = 2^k mod
+// p balanced: |s| < 2^49, exact in FP64, and its balanced remainder comes
+// from one rounded quotient (|s / p| < 2^42, error < 2^-10 < 1/(2p)).  For
+// p_0 = 256 the residue is the low byte of l.  Plane t:
 // P[t * plane + line * Kp + kk].
-// NP: pieces needed for |a'| < 2^(T-1): 3 below 2^63, 4 below 2^84, else 5.
 template <int NP>
-__device__ __forceinline__ void oz2_residue_body(int bx, int by, const double *__restrict__ X, int lines, int K,
-                                                 int64_t sl, int64_t sk, const int *__restrict__ ex, int nmod,
-                                                 int8_t *__restrict__ P, int64_t plane, int Kp)
+__device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles, int nbx,
+                                                  const double *__restrict__ X, int lines, int K, int64_t sl,
+                                                  int64_t sk, const int *__restrict__ ex, int nmod,
+                                                  int8_t *__restrict__ P, int64_t plane, int Kp)
 {
     __shared__ double tile[64][65];
     const int t = threadIdx.x;
-    const int l0 = bx * 64, k0 = by * 64;
-    if (sl == 1) {
+    if (tau0 >= ntiles) return;
+    // element p of this thread in a 64 x 64 tile: (line li, k ki), along the
+    // operand's contiguous direction across the wave
+    auto at = [&](int p, int &li, int &ki) __attribute__((always_inline)) {
+        if (sl == 1) { li = t & 63; ki = (t >> 6) + 4 * p; }
+        else { ki = t & 63; li = (t >> 6) + 4 * p; }
+    };
+    double v[16];
+    auto fetch = [&](int tau) __attribute__((always_inline)) {
+        const int l0 = (tau % nbx) * 64, k0 = (tau / nbx) * 64;
+#pragma unroll
         for (int p = 0; p < 16; ++p) {
-            const int li = t & 63, ki = (t >> 6) + 4 * p;
+            int li, ki;
+            at(p, li, ki);
             const int l = l0 + li, kk = k0 + ki;
-            tile[li][ki] = (l < lines && kk < K) ? X[l + (int64_t) kk * sk] : 0.0;
+            v[p] = (l < lines && kk < K) ? X[sl == 1 ? l + (int64_t) kk * sk : (int64_t) l * sl + kk] : 0.0;
         }
-    } else {
+    };
+    fetch(tau0);
+    for (int tau = tau0; tau < ntiles; tau += step) {
+        __syncthreads();                                // the previous tile's LDS reads are done
+#pragma unroll
         for (int p = 0; p < 16; ++p) {
-            const int ki = t & 63, li = (t >> 6) + 4 * p;
-            const int l = l0 + li, kk = k0 + ki;
-            tile[li][ki] = (l < lines && kk < K) ? X[(int64_t) l * sl + kk] : 0.0;
+            int li, ki;
+            at(p, li, ki);
+            tile[li][ki] = v[p];
         }
-    }
-    __syncthreads();
-    const int li = t >> 2, kq = (t & 3) * 16;
-    const int l = l0 + li;
-    const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
-    const int e = zero_line ? 0 : ex[l];
-    int x4[16], x3[16], x2[16], x1[16], x0[16];
+        __syncthreads();
+        if (tau + step < ntiles) fetch(tau + step);     // lands while this tile is reduced
+        const int l0 = (tau % nbx) * 64, k0 = (tau / nbx) * 64;
+        const int li = t >> 2, kq = (t & 3) * 16;
+        const int l = l0 + li;
+        const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
+        const int e = zero_line ? 0 : ex[l];
+        double ph[16], pm[16], pl[16];                  // NP = 2: (-, h, l); NP = 3: (h2, h1, l)
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        double a = zero_line ? 0.0 : rint(ldexp(tile[li][kq + j], e));
-        const double h4 = NP > 4 ? trunc(ldexp(a, -84)) : 0.0;
-        a -= ldexp(h4, 84);
-        const double h3 = NP > 3 ? trunc(ldexp(a, -63)) : 0.0;
-        a -= ldexp(h3, 63);
-        const double h2 = trunc(ldexp(a, -42));
-        a -= ldexp(h2, 42);
-        const double h1 = trunc(ldexp(a, -21));
-        a -= ldexp(h1, 21);
-        x4[j] = (int) h4; x3[j] = (int) h3; x2[j] = (int) h2; x1[j] = (int) h1; x0[j] = (int) a;
-    }
-    int8_t *base = P + (int64_t) l * Kp + k0 + kq;
-    {
-        u32x4 v;                                    // p_0 = 256: 2^21 = 0 (mod 256)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = crt_pack4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
-        __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(base));
-    }
-    for (int m = 1; m < nmod; ++m) {
-        const int c84 = c_crt.c84[m], c63 = c_crt.c63[m], c42 = c_crt.c42[m], c21 = c_crt.c21[m];
-        const double p = c_crt.pd[m], ip = c_crt.invpd[m];
-        u32x4 v;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int r[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = 4 * q + b;
-                int s = crt_mad24(x1[j], c21, x0[j]);
-                s = crt_mad24(x2[j], c42, s);
-                if (NP > 3) s = crt_mad24(x3[j], c63, s);
-                if (NP > 4) s = crt_mad24(x4[j], c84, s);
-                r[b] = crt_mod_i32(s, p, ip);
+        for (int j = 0; j < 16; ++j) {
+            double a = zero_line ? 0.0 : rint(ldexp(tile[li][kq + j], e));
+            if (NP == 3) {
+                ph[j] = trunc(a * 0x1p-64);
+                a = fma(-ph[j], 0x1p64, a);
+                pm[j] = trunc(a * 0x1p-32);
+                pl[j] = fma(-pm[j], 0x1p32, a);
+            } else {
+                pm[j] = trunc(a * 0x1p-27);
+                pl[j] = fma(-pm[j], 0x1p27, a);
             }
-            v[q] = crt_pack4(r[0], r[1], r[2], r[3]);
         }
-        __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(base + (int64_t) m * plane));
+        int8_t *base = P + (int64_t) l * Kp + k0 + kq;
+        {
+            u32x4 w;                                    // p_0 = 256: the low byte of l
+            int lo[16];                                 // |l| < 2^27 (NP = 2) fits INT32; l < 2^32 keeps its low word
+#pragma unroll
+            for (int j = 0; j < 16; ++j) lo[j] = NP == 2 ? (int) pl[j] : (int) (int64_t) pl[j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = crt_pack4(lo[4 * q], lo[4 * q + 1], lo[4 * q + 2], lo[4 * q + 3]);
+            __builtin_nontemporal_store(w, reinterpret_cast<u32x4 *>(base));
+        }
+        for (int m = 1; m < nmod; ++m) {
+            const double ch = c_crt.c64[m], cm = NP == 3 ? c_crt.c32[m] : c_crt.c27[m];
+            const double p = c_crt.pd[m], ip = c_crt.invpd[m];
+            u32x4 w;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int r[4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int j = 4 * q + b;
+                    double sv = fma(pm[j], cm, pl[j]);
+                    if (NP == 3) sv = fma(ph[j], ch, sv);
+                    r[b] = (int) fma(-rint(sv * ip), p, sv);
+                }
+                w[q] = crt_pack4(r[0], r[1], r[2], r[3]);
+            }
+            __builtin_nontemporal_store(w, reinterpret_cast<u32x4 *>(base + (int64_t) m * plane));
+        }
     }
 }
 
+// One workgroup per (nbx * nby) / gridDim.x tiles of 64 lines x 64 k, tile
+// tau = bx + by * nbx taken in grid-stride order.
 template <int NP>
 __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__restrict__ X, int lines, int K,
                                                               int64_t sl, int64_t sk, const int *__restrict__ ex,
-                                                              int nmod, int8_t *__restrict__ P, int64_t plane, int Kp)
+                                                              int nmod, int8_t *__restrict__ P, int64_t plane, int Kp,
+                                                              int nbx, int ntiles)
 {
-    oz2_residue_body<NP>(blockIdx.x, blockIdx.y, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
+    oz2_residue_tiles<NP>(blockIdx.x, gridDim.x, ntiles, nbx, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
 }
 
 // Pieces of a' = rint(x 2^e), |a'| <= 2^(T-1): 3 for T <= 63, 4 for T <= 84.
-constexpr int crt_pieces(int nmod) { return kCrtT[nmod] <= 63 ? 3 : (kCrtT[nmod] <= 84 ? 4 : 5); }
+constexpr int crt_pieces(int nmod) { return kCrtT[nmod] <= 69 ? 2 : 3; }
 
 // Residue planes are stored in the MFMA's own order: a plane is Mp x Np
 // int8 in 16 x 16 blocks of 256 contiguous bytes, block (rb, cb) at
@@ -1244,7 +1257,7 @@ struct PrepJob {
     int Kp;
 };
 
-template <int KIND, int NP = 5>
+template <int KIND, int NP = 3>
 __global__ __launch_bounds__(256) void ozaki2_prep_grouped_kernel(const PrepJob *__restrict__ jobs,
                                                                    const int *__restrict__ blk0, int njobs,
                                                                    int nmod)
@@ -1262,8 +1275,8 @@ __global__ __launch_bounds__(256) void ozaki2_prep_grouped_kernel(const PrepJob 
     } else if (KIND == 1) {
         oz2_norm_body(local % J.rbx, local / J.rbx, J.rby, J.X, J.lines, J.K, J.sl, J.sk, J.mx, J.S);
     } else {
-        oz2_residue_body<NP>(local % J.sbx, local / J.sbx, J.X, J.lines, J.K, J.sl, J.sk, J.ex, nmod, J.P,
-                             J.plane, J.Kp);
+        oz2_residue_tiles<NP>(local, 1 << 30, J.sbx * J.sby, J.sbx, J.X, J.lines, J.K, J.sl, J.sk, J.ex, nmod,
+                              J.P, J.plane, J.Kp);
     }
 }
 
@@ -1812,6 +1825,27 @@ int ozaki_core(int ta, int tb, int m, int n, int k, double alpha, const double *
 }
 
 // Sizes of one CRT-form product.
+// Workgroups of ozaki2_residue_kernel<NP> resident at once on the current
+// device (per-CU occupancy times CUs), queried once per device: the
+// grid-stride launch uses exactly that many, so no partial last round.
+template <int NP>
+int residue_slots()
+{
+    static std::atomic<int> cache[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    int v = cache[dev & 15].load();
+    if (v > 0) return v;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&ozaki2_residue_kernel<NP>),
+                                                      256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    v = per_cu * cus;
+    cache[dev & 15].store(v);
+    return v;
+}
+
 struct Oz2Plan {
     int Mp, Np, Kp, nchunks;
     size_t planesA, planesB, exps, words, res, total;
@@ -1880,15 +1914,15 @@ int ozaki2_core(int ta, int tb, int m, int n, int k, double alpha, const double 
     }
     ozaki2_scale_kernel<<<(nl + 255) / 256, 256, 0, st>>>(mx, S, nl, kCrtT[nmod], eA);
     auto residues = [&](auto NP) {
-        ozaki2_residue_kernel<decltype(NP)::value><<<dim3(Mp / 64, Kp / 64), 256, 0, st>>>(
-            A, m, k, a_sl, a_sk, eA, nmod, PA, (int64_t) Mp * Kp, Kp);
-        ozaki2_residue_kernel<decltype(NP)::value><<<dim3(Np / 64, Kp / 64), 256, 0, st>>>(
-            B, n, k, b_sl, b_sk, eB, nmod, PB, (int64_t) Np * Kp, Kp);
+        const int slots = residue_slots<decltype(NP)::value>();
+        const int ta = (Mp / 64) * (Kp / 64), tb = (Np / 64) * (Kp / 64);
+        ozaki2_residue_kernel<decltype(NP)::value><<<std::min(ta, slots), 256, 0, st>>>(
+            A, m, k, a_sl, a_sk, eA, nmod, PA, (int64_t) Mp * Kp, Kp, Mp / 64, ta);
+        ozaki2_residue_kernel<decltype(NP)::value><<<std::min(tb, slots), 256, 0, st>>>(
+            B, n, k, b_sl, b_sk, eB, nmod, PB, (int64_t) Np * Kp, Kp, Np / 64, tb);
     };
-    const int np = crt_pieces(nmod);
-    if (np == 3) residues(std::integral_constant<int, 3>{});
-    else if (np == 4) residues(std::integral_constant<int, 4>{});
-    else residues(std::integral_constant<int, 5>{});
+    if (crt_pieces(nmod) == 2) residues(std::integral_constant<int, 2>{});
+    else residues(std::integral_constant<int, 3>{});
     OzArgs g;
     memset(&g, 0, sizeof(g));
     g.M = m; g.N = n; g.Mp = Mp; g.Np = Np; g.Kp = Kp;
@@ -1974,13 +2008,10 @@ int oz_sequence(OzPlan *p, const OzTuning &tn, bool grouped, hipStream_t st)
     ozaki2_scale_kernel<<<(p->nlines + 255) / 256, 256, 0, st>>>(p->words, p->words + p->nlines, p->nlines,
                                                                   kCrtT[p->nmod], p->e);
     if (p->sblocks) {
-        const int np = crt_pieces(p->nmod);
-        if (np == 3)
-            ozaki2_prep_grouped_kernel<2, 3><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
-        else if (np == 4)
-            ozaki2_prep_grouped_kernel<2, 4><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+        if (crt_pieces(p->nmod) == 2)
+            ozaki2_prep_grouped_kernel<2, 2><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
         else
-            ozaki2_prep_grouped_kernel<2, 5><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
+            ozaki2_prep_grouped_kernel<2, 3><<<p->sblocks, 256, 0, st>>>(p->d_jobs, p->d_blk_s, p->njobs, p->nmod);
     }
     if (grouped) {
         OzArgs g0;
