@@ -773,7 +773,7 @@ constexpr int kCrtC = (kCrtMax + 1) / 2;
 
 struct CrtTables {
     double pd[kCrtMax], invpd[kCrtMax];              // modulus, 1/p (rounded)
-    double c27[kCrtMax], c32[kCrtMax], c64[kCrtMax]; // 2^27, 2^32, 2^64 mod p (balanced)
+    double c42[kCrtMax], c32[kCrtMax], c64[kCrtMax]; // 2^42, 2^32, 2^64 mod p (balanced)
     // Garner over the pairs: with M_i = Q_0 ... Q_i-1, g_i = M_i^-1 mod Q_i
     // and the idempotents E_a = 1 (mod p_2i), 0 (mod p_2i+1) and E_b the
     // other way round, digit i is V_i = (r_2i ga_i + r_2i+1 gb_i +
@@ -796,7 +796,7 @@ constexpr CrtTables crt_tables()
         const int p = kCrtModuli[t];
         c.pd[t] = (double) p;
         c.invpd[t] = 1.0 / (double) p;
-        c.c27[t] = crt_pow2(27, p);
+        c.c42[t] = crt_pow2(42, p);
         c.c32[t] = crt_pow2(32, p);
         c.c64[t] = crt_pow2(64, p);
     }
@@ -948,12 +948,12 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
 }
 
 // Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| <= 2^T)
-// is cut exactly into FP64 pieces, NP = 2: a' = h 2^27 + l (T <= 69, |h| <=
-// 2^42, |l| < 2^27); NP = 3: a' = h2 2^64 + h1 2^32 + l (|h1|, |l| < 2^32).
-// Then s = h c27 + l (or h2 c64 + h1 c32 + l) = a' (mod p) with c = 2^k mod
-// p balanced: |s| < 2^49, exact in FP64, and its balanced remainder comes
-// from one rounded quotient (|s / p| < 2^42, error < 2^-10 < 1/(2p)).  For
-// p_0 = 256 the residue is the low byte of l.  Plane t:
+// is cut exactly into FP64 pieces, NP = 2: a' = h 2^42 + l (T <= 84, |h|,
+// |l| <= 2^42); NP = 3: a' = h2 2^64 + h1 2^32 + l (|h1|, |l| < 2^32).
+// Then s = h c42 + l (or h2 c64 + h1 c32 + l) = a' (mod p) with c = 2^k mod
+// p balanced: |s| < 2^50, exact in FP64, and its balanced remainder comes
+// from one rounded quotient (|s / p| < 2^43, error < 2^-9 < 1/(2p)).  For
+// p_0 = 256 the residue is a' mod 256 = l - 256 floor(l / 256).  Plane t:
 // P[t * plane + line * Kp + kk].
 template <int NP>
 __device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles, int nbx,
@@ -1007,22 +1007,22 @@ __device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles
                 pm[j] = trunc(a * 0x1p-32);
                 pl[j] = fma(-pm[j], 0x1p32, a);
             } else {
-                pm[j] = trunc(a * 0x1p-27);
-                pl[j] = fma(-pm[j], 0x1p27, a);
+                pm[j] = trunc(a * 0x1p-42);
+                pl[j] = fma(-pm[j], 0x1p42, a);
             }
         }
         int8_t *base = P + (int64_t) l * Kp + k0 + kq;
         {
             u32x4 w;                                    // p_0 = 256: the low byte of l
-            int lo[16];                                 // |l| < 2^27 (NP = 2) fits INT32; l < 2^32 keeps its low word
+            int lo[16];                                 // a' mod 256 in [0, 256), exact
 #pragma unroll
-            for (int j = 0; j < 16; ++j) lo[j] = NP == 2 ? (int) pl[j] : (int) (int64_t) pl[j];
+            for (int j = 0; j < 16; ++j) lo[j] = (int) fma(-floor(pl[j] * 0x1p-8), 256.0, pl[j]);
 #pragma unroll
             for (int q = 0; q < 4; ++q) w[q] = crt_pack4(lo[4 * q], lo[4 * q + 1], lo[4 * q + 2], lo[4 * q + 3]);
             __builtin_nontemporal_store(w, reinterpret_cast<u32x4 *>(base));
         }
         for (int m = 1; m < nmod; ++m) {
-            const double ch = c_crt.c64[m], cm = NP == 3 ? c_crt.c32[m] : c_crt.c27[m];
+            const double ch = c_crt.c64[m], cm = NP == 3 ? c_crt.c32[m] : c_crt.c42[m];
             const double p = c_crt.pd[m], ip = c_crt.invpd[m];
             u32x4 w;
 #pragma unroll
@@ -1054,7 +1054,7 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
 }
 
 // Pieces of a' = rint(x 2^e), |a'| <= 2^(T-1): 3 for T <= 63, 4 for T <= 84.
-constexpr int crt_pieces(int nmod) { return kCrtT[nmod] <= 69 ? 2 : 3; }
+constexpr int crt_pieces(int nmod) { return kCrtT[nmod] <= 84 ? 2 : 3; }
 
 // Residue planes are stored in the MFMA's own order: a plane is Mp x Np
 // int8 in 16 x 16 blocks of 256 contiguous bytes, block (rb, cb) at
