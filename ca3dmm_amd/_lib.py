@@ -70,6 +70,8 @@ SIGNATURES = {
     "hip_ozaki_workspace_bytes": (SZ, [I, I, I, I]),
     "hip_ozaki2_dgemm": (I, [I, I, I, I, I, D, P, I, P, I, D, P, I, I]),
     "hip_ozaki2_workspace_bytes": (SZ, [I, I, I, I]),
+    "hip_ozaki_choose": (I, [I, I, I, I, I, P, I, P, I, IP, IP]),
+    "hip_ozaki_counts": (I, [D, D, I, IP, IP]),
     "hip_dsyrk_gram": (I, [I, I, D, P, I, D, P, I, I]),
     "hip_ozaki_gram": (I, [I, I, D, P, I, D, P, I, I, I]),
     "hip_dtrtri_upper": (I, [I, P, I, P, I]),

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -36,6 +37,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "ca3dmm_ext.h"
 #include "grouped.h"
@@ -1051,6 +1053,50 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
                                                               int nbx, int ntiles)
 {
     oz2_residue_tiles<NP>(blockIdx.x, gridDim.x, ntiles, nbx, X, lines, K, sl, sk, ex, nmod, P, plane, Kp);
+}
+
+// Spread statistics for the slice / modulus count chooser: for sampled
+// entry s of C = op(A) op(B), i_s = (s 2654435761 + 2135587861) mod 2^32 mod
+// m and j_s = (s 2246822519 + 3266489917) mod 2^32 mod n, with a = |row i_s
+// of op(A)| and b = |column j_s of op(B)|: sum a b, sum a^2, sum b^2,
+// sum a, sum b, max a, max b (one workgroup per sample; an estimate, so the
+// summation order is free).
+constexpr int kSpreadSamples = 4096;
+__global__ __launch_bounds__(256) void ozaki_spread_kernel(const double *__restrict__ A, int64_t a_sl, int64_t a_sk,
+                                                            int m, const double *__restrict__ B, int64_t b_sl,
+                                                            int64_t b_sk, int n, int k, double *__restrict__ out)
+{
+    const int s = blockIdx.x;
+    const uint64_t i = (((uint64_t) s * 2654435761ull + 2135587861ull) & 0xffffffffull) % (uint64_t) m;
+    const uint64_t j = (((uint64_t) s * 2246822519ull + 3266489917ull) & 0xffffffffull) % (uint64_t) n;
+    const double *a = A + (int64_t) i * a_sl;
+    const double *b = B + (int64_t) j * b_sl;
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int l = threadIdx.x; l < k; l += 256) {
+        const double x = fabs(a[(int64_t) l * a_sk]), y = fabs(b[(int64_t) l * b_sk]);
+        v[0] = fma(x, y, v[0]);
+        v[1] = fma(x, x, v[1]);
+        v[2] = fma(y, y, v[2]);
+        v[3] += x;
+        v[4] += y;
+        v[5] = fmax(v[5], x);
+        v[6] = fmax(v[6], y);
+    }
+    __shared__ double red[7][256];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int) threadIdx.x < w) {
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                const double o = red[q][threadIdx.x + w];
+                red[q][threadIdx.x] = q >= 5 ? fmax(red[q][threadIdx.x], o) : red[q][threadIdx.x] + o;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) out[(int64_t) s * 7 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Pieces of a' = rint(x 2^e), |a'| <= 2^(T-1): 3 for T <= 63, 4 for T <= 84.
@@ -2354,6 +2400,59 @@ int hip_ozaki2_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int
         if (!B || ldB < std::max(1, tb ? n : k)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki2: bad B / ldB");
     }
     return ozaki2_core(ta, tb, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, nmoduli);
+}
+
+int hip_ozaki_counts(double L, double Ls, int k, int *nslices, int *nmoduli)
+{
+    if (std::isnan(L)) L = INFINITY;
+    if (std::isnan(Ls)) Ls = INFINITY;
+    int N = kCrtMax;
+    for (int c = kCrtMin; c <= kCrtMax; ++c)
+        if (kCrtT[c] >= L + 54.0) {
+            N = c;
+            break;
+        }
+    const double off = 54.0 + (k > 1024 ? 2.33 * std::log2(k / 1024.0) : 0.0);
+    int sl = kMaxSlices;
+    for (int c = 1; c <= kMaxSlices; ++c)
+        if (8.0 * c >= Ls + off) {
+            sl = c;
+            break;
+        }
+    if (nslices) *nslices = sl;
+    if (nmoduli) *nmoduli = N;
+    return HIP_PROXY_OK;
+}
+
+int hip_ozaki_choose(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int m, const int n, const int k,
+                     const double *A, const int ldA, const double *B, const int ldB, int *nslices, int *nmoduli)
+{
+    const int ta = (transA != HipNoTrans), tb = (transB != HipNoTrans);
+    if (m < 0 || n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki_choose: negative size");
+    if (m == 0 || n == 0 || k == 0) return hip_ozaki_counts(0.0, 0.0, k, nslices, nmoduli);
+    if (!A || ldA < std::max(1, ta ? k : m)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki_choose: bad A / ldA");
+    if (!B || ldB < std::max(1, tb ? n : k)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "ozaki_choose: bad B / ldB");
+    int rc = 0;
+    double *d = (double *) workspace((size_t) kSpreadSamples * 7 * sizeof(double), &rc);
+    if (rc) return rc;
+    const int64_t a_sl = ta ? ldA : 1, a_sk = ta ? 1 : ldA;
+    const int64_t b_sl = tb ? 1 : ldB, b_sk = tb ? ldB : 1;
+    hipStream_t st = ca3::stream();
+    ozaki_spread_kernel<<<kSpreadSamples, 256, 0, st>>>(A, a_sl, a_sk, m, B, b_sl, b_sk, n, k, d);
+    std::vector<double> h((size_t) kSpreadSamples * 7);
+    hipError_t e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return ca3::fail(e, "ozaki_choose");
+    double L = -INFINITY, Ls = -INFINITY;
+    for (int q = 0; q < kSpreadSamples; ++q) {
+        const double *v = &h[(size_t) q * 7];
+        if (!(v[1] > 0.0 && v[2] > 0.0)) continue;         // an empty line: nothing to lose
+        const double lg = std::log2(v[0]);                  // -inf for disjoint supports
+        L = std::max(L, 0.5 * std::log2(v[1]) + 0.5 * std::log2(v[2]) - lg);
+        Ls = std::max(Ls, std::log2(v[5] * v[4] + v[6] * v[3]) - lg);
+    }
+    if (L == -INFINITY) L = Ls = 0.0;
+    return hip_ozaki_counts(L, Ls, k, nslices, nmoduli);
 }
 
 size_t hip_ozaki_workspace_bytes(int m, int n, int k, int nslices)

@@ -130,6 +130,22 @@ def ozaki2_dgemm(transA, transB, m, n, k, alpha, A, B, beta, Cm, nmoduli, lda=No
                  "hip_ozaki2_dgemm")
 
 
+def ozaki_choose(transA, transB, m, n, k, A, B, lda=None, ldb=None):
+    """(slices, moduli) for hip_ozaki_dgemm / hip_ozaki2_dgemm chosen from
+    the operands' spread on 4096 sampled entries (synchronous)."""
+    s, N = C.c_int(0), C.c_int(0)
+    check(lib.hip_ozaki_choose(transA, transB, m, n, k, A.ptr, lda or _ld(A), B.ptr, ldb or _ld(B),
+                               C.byref(s), C.byref(N)), "hip_ozaki_choose")
+    return s.value, N.value
+
+
+def ozaki_counts(L, Ls, k):
+    """The count rule of hip_ozaki_choose on given spread statistics."""
+    s, N = C.c_int(0), C.c_int(0)
+    check(lib.hip_ozaki_counts(float(L), float(Ls), k, C.byref(s), C.byref(N)), "hip_ozaki_counts")
+    return s.value, N.value
+
+
 def dgemm_fold(transA, transB, m, n, k, alpha, A, B, beta, Cm, parts):
     arr = (C.c_void_p * len(parts))(*[p.ptr for p in parts])
     lds = (C.c_int * len(parts))(*[_ld(p) for p in parts])

@@ -84,6 +84,29 @@ int hip_ozaki2_dgemm(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
 /* Device workspace the library keeps for hip_ozaki2_dgemm at this size. */
 size_t hip_ozaki2_workspace_bytes(int m, int n, int k, int nmoduli);
 
+/* Slice count for hip_ozaki_dgemm and modulus count for hip_ozaki2_dgemm,
+ * chosen per problem from the operands' spread (oracle: ozaki.spread and
+ * ozaki.counts_from_spread).  4096 deterministic entries (i, j) of C are
+ * sampled; with a = |row i of op(A)| and b = |column j of op(B)|,
+ *   L   = max log2(||a||_2 ||b||_2 / a.b)                        (CRT form),
+ *   L_s = max log2((max(a) ||b||_1 + max(b) ||a||_1) / a.b)      (slice form),
+ * and the counts are the smallest with T(nmoduli) >= L + 54 and
+ * 8 nslices >= L_s + 54 + 2.33 log2(k / 1024) (the log term for k > 1024),
+ * clamped to 25 and 16.  The offsets aim at errors no worse than
+ * hip_dgemm's and were calibrated on uniform[-0.5, 0.5] exp(phi N(0,1))
+ * inputs (phi 0.5, 2, 4; 8192^3 and 65536 x 65536 x 1024): an estimate,
+ * not a bound.  Synchronous (reads the sample statistics back).  k = 0 or
+ * an empty C gives the counts of L = L_s = 0.  No reference counterpart. */
+int hip_ozaki_choose(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
+                     const int m, const int n, const int k,
+                     const double *A, const int ldA, const double *B, const int ldB,
+                     int *nslices, int *nmoduli);
+
+/* The count rule of hip_ozaki_choose on given spread statistics (host
+ * only, no device work): nslices from L_s and k, nmoduli from L; NaN counts
+ * as +inf. */
+int hip_ozaki_counts(double L, double Ls, int k, int *nslices, int *nmoduli);
+
 /* hip_dsyrk_gram (below) through the Ozaki scheme: C := alpha * A^T A +
  * beta * C for A k x n (column-major), upper triangle only, mirror != 0
  * copies it to the lower triangle.  A is sliced once and serves as both

@@ -374,3 +374,74 @@ def ozaki2_dgemm(opA, opB, N, alpha=1.0, beta=0.0, Cin=None):
     if beta != 0.0:
         C = C + beta * Cin
     return C
+
+
+# ---------------------------------------------------------------------------
+# Slice and modulus counts from the operands' spread (no reference
+# counterpart; the library's hip_ozaki_choose and hip_ozaki_counts).
+#
+# S = 4096 deterministic entries (i_s, j_s) of C are sampled.  For each, with
+# a = row i_s of op(A) and b = column j_s of op(B):
+#   c   = sum |a_l| |b_l| / (||a||_2 ||b||_2)                 (CRT form)
+#   c_s = sum |a_l| |b_l| / (max|a| ||b||_1 + max|b| ||a||_1)  (slice form)
+# The CRT form's rounding error on an entry scales like 2^-T ||a|| ||b||
+# against the native product's u |a|.|b|, so it needs T to grow with
+# L = log2 max 1/c; the slice form keeps 8s bits relative to the line
+# maxima, so it needs 8s to grow with L_s = log2 max 1/c_s.  The offsets
+# are calibrated on uniform[-0.5, 0.5] exp(phi N(0, 1)) inputs, phi in
+# {0.5, 2, 4}, at 8192^3 and 65536 x 65536 x 1024 (two seeds each) against
+# the modulus and slice counts whose errors first matched the FP64 MFMA
+# DGEMM's on the device (4096 sampled entries, double-double reference):
+#   N = the smallest modulus count with T(N) >= L + 54   (at most 25),
+#   s = the smallest slice count with 8 s >= L_s + 54 + 2.33 log2(k / 1024)
+#       (log2 term for k > 1024 only; at most 16).
+# On those inputs the rule never asks for fewer moduli or slices than
+# measured, and at most two more moduli (phi = 2 needs less than its spread
+# suggests) and one more slice (the slice offset keeps a margin because
+# cancellation, which the statistic does not see, sets the relative error).  An
+# estimate, not a bound: entries outside the sample can be worse.
+# ---------------------------------------------------------------------------
+SPREAD_SAMPLES = 4096
+
+
+def spread_samples(m, n, S=SPREAD_SAMPLES):
+    """Sampled entries: i_s = (s 2654435761 + 2135587861) mod 2^32 mod m,
+    j_s = (s 2246822519 + 3266489917) mod 2^32 mod n."""
+    s = np.arange(S, dtype=np.uint64)
+    i = ((s * np.uint64(2654435761) + np.uint64(2135587861)) % np.uint64(2 ** 32)) % np.uint64(m)
+    j = ((s * np.uint64(2246822519) + np.uint64(3266489917)) % np.uint64(2 ** 32)) % np.uint64(n)
+    return i.astype(np.int64), j.astype(np.int64)
+
+
+def spread(opA, opB, S=SPREAD_SAMPLES):
+    """(L, L_s) of the sampled entries; +inf when a sampled entry has
+    |a|.|b| = 0 with a and b both nonzero; 0 when every sample is empty."""
+    m, n = opA.shape[0], opB.shape[1]
+    i, j = spread_samples(m, n, S)
+    a, b = np.abs(opA[i]), np.abs(opB[:, j].T)
+    dot = np.sum(a * b, axis=1)
+    na, nb = np.sqrt(np.sum(a * a, axis=1)), np.sqrt(np.sum(b * b, axis=1))
+    den_s = a.max(axis=1, initial=0.0) * b.sum(axis=1) + b.max(axis=1, initial=0.0) * a.sum(axis=1)
+    live = (na > 0) & (nb > 0)
+    if not np.any(live):
+        return 0.0, 0.0
+    with np.errstate(divide="ignore"):
+        L = np.max(np.where(live, np.log2(na * nb) - np.log2(dot), -np.inf))
+        Ls = np.max(np.where(live, np.log2(den_s) - np.log2(dot), -np.inf))
+    return float(L), float(Ls)
+
+
+def counts_from_spread(L, Ls, k):
+    """(slices, moduli) for the spread statistics, as hip_ozaki_counts."""
+    N = CRT_MAX
+    for cand in range(CRT_MIN, CRT_MAX + 1):
+        if crt_T(cand) >= L + 54.0:
+            N = cand
+            break
+    off = 54.0 + (2.33 * np.log2(k / 1024.0) if k > 1024 else 0.0)
+    s = 16
+    for cand in range(1, 17):
+        if 8 * cand >= Ls + off:
+            s = cand
+            break
+    return s, N

@@ -1,0 +1,91 @@
+"""CPU: the slice / modulus count chooser.  The spread statistics of the
+oracle (oracle/ozaki.py, spread) are pinned by inputs whose value is known
+in closed form; the library's count rule (hip_ozaki_counts, host code, no
+device work) equals the oracle's on a grid of statistics; the rule maps the
+calibration runs' statistics to the counts the device measured."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import ozaki as Z
+
+
+def _lib_counts(L, Ls, k):
+    from ca3dmm_amd import LIB_PATH
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.hip_ozaki_counts.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    s, N = ctypes.c_int(0), ctypes.c_int(0)
+    assert lib.hip_ozaki_counts(L, Ls, k, ctypes.byref(s), ctypes.byref(N)) == 0
+    return s.value, N.value
+
+
+def test_sample_indices_follow_the_formula():
+    i, j = Z.spread_samples(1000, 777)
+    for s in (0, 1, 2, 4095):
+        assert i[s] == ((s * 2654435761 + 2135587861) % 2 ** 32) % 1000
+        assert j[s] == ((s * 2246822519 + 3266489917) % 2 ** 32) % 777
+    assert len(i) == Z.SPREAD_SAMPLES == 4096
+
+
+def test_spread_closed_forms():
+    m, k, n = 40, 64, 30
+    # all-ones: a.b = k = ||a|| ||b||, (max(a)|b|_1 + max(b)|a|_1) / a.b = 2
+    L, Ls = Z.spread(np.ones((m, k)), -np.ones((k, n)))
+    assert L == pytest.approx(0.0, abs=1e-12) and Ls == pytest.approx(1.0, abs=1e-12)
+    # one nonzero per line on the same k: L = 0, L_s = 1 whatever the values
+    A = np.zeros((m, k))
+    B = np.zeros((k, n))
+    A[:, 5] = np.arange(1, m + 1) * 2.0 ** -30
+    B[5, :] = -np.arange(1, n + 1) * 2.0 ** 40
+    L, Ls = Z.spread(A, B)
+    assert L == pytest.approx(0.0, abs=1e-12) and Ls == pytest.approx(1.0, abs=1e-12)
+    # disjoint supports: every sampled a.b is 0 while a and b are not
+    A = np.zeros((m, k))
+    B = np.zeros((k, n))
+    A[:, :32] = 1.0
+    B[32:, :] = 1.0
+    assert Z.spread(A, B) == (np.inf, np.inf)
+    # empty operands carry no spread
+    assert Z.spread(np.zeros((m, k)), np.ones((k, n))) == (0.0, 0.0)
+
+
+def test_spread_scale_invariant_and_grows_with_phi():
+    rnd = np.random.default_rng(5)
+    A, B = Z.phi_matrix(rnd, (64, 512), 2.0), Z.phi_matrix(rnd, (512, 48), 2.0)
+    L, Ls = Z.spread(A, B)
+    L2, Ls2 = Z.spread(A * 2.0 ** 100, B * 2.0 ** -70)
+    assert L2 == pytest.approx(L, abs=1e-9) and Ls2 == pytest.approx(Ls, abs=1e-9)
+    lows = [Z.spread(Z.phi_matrix(rnd, (64, 512), p), Z.phi_matrix(rnd, (512, 48), p))[0] for p in (0.5, 2.0, 4.0)]
+    assert lows[0] < 2.0 < lows[1] < lows[2]
+
+
+def test_library_rule_equals_oracle_rule():
+    rnd = np.random.default_rng(11)
+    Ls_vals = list(rnd.uniform(-3, 90, 60)) + [0.0, np.inf, np.nan]
+    L_vals = list(rnd.uniform(-3, 60, 60)) + [0.0, np.inf, np.nan]
+    for k in (0, 1, 700, 1024, 1025, 4096, 8192, 65536, 1 << 23):
+        for L, Ls in zip(L_vals, Ls_vals):
+            want = Z.counts_from_spread(np.inf if np.isnan(L) else L, np.inf if np.isnan(Ls) else Ls, k)
+            assert _lib_counts(L, Ls, k) == want, (L, Ls, k)
+
+
+# Calibration runs: (k, phi, L, L_s) of 8192^3 and 65536 x 65536 x 1024
+# inputs (two seeds), and the modulus and slice counts whose sampled errors
+# first matched the FP64 MFMA DGEMM's on the device.
+CALIBRATION = [
+    (8192, 0.5, 0.84, 5.13, 15, 8), (8192, 0.5, 0.83, 5.16, 15, 8),
+    (8192, 2.0, 9.56, 13.58, 16, 9), (8192, 2.0, 9.97, 13.67, 16, 9),
+    (8192, 4.0, 23.70, 25.18, 21, 11), (8192, 4.0, 23.50, 24.55, 21, 11),
+    (1024, 0.5, 0.97, 4.84, 15, 7), (1024, 0.5, 0.96, 4.78, 15, 7),
+    (1024, 2.0, 11.35, 13.30, 16, 8), (1024, 2.0, 9.91, 12.75, 16, 8),
+    (1024, 4.0, 25.98, 27.04, 20, 10), (1024, 4.0, 22.01, 23.08, 20, 10),
+]
+
+
+@pytest.mark.parametrize("k,phi,L,Ls,N_meas,s_meas", CALIBRATION)
+def test_rule_reproduces_the_calibration(k, phi, L, Ls, N_meas, s_meas):
+    s, N = Z.counts_from_spread(L, Ls, k)
+    assert N_meas <= N <= N_meas + 2          # never fewer moduli than measured
+    assert s_meas <= s <= s_meas + 1
