@@ -50,7 +50,8 @@ def ozaki_at_equal_error(hip, n, rnd):
     at n^3: for each form, the smallest slice count (slices) or modulus count
     (CRT form) whose errors -- max |C - C_dd| / (|A||B|) and max relative
     error, on 4096 sampled entries against double-double -- are both no worse
-    than the native product's; `best` is the faster of the two."""
+    than the native product's; `best` is the faster of the two.  `chooser`
+    is the CRT form at the count hip_ozaki_choose picks without the search."""
     import numpy as np
 
     from oracle import oracle as O
@@ -102,11 +103,18 @@ def ozaki_at_equal_error(hip, n, rnd):
                     lambda s: s * (s + 1) // 2)
     crt = search("moduli", range(13, 26), lambda N: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, N),
                  lambda N: N)
+    # the counts hip_ozaki_choose picks from the operands' spread, timed the same way
+    ch_s, ch_N = hip.ozaki_choose(0, 0, n, n, n, dA, dB)
+    ms = timed(lambda: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, ch_N))
+    e_ab, e_rel = errs()
+    chooser = {"slices": ch_s, "moduli": ch_N, "crt_ms": round(ms, 3), "crt_max_err_over_absAabsB": e_ab,
+               "crt_max_rel_err": e_rel, "no_worse_than_native": bool(e_ab <= nat_ab and e_rel <= nat_rel),
+               "speedup_vs_native": round(nat_ms / ms, 3)}
     hip.set_sync(True)
     out = {"inputs": "uniform[-0.5,0.5]*exp(0.5*N(0,1)), phi=0.5", "m": n, "n": n, "k": n, "samples": 4096,
            "oracle": "double-double dot products", "native_ms": round(nat_ms, 3),
            "native_tflops": round(flops / nat_ms / 1e9, 3), "native_max_err_over_absAabsB": nat_ab,
-           "native_max_rel_err": nat_rel, "slices": slices, "crt": crt}
+           "native_max_rel_err": nat_rel, "slices": slices, "crt": crt, "chooser": chooser}
     cands = [(f["at_equal_error"]["ms"], name, f["at_equal_error"])
              for name, f in (("ozaki-int8-slices", slices), ("ozaki-int8-crt", crt)) if f["at_equal_error"]]
     if cands:

@@ -1982,7 +1982,13 @@ int ozaki2_core(int ta, int tb, int m, int n, int k, double alpha, const double 
     g.kcb = kCrtChunk / kKb;
     g.R8 = R;
     g.rplane = (int64_t) Mp * Np;
-    ozaki_pipe_kernel<false, 128, 64, true, 8, true><<<(Mp / 256) * (Np / 256), 512, 0, st>>>(g);
+    // CA3DMM_OZAKI_WAVES=4: 256 x 128 tiles of 4 waves instead of 256 x 256 of 8
+    static const bool waves4 = [] {
+        const char *w = getenv("CA3DMM_OZAKI_WAVES");
+        return w && strcmp(w, "4") == 0;
+    }();
+    if (waves4) ozaki_pipe_kernel<false, 128, 64, true, 4, true><<<(Mp / 256) * (Np / 128), 256, 0, st>>>(g);
+    else ozaki_pipe_kernel<false, 128, 64, true, 8, true><<<(Mp / 256) * (Np / 256), 512, 0, st>>>(g);
     const dim3 grid((unsigned) ((((int64_t) (m + 15) / 16) * ((n + 15) / 16) + 3) / 4));
     launch_crt_n(nmod, std::make_integer_sequence<int, kCrtMax - kCrtMin + 1>{}, grid, st, R, g.rplane, Mp, m, n,
                  eA, eB, alpha, beta, C, ldC);
