@@ -143,14 +143,16 @@ struct OperandLoader {
     }
 };
 
-// 16 k-steps of MFMAs on the LDS stage (4 x v_mfma_f64_16x16x4 k-slices).
+// k-slices [KS0, KS1) of the 16 k-steps of MFMAs on the LDS stage (4 x
+// v_mfma_f64_16x16x4 k-slices in all).
+template <int KS0 = 0, int KS1 = kBK / 4>
 __device__ __forceinline__ void stage_mfma(const double *As, const double *Bs, f64x4 (&acc)[4][4])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int li = lane & 15;
 #pragma unroll
-    for (int ks = 0; ks < kBK / 4; ++ks) {
+    for (int ks = KS0; ks < KS1; ++ks) {
         const int kk = ks * 4 + (lane >> 4);
         double a[4], b[4];
 #pragma unroll
@@ -218,9 +220,14 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, boo
             const int kn = kbeg + min(kt + 1, nfull - 1) * kBK;
             la.load_full(kn);
             lb.load_full(kn);
-            stage_mfma(As[cur], Bs[cur], acc);
+            // the next stage goes to the other buffer halfway through this
+            // one's MFMAs (its loads have had half a stage to land), so the
+            // LDS writes complete under the second half instead of before
+            // the barrier
+            stage_mfma<0, kBK / 8>(As[cur], Bs[cur], acc);
             la.store(As[cur ^ 1]);
             lb.store(Bs[cur ^ 1]);
+            stage_mfma<kBK / 8, kBK / 4>(As[cur], Bs[cur], acc);
             __syncthreads();
         }
     }
