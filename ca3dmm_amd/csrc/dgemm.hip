@@ -22,7 +22,7 @@
 //    stores (16 k rows, one column) and the MFMA operand reads (16
 //    consecutive m or n, two k rows per half wave) hit distinct banks.
 //  * Tiles are visited XCD-contiguously (workgroup b runs on XCD b % 8) and
-//    in groups of 8 tile-rows, so the tiles resident on one XCD share A and
+//    in groups of 16 tile-rows, so the tiles resident on one XCD share A and
 //    B panels in that XCD's L2.
 #include <algorithm>
 #include <cstring>

@@ -44,13 +44,13 @@ __device__ __forceinline__ int swz(int kk, int x)
 }
 
 // Linear workgroup id -> (tile row, tile col): XCD-contiguous ranges, then
-// groups of 8 tile-rows walked column by column.
+// groups of 16 tile-rows walked column by column.
 __device__ __forceinline__ void tile_coords(int bid, int nb, int M, int N, int *tm, int *tn)
 {
     const int tiles_m = (M + kBM - 1) / kBM, tiles_n = (N + kBN - 1) / kBN;
     (void) tiles_n;
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    constexpr int G = 8;
+    constexpr int G = 16;
     const int per = G * tiles_n;
     const int grp = bid / per;
     const int first = grp * G;
