@@ -930,8 +930,15 @@ __global__ __launch_bounds__(256) void ozaki2_norm_kernel(const double *__restri
     oz2_norm_body(blockIdx.x, blockIdx.y, gridDim.y, X, lines, K, sl, sk, mx, S);
 }
 
-// e = T + 19 - E - ceil(bitlen(S - 1) / 2): the largest power with
-// 2^e ||line||_2 <= 2^(T-1); kExpEmpty for a zero line.
+// e = T + 20 - E - h with h the smallest integer >= 0 such that
+// S <= 4^h - ceil(2897 2^(2h - T)): then 4^(T-h) S <= (2^T - 2897/2)^2, so
+// 2^e ||line||_2 plus the rounding slack sqrt(k)/2 <= 2897/2 (k <= 2^23)
+// stays within 2^T.  The search starts at ceil(bitlen(S - 1) / 2) - 1,
+// below which 4^h < S.  kExpEmpty for a zero line.
+constexpr int kCrtSlack = 2897;     // ceil(sqrt(kCrtKMax))
+static_assert((long long) kCrtSlack * kCrtSlack >= kCrtKMax &&
+                  (long long) (kCrtSlack - 1) * (kCrtSlack - 1) < kCrtKMax,
+              "rounding slack bound");
 __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, const unsigned long long *__restrict__ S,
                                     int n, int T, int *__restrict__ e)
 {
@@ -944,9 +951,17 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
     }
     int E;
     (void) frexp(v, &E);
-    const unsigned long long s = S[l];
-    const int b = s >= 2 ? 64 - __clzll((long long) (s - 1)) : 0;
-    e[l] = T + 19 - E - (b + 1) / 2;
+    typedef unsigned __int128 u128;
+    const u128 s = S[l];
+    const int b = s >= 2 ? 64 - __clzll((long long) (S[l] - 1)) : 0;
+    int h = max(0, (b + 1) / 2 - 1);
+    for (;; ++h) {
+        const int sh = 2 * h - T;
+        const u128 sub = sh >= 0 ? (u128) kCrtSlack << sh
+                                 : ((u128) kCrtSlack + (((u128) 1 << -sh) - 1)) >> -sh;
+        if (s + sub <= ((u128) 1 << (2 * h))) break;
+    }
+    e[l] = T + 20 - E - h;
 }
 
 // Residue planes: a' = rint(x 2^e) (an integer-valued double, |a'| <= 2^T)

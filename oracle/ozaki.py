@@ -237,9 +237,13 @@ def diagonals8(SA, SB, s):
 # exponent of the line's max |x| and, per element, ex its exponent and
 # m4 = ceil(16 |frac|) in [8, 16], x^2 <= m4^2 4^(ex-4), so
 # ||line||^2 <= 4^E 2^-40 S with S = sum ceil(m4^2 2^(2(ex-E)+32)) (an
-# integer <= k 2^40), and e = T + 19 - E - ceil(bitlen(S - 1) / 2) is the
-# largest power with 2^e ||line|| <= 2^(T-1).  Rounding adds at most
-# sqrt(k)/2 <= 2^(T-1) to the norm (k <= 4^T), hence ||a'|| <= 2^T.
+# integer <= k 2^40).  Rounding adds at most sqrt(k)/2 <= 2897/2 to the norm
+# (k <= 2^23 <= 2897^2), so ||a'|| <= 2^T holds when
+# 2^e ||line|| <= 2^T - 2897/2.  With e = T + 20 - E - h that reads
+# 4^(T-h) S <= (2^T - 2897/2)^2, which S <= 4^h - ceil(2897 2^(2h-T))
+# implies ((1 - x)^2 >= 1 - 2x); h is the smallest such integer.  (Against
+# the plain 2^e ||line|| <= 2^(T-1), this is one more bit for almost every
+# line.)
 # ---------------------------------------------------------------------------
 CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
               223, 217, 211, 199, 197, 193, 191, 181, 179, 173,
@@ -277,9 +281,23 @@ def crt_line_exponents(X, axis, N):
     dn = np.where(m2 > 0, ((m2 - 1) >> np.clip(-sh, 0, 62)) + 1, 0)
     w = np.where(sh >= 0, up, dn)
     S = np.sum(w, axis=axis, dtype=np.uint64).astype(object)
-    b = np.array([int(v - 1).bit_length() if v >= 1 else 0 for v in S.ravel()], dtype=np.int64)
-    e = T + 19 - E - (b + 1) // 2
+    h = np.array([crt_scale_h(int(v), T) for v in S.ravel()], dtype=np.int64)
+    e = T + 20 - E - h
     return e, mx == 0
+
+
+CRT_SLACK = 2897            # ceil(sqrt(CRT_KMAX)): 2 x the largest rounding slack of a line norm
+
+
+def crt_scale_h(S, T):
+    """The smallest h >= 0 with S <= 4^h - ceil(2897 2^(2h - T)) (Python ints)."""
+    h = 0
+    while True:
+        sh = 2 * h - T
+        sub = CRT_SLACK << sh if sh >= 0 else -((-CRT_SLACK) >> (-sh))
+        if S <= 4 ** h - sub:
+            return h
+        h += 1
 
 
 def crt_scaled(X, axis, N):

@@ -29,6 +29,7 @@ def test_moduli_and_budget():
         assert 2 ** (2 * T + 1) < P                 # |C'| <= 2^2T < P/2
         assert 2 ** (2 * T + 5) > P                 # within two bits of the largest such T
         assert 4 ** T >= Z.CRT_KMAX                 # rounding slack sqrt(k)/2 <= 2^(T-1)
+        assert Z.CRT_SLACK ** 2 >= Z.CRT_KMAX > (Z.CRT_SLACK - 1) ** 2 and Z.CRT_SLACK < 2 ** T
 
 
 @pytest.mark.parametrize("N", [4, 9, 16, 20, 25])
@@ -94,9 +95,9 @@ def test_line_scale_keeps_the_norm_bound(N, phi):
         ints = [int(v) for v in Ai[i]]
         assert all(float(v) == w for v, w in zip(ints, Ai[i]))
         assert sum(v * v for v in ints) <= 4 ** T       # ||a'||_2 <= 2^T
-        # not wasteful: the scaled line reaches at least 2^(T-3) in norm
+        # not wasteful: the scaled line reaches at least 2^(T-2) in norm
         nrm = np.sqrt(np.sum(np.ldexp(X[i], int(e[i])) ** 2))
-        assert nrm > 2.0 ** (T - 3)
+        assert nrm > 2.0 ** (T - 2)
     # columns of op(B) follow the same rule as rows of op(B)^T
     Bi, eb, zb = Z.crt_scaled(X.T.copy(), 0, N)
     assert np.array_equal(eb[~zb], e[~zero]) and np.array_equal(Bi.T, Ai)
@@ -190,3 +191,18 @@ def test_error_falls_with_moduli():
     for a, b in zip(errs, errs[1:]):
         assert b < a / 100
     assert errs[-1] < 1e-14
+
+
+@pytest.mark.parametrize("T", [15, 31, 54, 58, 80, 95])
+def test_scale_rule_is_the_tightest_power_of_two(T):
+    """h is minimal, and the bound it certifies, 4^(T-h) S <= (2^T - 2897/2)^2,
+    holds exactly: e = T + 20 - E - h is the largest exponent the rule admits."""
+    rnd = np.random.default_rng(T)
+    for _ in range(3000):
+        S = max(1, int(rnd.integers(1, 2 ** 62)) >> int(rnd.integers(0, 62)))
+        h = Z.crt_scale_h(S, T)
+        assert 4 * 4 ** (T - h) * S <= (2 ** (T + 1) - Z.CRT_SLACK) ** 2
+        if h > 0:                                       # one less fails the rule
+            sh = 2 * (h - 1) - T
+            sub = Z.CRT_SLACK << sh if sh >= 0 else -((-Z.CRT_SLACK) >> (-sh))
+            assert S > 4 ** (h - 1) - sub
