@@ -77,10 +77,10 @@ def test_library_rule_equals_oracle_rule():
 CALIBRATION = [
     (8192, 0.5, 0.84, 5.13, 15, 8), (8192, 0.5, 0.83, 5.16, 15, 8),
     (8192, 2.0, 9.56, 13.58, 16, 9), (8192, 2.0, 9.97, 13.67, 16, 9),
-    (8192, 4.0, 23.70, 25.18, 21, 11), (8192, 4.0, 23.50, 24.55, 21, 11),
-    (1024, 0.5, 0.97, 4.84, 15, 7), (1024, 0.5, 0.96, 4.78, 15, 7),
-    (1024, 2.0, 11.35, 13.30, 16, 8), (1024, 2.0, 9.91, 12.75, 16, 8),
-    (1024, 4.0, 25.98, 27.04, 20, 10), (1024, 4.0, 22.01, 23.08, 20, 10),
+    (8192, 4.0, 23.70, 25.18, 20, 11), (8192, 4.0, 23.50, 24.55, 20, 11),
+    (1024, 0.5, 0.97, 4.84, 14, 7), (1024, 0.5, 0.96, 4.78, 14, 7),
+    (1024, 2.0, 11.35, 13.30, 15, 8), (1024, 2.0, 9.91, 12.75, 15, 8),
+    (1024, 4.0, 25.98, 27.04, 19, 10), (1024, 4.0, 22.01, 23.08, 19, 10),
 ]
 
 
