@@ -2428,7 +2428,7 @@ int hip_ozaki_counts(double L, double Ls, int k, int *nslices, int *nmoduli)
     if (std::isnan(L)) L = INFINITY;
     if (std::isnan(Ls)) Ls = INFINITY;
     int N = kCrtMax;
-    const double cm = 52.5 + (k > 1024 ? 0.5 * std::log2(k / 1024.0) : 0.0);
+    const double cm = 53.5 + (k > 1024 ? 0.5 * std::log2(k / 1024.0) : 0.0);
     for (int c = kCrtMin; c <= kCrtMax; ++c)
         if (kCrtT[c] >= L + cm) {
             N = c;

@@ -90,7 +90,7 @@ size_t hip_ozaki2_workspace_bytes(int m, int n, int k, int nmoduli);
  * sampled; with a = |row i of op(A)| and b = |column j of op(B)|,
  *   L   = max log2(||a||_2 ||b||_2 / a.b)                        (CRT form),
  *   L_s = max log2((max(a) ||b||_1 + max(b) ||a||_1) / a.b)      (slice form),
- * and the counts are the smallest with T(nmoduli) >= L + 52.5 +
+ * and the counts are the smallest with T(nmoduli) >= L + 53.5 +
  * 0.5 log2(k / 1024) and 8 nslices >= L_s + 54 + 2.33 log2(k / 1024) (the
  * log terms for k > 1024),
  * clamped to 25 and 16.  The offsets aim at errors no worse than

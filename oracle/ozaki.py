@@ -410,15 +410,17 @@ def ozaki2_dgemm(opA, opB, N, alpha=1.0, beta=0.0, Cin=None):
 # {0.5, 2, 4}, at 8192^3 and 65536 x 65536 x 1024 (two seeds each) against
 # the modulus and slice counts whose errors first matched the FP64 MFMA
 # DGEMM's on the device (4096 sampled entries, double-double reference):
-#   N = the smallest modulus count with T(N) >= L + 52.5 + 0.5 log2(k / 1024)
+#   N = the smallest modulus count with T(N) >= L + 53.5 + 0.5 log2(k / 1024)
 #       (log2 term for k > 1024 only; at most 25),
 #   s = the smallest slice count with 8 s >= L_s + 54 + 2.33 log2(k / 1024)
 #       (log2 term for k > 1024 only; at most 16).
 # On those inputs the rule never asks for fewer moduli or slices than
 # measured, and at most two more moduli (phi = 2 needs less than its spread
-# suggests) and one more slice (the slice offset keeps a margin because
-# cancellation, which the statistic does not see, sets the relative error).  An
-# estimate, not a bound: entries outside the sample can be worse.
+# suggests) and one more slice.  Both offsets keep a margin because
+# cancellation, which the statistic does not see, sets the relative error
+# (at 1024^3, phi = 0.5, 14 moduli reach the native |A||B|-relative error
+# but not its max relative error).  An estimate, not a bound: entries
+# outside the sample can be worse.
 # ---------------------------------------------------------------------------
 SPREAD_SAMPLES = 4096
 
@@ -453,7 +455,7 @@ def spread(opA, opB, S=SPREAD_SAMPLES):
 def counts_from_spread(L, Ls, k):
     """(slices, moduli) for the spread statistics, as hip_ozaki_counts."""
     N = CRT_MAX
-    c = 52.5 + (0.5 * np.log2(k / 1024.0) if k > 1024 else 0.0)
+    c = 53.5 + (0.5 * np.log2(k / 1024.0) if k > 1024 else 0.0)
     for cand in range(CRT_MIN, CRT_MAX + 1):
         if crt_T(cand) >= L + c:
             N = cand

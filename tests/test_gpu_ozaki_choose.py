@@ -15,7 +15,7 @@ def _near_boundary(L, Ls, k):
     """True when a rounding difference in the statistics could move a count."""
     T = [Z.crt_T(N) for N in range(Z.CRT_MAX + 1)]
     off = 54.0 + (2.33 * np.log2(k / 1024.0) if k > 1024 else 0.0)
-    c = 52.5 + (0.5 * np.log2(k / 1024.0) if k > 1024 else 0.0)
+    c = 53.5 + (0.5 * np.log2(k / 1024.0) if k > 1024 else 0.0)
     return (np.isfinite(L) and min(abs(t - (L + c)) for t in T) < 1e-6) or \
            (np.isfinite(Ls) and min(abs(8 * s - (Ls + off)) for s in range(1, 17)) < 1e-6)
 
