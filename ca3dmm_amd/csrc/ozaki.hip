@@ -845,14 +845,18 @@ static_assert(crt_tables().ga[0] == -255.0f && crt_tables().gb[0] == 256.0f && c
                   crt_tables().gb[1] == -17457.0f && crt_tables().h[1][0] == 12186.0,
               "composite Garner constants");
 
-// Balanced residue modulo an odd p (149 <= p <= 255) of an INT32 x: FP64
-// holds x exactly, |x / p| < 2^24 so the rounded quotient x (1/p) errs by
-// less than 2^-27, while x / p is at least 1/(2p) from a half-integer; rint
-// finds the nearest quotient and the remainder is exactly the balanced one.
-__device__ __forceinline__ int crt_mod_i32(int x, double p, double invp)
+// Balanced residue modulo an odd p (149 <= p <= 255) of an INT32 x, exact in
+// its low byte, in three vector instructions.  FP64 holds x exactly and
+// |x / p| < 2^24, so x (1/p) errs by less than 2^-30, while x / p is at least
+// 1/(2p) from a half-integer.  fma(x, 1/p, 1.5 2^52) therefore rounds once, to
+// 1.5 2^52 + q with q = rint(x / p), and the low word of that double is
+// q mod 2^32.  r = x - p q is the balanced residue, |r| <= 127.  The 24-bit
+// multiply-add uses q mod 2^24 (exactly q when |q| < 2^23), which moves the
+// result by a multiple of 2^24 and leaves its low byte.  negp = -p.
+__device__ __forceinline__ int crt_mod_i32(int x, int negp, double invp)
 {
-    const double y = (double) x;
-    return (int) fma(-rint(y * invp), p, y);
+    const double t = fma((double) x, invp, 0x1.8p52);
+    return x + __mul24((int) __double2loint(t), negp);
 }
 
 // Low bytes of r0 .. r3 as one word (byte q = r_q), three byte permutes.
@@ -1141,7 +1145,8 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave & 1, wn = wave >> 1;
-    const double p = c_crt.pd[t], ip = c_crt.invpd[t];
+    const int negp = -(int) c_crt.pd[t];
+    const double ip = c_crt.invpd[t];
     typedef __attribute__((address_space(1))) uint32_t gu32;
     gu32 *Rt = (gu32 *) oz_rflp(reinterpret_cast<uint32_t *>(g.R8 + (int64_t) t * g.rplane));
     const int mp16 = __builtin_amdgcn_readfirstlane(g.Mp >> 4);
@@ -1160,7 +1165,7 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int v = acc[i][j][r] + (int) (int8_t) (prev >> (8 * r));
-                    res[r] = decltype(ODD)::value ? crt_mod_i32(v, p, ip) : v;
+                    res[r] = decltype(ODD)::value ? crt_mod_i32(v, negp, ip) : v;
                 }
                 wp[lane] = crt_pack4(res[0], res[1], res[2], res[3]);
             }

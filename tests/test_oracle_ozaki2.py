@@ -206,3 +206,40 @@ def test_scale_rule_is_the_tightest_power_of_two(T):
             sh = 2 * (h - 1) - T
             sub = Z.CRT_SLACK << sh if sh >= 0 else -((-Z.CRT_SLACK) >> (-sh))
             assert S > 4 ** (h - 1) - sub
+
+
+def _fma_magic_low_word(x, p):
+    """Low 32 bits of fl(x * fl(1/p) + 1.5 2^52), FP64 round to nearest even:
+    the sum lies in [2^52, 2^53), where doubles are the integers."""
+    from fractions import Fraction
+    y = (3 << 51) + x * Fraction(1.0 / p)
+    f = y.numerator // y.denominator
+    rem = y - f
+    t = f + (1 if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and f % 2) else 0)
+    return (t - (3 << 51)) & 0xFFFFFFFF
+
+
+def _s24(v):
+    v &= 0xFFFFFF
+    return v - (1 << 24) if v & 0x800000 else v
+
+
+def test_product_epilogue_reduction_low_byte():
+    """The CRT product's epilogue reduces an INT32 accumulator x modulo an odd
+    p as x + mul24(low word of fma(x, 1/p, 1.5 2^52), -p) and keeps the low
+    byte.  That byte must be the balanced residue's for every INT32 x: checked
+    at the extremes and at the values closest to a half-integer quotient,
+    where a wrong rounding would show, and at random values."""
+    rng = np.random.default_rng(5)
+    for p in [q for q in Z.CRT_MODULI if q % 2]:
+        xs = [-(1 << 31), (1 << 31) - 1, 0, 1, -1, p // 2, -(p // 2)]
+        for c in rng.integers(-(1 << 31) // p, (1 << 31) // p, 60):
+            for dlt in ((p - 1) // 2, -(p - 1) // 2, (p + 1) // 2):
+                x = int(c) * p + dlt
+                if -(1 << 31) <= x < (1 << 31):
+                    xs.append(x)
+        xs += [int(v) for v in rng.integers(-(1 << 31), 1 << 31, 60)]
+        for x in xs:
+            q = _fma_magic_low_word(x, p)
+            got = (x + _s24(q) * _s24(-p)) & 0xFF
+            assert got == _bal(x, p) & 0xFF, (p, x)
