@@ -70,8 +70,9 @@ def test_ozaki2_k_zero_and_zero_operands(hip):
 
 
 def test_ozaki2_k_chunks(hip):
-    """k above one INT32-exact chunk (130816): residues of the chunks are
-    added modulo p in the epilogue; the result is still the exact residue."""
+    """k above one INT32-exact chunk (65280; k = 140000 is three chunks):
+    residues of the chunks are added modulo p in the epilogue; the result
+    is still the exact residue."""
     rnd = np.random.default_rng(4)
     m, n, k = 40, 70, 140000
     opA = Z.phi_matrix(rnd, (m, k), 1.0)
