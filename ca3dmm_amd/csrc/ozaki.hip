@@ -992,6 +992,7 @@ __device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles
         else { ki = t & 63; li = (t >> 6) + 4 * p; }
     };
     double v[16];
+    int exn;                                            // exponent of this thread's line in the fetched tile
     auto fetch = [&](int tau) __attribute__((always_inline)) {
         const int l0 = (tau % nbx) * 64, k0 = (tau / nbx) * 64;
 #pragma unroll
@@ -1001,6 +1002,10 @@ __device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles
             const int l = l0 + li, kk = k0 + ki;
             v[p] = (l < lines && kk < K) ? X[sl == 1 ? l + (int64_t) kk * sk : (int64_t) l * sl + kk] : 0.0;
         }
+        // loaded with the tile: a load issued after the prefetch and waited
+        // for at once would also wait for the prefetch (vmcnt retires in
+        // issue order)
+        exn = ex[min(l0 + (t >> 2), lines - 1)];
     };
     fetch(tau0);
     for (int tau = tau0; tau < ntiles; tau += step) {
@@ -1012,12 +1017,13 @@ __device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles
             tile[li][ki] = v[p];
         }
         __syncthreads();
+        const int exc = exn;
         if (tau + step < ntiles) fetch(tau + step);     // lands while this tile is reduced
         const int l0 = (tau % nbx) * 64, k0 = (tau / nbx) * 64;
         const int li = t >> 2, kq = (t & 3) * 16;
         const int l = l0 + li;
-        const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
-        const int e = zero_line ? 0 : ex[l];
+        const bool zero_line = (l >= lines) || exc == kExpEmpty;
+        const int e = zero_line ? 0 : exc;
         double ph[16], pm[16], pl[16];                  // NP = 2: (-, h, l); NP = 3: (h2, h1, l)
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
