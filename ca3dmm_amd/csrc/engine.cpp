@@ -10,6 +10,7 @@
 // the single-rank exec with identity redistribution is a single DGEMM.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -359,6 +360,18 @@ mat_redist_engine_p redist_build(int nproc, int rank, const int *all_src, const 
         if (rect_overlap(q, all_src + 4 * peer, o)) rl.insert(rl.end(), {o[0], o[1], o[2], o[3], peer});
     }
     const int ns = (int) sl.size() / 5, nr = (int) rl.size() / 5;
+    // Block sizes, displacements and totals are int fields of the public
+    // struct (mat_redist.h:7-45); a list whose running total passes INT_MAX
+    // would wrap them, so it is refused here instead.
+    int64_t stot = 0, rtot = 0;
+    for (int i = 0; i < ns; ++i) stot += (int64_t) sl[5 * i + 2] * sl[5 * i + 3];
+    for (int i = 0; i < nr; ++i) rtot += (int64_t) rl[5 * i + 2] * rl[5 * i + 3];
+    if (stot > INT_MAX || rtot > INT_MAX) {
+        ENG_ERR("mat_redist: rank %d would send %lld / receive %lld elements; counts above INT_MAX "
+                "(%d) are not supported\n", rank, (long long) stot, (long long) rtot, INT_MAX);
+        free(e);
+        return nullptr;
+    }
     e->n_proc_send = ns;
     e->n_proc_recv = nr;
     e->send_ranks = (int *) malloc(sizeof(int) * std::max(1, ns));
@@ -486,9 +499,19 @@ cannon_engine_p cannon_build(int m, int n, int k, int q, int rank2d, dev_type_t 
     calc_block_spos_size(n, q, e->rank_col, &e->B_scol, &e->B_ncol);
     calc_block_spos_size(m, q, e->rank_row, &e->C_srow, &e->C_nrow);
     calc_block_spos_size(n, q, e->rank_col, &e->C_scol, &e->C_ncol);
-    e->max_A_blk_size = (m / q + 1) * (k / q + 1);
-    e->max_B_blk_size = (k / q + 1) * (n / q + 1);
-    e->max_C_blk_size = (m / q + 1) * (n / q + 1);
+    // int fields of the public struct (cannon.h:7-50): refuse shapes whose
+    // block sizes would wrap them
+    const int64_t mA = (int64_t) (m / q + 1) * (k / q + 1), mB = (int64_t) (k / q + 1) * (n / q + 1),
+                  mC = (int64_t) (m / q + 1) * (n / q + 1);
+    if (std::max(mA, std::max(mB, mC)) > INT_MAX) {
+        ENG_ERR("cannon: a block of %lld elements exceeds INT_MAX (%d)\n",
+                (long long) std::max(mA, std::max(mB, mC)), INT_MAX);
+        free(priv);
+        return nullptr;
+    }
+    e->max_A_blk_size = (int) mA;
+    e->max_B_blk_size = (int) mB;
+    e->max_C_blk_size = (int) mC;
     // The reference's GPU default (256) is overwritten by the env macro's
     // default (160) whenever the variable is unset (cannon.c:89-91); the
     // observable default is 160 on every device type.
@@ -565,6 +588,15 @@ ca3dmm_engine_p build_rank(const Problem &P, int rank)
     e->comm_2dmm.nproc = r.active ? r.group : 0;  e->comm_2dmm.rank = r.rank2d;
     e->comm_C_rs.nproc = r.active ? P.grid.kp : 0; e->comm_C_rs.rank = r.rank_k;
     e->comm_AB_agv.nproc = r.active ? r.tmn * r.tnn : 0; e->comm_AB_agv.rank = r.tmi + r.tni;
+    // the C_2dmm block and the allgathered operand are counted in int fields
+    // (C_rs_recvcnts, AB_agv_*; ca3dmm.h:10-82)
+    if (r.active && ((int64_t) r.C_2dmm.nrow * r.C_2dmm.ncol > INT_MAX ||
+                     (int64_t) r.A_2dmm.nrow * r.A_2dmm.ncol > INT_MAX ||
+                     (int64_t) r.B_2dmm.nrow * r.B_2dmm.ncol > INT_MAX)) {
+        ENG_ERR("ca3dmm: rank %d's blocks exceed INT_MAX (%d) elements\n", rank, INT_MAX);
+        free(e);
+        return nullptr;
+    }
     const int tk = r.active ? P.grid.kp : 0;
     e->C_rs_recvcnts = (int *) malloc(sizeof(int) * std::max(1, tk));
     for (int t = 0; t < tk; ++t) {

@@ -1124,7 +1124,9 @@ __global__ __launch_bounds__(256) void ozaki_spread_kernel(const double *__restr
     if (threadIdx.x < 7) out[(int64_t) s * 7 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Pieces of a' = rint(x 2^e), |a'| <= 2^(T-1): 3 for T <= 63, 4 for T <= 84.
+// Exact FP64 pieces of a' = rint(x 2^e), |a'| <= 2^T (the residue kernel's
+// bound): two pieces (a' = h 2^42 + l) for T <= 84, three 32-bit pieces
+// for 84 < T <= 95.
 constexpr int crt_pieces(int nmod) { return kCrtT[nmod] <= 84 ? 2 : 3; }
 
 // Residue planes are stored in the MFMA's own order: a plane is Mp x Np

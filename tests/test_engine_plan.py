@@ -181,3 +181,28 @@ def test_cannon_min_kblk_env(monkeypatch):
                         allocate=False)
         assert sim.engine(0).cannon_engine.contents.gemm_cycle == w["plans"][0].cannon.gemm_cycle == want
         sim.free()
+
+
+@pytest.mark.parametrize("p,m,n,k", [(1, 46341, 46341, 64), (1, 64, 46341, 46341), (4, 100000, 100000, 100)])
+def test_counts_above_int_max_refused(p, m, n, k):
+    """Redistribution counts, Cannon block sizes and reduce-scatter counts are
+    int fields of the public structs (src/mat_redist.h:7-45, src/cannon.h:7-50,
+    src/ca3dmm.h:10-82).  A shape whose counts pass INT_MAX is refused at
+    planning time with an error instead of wrapping them (plan only, nothing
+    is allocated)."""
+    from ca3dmm_amd import HipError
+    col = lambda nr, nc: [[0, nr, nc * r // p, nc * (r + 1) // p - nc * r // p] for r in range(p)]
+    with pytest.raises(HipError):
+        E.SimGrid(p, m, n, k, 0, 0, col(m, k), col(k, n), col(m, n), allocate=False)
+
+
+def test_counts_just_below_int_max_planned():
+    """The same planner accepts a rank whose largest count stays below
+    INT_MAX: at m = n = 46339 the Cannon bound (m/q + 1)(n/q + 1) is
+    46340^2 = 2,147,395,600."""
+    m = n = 46339
+    sim = E.SimGrid(1, m, n, 64, 0, 0, [[0, m, 0, 64]], [[0, 64, 0, n]], [[0, m, 0, n]], allocate=False)
+    e = sim.engine(0)
+    assert e.C_2dmm_nrow * e.C_2dmm_ncol == m * n
+    assert e.redist_C.contents.send_cnt == m * n
+    sim.free()

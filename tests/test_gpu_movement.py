@@ -84,17 +84,18 @@ def test_fold_sum_bit_exact(hip, nsrc, nrow, ncol):
     parts = [np.asfortranarray(rnd.standard_normal((nrow, ncol)) * 10.0 ** rnd.integers(-8, 8))
              for _ in range(nsrc)]
     want = np.zeros((nrow, ncol), order="F")
-    O.fold_sum([p.ravel(order="F") for p in parts], nrow, ncol, [nrow] * nsrc,
-               want.ravel(order="F"), nrow)
-    want = want  # filled in place through the F-order view
+    wflat = want.ravel(order="F")
+    assert np.shares_memory(wflat, want)       # the oracle writes through this view
+    O.fold_sum([p.ravel(order="F") for p in parts], nrow, ncol, [nrow] * nsrc, wflat, nrow)
     dparts = [hip.DeviceArray.from_numpy(p) for p in parts]
     out = hip.DeviceArray.zeros((nrow, ncol))
     hip.fold_sum(dparts, nrow, ncol, out)
     got = out.numpy()
+    assert np.array_equal(got, want)           # the oracle's fixed-order fold
     ref = parts[0].copy()
     for t in range(1, nsrc):
         ref = ref + parts[t]
-    assert np.array_equal(got, ref)
+    assert np.array_equal(want, ref)           # which is ((P0 + P1) + P2) + ...
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32, np.int32])
