@@ -75,6 +75,7 @@ SIGNATURES = {
     "hip_dsyrk_gram": (I, [I, I, D, P, I, D, P, I, I]),
     "hip_ozaki_gram": (I, [I, I, D, P, I, D, P, I, I, I]),
     "hip_dtrtri_upper": (I, [I, P, I, P, I]),
+    "hip_cholesky_qr": (I, [I, I, P, I, P, I, I, I, IP]),
     # dev_type.h
     "is_dev_type_valid": (I, [I]),
     "dev_type_malloc": (P, [SZ, I]),

@@ -1,9 +1,10 @@
-"""CholeskyQR2 on the device, through the C-ABI.
+"""CholeskyQR2 and shifted CholeskyQR3 on the device, through the C-ABI.
 
 Follows examples/example_CholeskyQR.c (S = B^T B, R = chol(S) upper,
-B_orth = B R^{-1}, check Q^T Q ~ I), run twice (CholeskyQR2) with every
-step on the GPU: Gram (hip_dsyrk_gram, upper triangle + mirror), Cholesky
-(hip_dpotrf 'U'), Q = A R^{-1} (hip_dtrsm), R = R2 R1 (hip_dgemm).
+B_orth = B R^{-1}, check Q^T Q ~ I), with every pass on the GPU inside
+hip_cholesky_qr: Gram (hip_dsyrk_gram, or hip_ozaki_gram), optional
+diagonal shift, Cholesky (hip_dpotrf 'U'), strictly lower part zeroed on the
+device, Q = A R^{-1} (hip_dtrsm), R = R_p R (hip_dgemm).
 Norms come from Gram traces: ||X||_F^2 = trace(X^T X)."""
 import numpy as np
 
@@ -18,38 +19,24 @@ class CholeskyBreakdown(RuntimeError):
         self.step = step
 
 
-def _zero_lower(R):
-    """R (device, n x n): strictly lower part := 0 (examples/example_CholeskyQR.c:130-135)."""
-    h = R.numpy()
-    h = np.triu(h)
-    return hip.DeviceArray.from_numpy(h)
-
-
-def cholesky_qr2(dA, m, n, passes=2, gram_slices=None):
-    """dA: device m x n (column-major).  Returns (dQ, dR) with Q m x n
-    orthonormal and R n x n upper triangular, A = Q R.  gram_slices: compute
-    each Gram through the Ozaki scheme with that many INT8 slices
+def cholesky_qr(dA, m, n, variant=hip.CHOLQR2, gram_slices=None):
+    """dA: device m x n (column-major, left unchanged).  Returns (dQ, dR,
+    passes) with Q m x n orthonormal, R n x n upper triangular, A = Q R.
+    variant: hip.CHOLQR2, hip.SCHOLQR3 or hip.CHOLQR_AUTO.  gram_slices:
+    compute each Gram through the Ozaki scheme with that many INT8 slices
     (hip_ozaki_gram) instead of the FP64 MFMA kernel."""
     dQ = hip.DeviceArray((m, n))
     hip.copy_matrix_block(8, n, m, dA.ptr, m, dQ.ptr, m)
-    dR = None
-    for step in range(1, passes + 1):
-        dG = hip.DeviceArray.zeros((n, n))
-        if gram_slices:
-            hip.ozaki_gram(n, m, dQ, dG, gram_slices)
-        else:
-            hip.gram(n, m, dQ, dG)
-        info = hip.potrf_upper(n, dG)
-        if info != 0:
-            raise CholeskyBreakdown(info, step)
-        dRs = _zero_lower(dG)
-        hip.trsm_right_upper(m, n, 1.0, dRs, dQ)
-        if dR is None:
-            dR = dRs
-        else:
-            dRn = hip.DeviceArray.zeros((n, n))
-            hip.dgemm(0, 0, n, n, n, 1.0, dRs, dR, 0.0, dRn)
-            dR = dRn
+    dR = hip.DeviceArray((n, n))
+    info, passes = hip.cholesky_qr(m, n, dQ, dR, variant=variant, gram_slices=gram_slices or 0)
+    if info > 0:
+        raise CholeskyBreakdown(info, passes + 1)
+    return dQ, dR, passes
+
+
+def cholesky_qr2(dA, m, n, gram_slices=None):
+    """CholeskyQR2 (two plain passes): (dQ, dR)."""
+    dQ, dR, _ = cholesky_qr(dA, m, n, hip.CHOLQR2, gram_slices)
     return dQ, dR
 
 

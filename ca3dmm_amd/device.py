@@ -203,6 +203,17 @@ def trtri_upper(n, R, X):
 
 
 HIP_RIGHT, HIP_UPPER, HIP_NONUNIT = 4, 5, 8
+CHOLQR_AUTO, CHOLQR2, SCHOLQR3 = 0, 2, 3
+
+
+def cholesky_qr(m, n, A, R, variant=CHOLQR_AUTO, gram_slices=0, lda=None, ldr=None):
+    """hip_cholesky_qr: A (m x n) := Q in place, R (n x n) := the upper
+    triangular factor.  Returns (info, passes): info is 0 on success or
+    LAPACK's info (> 0) of a Cholesky breakdown the variant did not recover."""
+    passes = C.c_int(0)
+    info = check(lib.hip_cholesky_qr(m, n, A.ptr, lda or _ld(A), R.ptr, ldr or _ld(R), variant, gram_slices,
+                                     C.byref(passes)), "hip_cholesky_qr")
+    return info, passes.value
 
 
 def trsm_right_upper(m, n, alpha, R, B):

@@ -131,6 +131,35 @@ int hip_dsyrk_gram(const int n, const int k, const double alpha, const double *A
  * lower part of X is set to zero (examples/example_CholeskyQR.c:137). */
 int hip_dtrtri_upper(const int n, const double *R, const int ldR, double *X, const int ldX);
 
+/* CholeskyQR of a tall-skinny A (m x n column-major, m >= n, n <= 8192),
+ * every step on the device: the driver of examples/example_CholeskyQR.c:109-168
+ * (Gram, Cholesky 'U' with the strictly lower part zeroed, Q = A R^{-1}),
+ * repeated.  On return A holds Q and R (n x n, leading dim ldR) the upper
+ * triangular factor with A = Q R (strictly lower part zero).
+ *   variant HIP_CHOLQR2:  two plain passes (CholeskyQR2);
+ *   variant HIP_SCHOLQR3: one shifted pass, G + s I with
+ *                         s = 11 (m n + n (n + 1)) 2^-53 ||A||_F^2, then two
+ *                         plain passes (shifted CholeskyQR3, for condition
+ *                         numbers up to about 2^53);
+ *   variant HIP_CHOLQR_AUTO: CholeskyQR2, but a plain pass whose Cholesky
+ *                         breaks down, or whose max / min of diag(R_p)
+ *                         exceeds 2^23 (kappa(A) too close to u^-1/2 for
+ *                         CholeskyQR2), is redone shifted and followed by two
+ *                         plain passes.
+ * gram_slices > 0 computes each Gram through hip_ozaki_gram with that many
+ * slices, 0 through hip_dsyrk_gram.  Returns 0 on success, LAPACK's info
+ * (> 0, the order of the leading minor that is not positive definite) when
+ * a Cholesky breaks down and the variant does not recover, < 0 on errors.
+ * *passes_run (may be NULL) receives the number of completed passes.
+ * Synchronous (one read-back of the Cholesky's info per pass).  The
+ * reference runs one pass and only prints a warning on breakdown
+ * (examples/example_CholeskyQR.c:125-129). */
+#define HIP_CHOLQR_AUTO 0
+#define HIP_CHOLQR2     2
+#define HIP_SCHOLQR3    3
+int hip_cholesky_qr(const int m, const int n, double *A, const int ldA, double *R, const int ldR,
+                    const int variant, const int gram_slices, int *passes_run);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,3 +114,48 @@ def test_cholesky_breakdown_is_reported(hip):
     with pytest.raises(cholqr.CholeskyBreakdown) as ex:
         cholqr.cholesky_qr2(dA, m, n)
     assert ex.value.info == 6 and ex.value.step == 1
+
+
+@pytest.mark.parametrize("variant", ["scholqr3", "auto"])
+def test_shifted_cholesky_qr3_beyond_cholqr2(hip, variant):
+    """kappa = 1e12 is beyond CholeskyQR2 (its Gram has kappa 1e24 > 1/u):
+    plain CholeskyQR2 breaks down or loses orthogonality, while shifted
+    CholeskyQR3 -- asked for, or chosen by the automatic variant from
+    diag(R) -- keeps ||Q^T Q - I||_F and the residual at working precision."""
+    m, n = 100000, 128
+    dA, _ = cholqr.test_matrix(m, n, 1e12, seed=12)
+    try:
+        dQ, dR = cholqr.cholesky_qr2(dA, m, n)
+        orth2, _ = cholqr.qr_errors(dA, dQ, dR, m, n)
+        assert orth2 > 1e-10, orth2
+    except cholqr.CholeskyBreakdown:
+        pass
+    v = hip.SCHOLQR3 if variant == "scholqr3" else hip.CHOLQR_AUTO
+    dQ, dR, passes = cholqr.cholesky_qr(dA, m, n, v)
+    orth, res = cholqr.qr_errors(dA, dQ, dR, m, n)
+    assert passes >= 3, passes
+    assert orth < 1e-13, orth
+    assert res < 1e-14, res
+    R = dR.numpy()
+    assert np.array_equal(np.tril(R, -1), np.zeros((n, n)))
+    assert np.all(np.diag(R) > 0)
+
+
+def test_cholesky_qr_auto_keeps_cholqr2_when_well_conditioned(hip):
+    m, n = 40000, 128
+    dA, _ = cholqr.test_matrix(m, n, 1e3, seed=13)
+    dQ, dR, passes = cholqr.cholesky_qr(dA, m, n, hip.CHOLQR_AUTO)
+    assert passes == 2
+    dQ2, dR2 = cholqr.cholesky_qr2(dA, m, n)
+    assert np.array_equal(dQ.numpy(), dQ2.numpy())
+    assert np.array_equal(dR.numpy(), dR2.numpy())
+
+
+def test_cholesky_qr_rejects_bad_arguments(hip):
+    from ca3dmm_amd import HipError
+    dA = hip.DeviceArray.zeros((10, 20))
+    dR = hip.DeviceArray.zeros((20, 20))
+    with pytest.raises(HipError):
+        hip.cholesky_qr(10, 20, dA, dR)            # m < n
+    with pytest.raises(HipError):
+        hip.cholesky_qr(20, 10, dA, dR, variant=5, lda=20)
