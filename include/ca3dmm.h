@@ -161,4 +161,10 @@ void ca3dmm_engine_print_stat(ca3dmm_engine_p engine);
         attach_func(engine, workbuf_h, workbuf_d);                                              \
     } while (0)
 
+/* Callers built with -DCA3DMM_MPI get the reference's MPI-typed
+ * constructors (include/ca3dmm_mpi.h, libca3dmm_amd_mpi.so). */
+#ifdef CA3DMM_MPI
+#include "ca3dmm_mpi.h"
+#endif
+
 #endif

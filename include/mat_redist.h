@@ -82,4 +82,10 @@ void mat_redist_engine_free(mat_redist_engine_p *engine_);
 }
 #endif
 
+/* Callers built with -DCA3DMM_MPI get the reference's MPI-typed
+ * constructors (include/ca3dmm_mpi.h, libca3dmm_amd_mpi.so). */
+#if defined(CA3DMM_MPI) && !defined(CA3DMM_AMD_CA3DMM_H)
+#include "ca3dmm_mpi.h"
+#endif
+
 #endif

@@ -1,5 +1,7 @@
 """CPU: the C-ABI library loads and exports every function include/*.h declares
-(no compute calls: there is no GPU here)."""
+(no compute calls: there is no GPU here).  include/ca3dmm_mpi.h belongs to
+libca3dmm_amd_mpi.so (built where an MPI is installed); every other header to
+libca3dmm_amd.so."""
 import ctypes
 import os
 import re
@@ -9,11 +11,16 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+MPI_HEADER = "ca3dmm_mpi.h"
+
+
+def declared_functions(headers=None):
     names = set()
     pat = re.compile(r"^\s*(?:[A-Za-z_][\w\s\*]*?[\s\*])([A-Za-z_]\w*)\s*\(", re.M)
     for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
-        if not fn.endswith(".h"):
+        if not fn.endswith(".h") or (headers is None and fn == MPI_HEADER):
+            continue
+        if headers is not None and fn not in headers:
             continue
         text = open(os.path.join(ROOT, "include", fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -38,6 +45,17 @@ def test_library_exports_every_declared_symbol():
     from ca3dmm_amd import LIB_PATH
     lib = ctypes.CDLL(LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_mpi_library_exports_the_mpi_header():
+    path = os.path.join(ROOT, "ca3dmm_amd", "lib", "libca3dmm_amd_mpi.so")
+    if not os.path.exists(path):
+        pytest.skip("libca3dmm_amd_mpi.so is built only where an MPI is installed")
+    lib = ctypes.CDLL(path)
+    names = declared_functions([MPI_HEADER])
+    assert "ca3dmm_engine_init_mpi" in names and "mat_redist_engine_init_mpi" in names
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, f"declared but not exported: {missing}"
 
 
