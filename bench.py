@@ -3,14 +3,18 @@
 
 Workload: the plain-FP64 configuration -- C = op(A) * op(B) with
 M = N = K = 8192, FP64 entries uniform[-1,1] (fixed seed), for all four
-transA/transB combinations, through the drop-in hip_dgemm (the Cannon local
-DGEMM of a 1x1x1 grid).  One step = the four products.  Inputs are resident
-in HBM before timing starts.
+transA/transB combinations, through the drop-in engine API: one
+ca3dmm_engine_init per case on the single-process group with DEV_TYPE_HIP,
+and ca3dmm_engine_exec (src/ca3dmm.h:191-194) per product, which on the
+1x1x1 grid is the FP64-MFMA DGEMM of the Cannon step.  One step = the four
+products.  Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line (last line of stdout).  The metric counts algorithmic
 flops 2*M*N*K per product.  `roofline` is measured live with HIP events on
-the library stream; `cpu_baseline` times the reference's CPU local DGEMM
-(cblas_dgemm NN, here through numpy's OpenBLAS) on a bounded sample.
+the library stream around back-to-back launches of the same DGEMM;
+`cpu_baseline` runs the CPU restatement of the reference's exec sequence on
+the example_AB CPU workload (4 simulated ranks, 2000^3, NN) on the host's
+cores.
 """
 import argparse
 import json
@@ -45,26 +49,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def ozaki_at_equal_error(hip, n, rnd):
-    """Native FP64-MFMA DGEMM vs the two INT8 Ozaki forms on phi = 0.5 inputs
-    at n^3: for each form, the smallest slice count (slices) or modulus count
-    (CRT form) whose errors -- max |C - C_dd| / (|A||B|) and max relative
-    error, on 4096 sampled entries against double-double -- are both no worse
-    than the native product's; `best` is the faster of the two.  `chooser`
-    is the CRT form at the count hip_ozaki_choose picks without the search."""
+def ozaki_at_equal_error(hip, m, n, k, phi, rnd, slice_counts=range(6, 12), crt_counts=range(13, 26)):
+    """Native FP64-MFMA DGEMM vs the two INT8 Ozaki forms on
+    uniform[-0.5,0.5]*exp(phi N(0,1)) inputs at m x n x k: for each form, the
+    smallest slice count (slices) or modulus count (CRT form) whose errors --
+    max |C - C_dd| / (|A||B|) and max relative error, on 4096 sampled entries
+    against double-double -- are both no worse than the native product's;
+    `best` is the faster of the two.  `chooser` is the CRT form at the count
+    hip_ozaki_choose picks without the search."""
     import numpy as np
 
     from oracle import oracle as O
     from oracle import ozaki as Z
-    A = Z.phi_matrix(rnd, (n, n), 0.5)
-    B = Z.phi_matrix(rnd, (n, n), 0.5)
+    A = np.asfortranarray(Z.phi_matrix(rnd, (m, k), phi))
+    B = np.asfortranarray(Z.phi_matrix(rnd, (k, n), phi))
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
-    dC = hip.DeviceArray.zeros((n, n))
-    ii, jj = rnd.integers(0, n, 4096), rnd.integers(0, n, 4096)
-    ref, ab = O.dd_sample(0, 0, n, n, n, np.asfortranarray(A), n, np.asfortranarray(B), n, ii, jj)
+    dC = hip.DeviceArray.zeros((m, n))
+    ii, jj = rnd.integers(0, m, 4096), rnd.integers(0, n, 4096)
+    ref, ab = O.dd_sample(0, 0, m, n, k, A, m, B, k, ii, jj)
     del A, B
     timer = hip.Timer()
-    flops = 2.0 * n ** 3
+    flops = 2.0 * m * n * k
 
     def timed(f, reps=3):
         f()
@@ -80,7 +85,7 @@ def ozaki_at_equal_error(hip, n, rnd):
         return float(np.max(e / ab)), float(np.max(e / np.abs(ref)))
 
     hip.set_sync(False)
-    nat_ms = timed(lambda: hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC))
+    nat_ms = timed(lambda: hip.dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC))
     nat_ab, nat_rel = errs()
 
     def search(key, counts, call, products):
@@ -99,19 +104,19 @@ def ozaki_at_equal_error(hip, n, rnd):
                 break
         return {"tried": tried, "at_equal_error": best}
 
-    slices = search("s", range(6, 12), lambda s: hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, s),
+    slices = search("s", slice_counts, lambda s: hip.ozaki_dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, s),
                     lambda s: s * (s + 1) // 2)
-    crt = search("moduli", range(13, 26), lambda N: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, N),
+    crt = search("moduli", crt_counts, lambda N: hip.ozaki2_dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, N),
                  lambda N: N)
     # the counts hip_ozaki_choose picks from the operands' spread, timed the same way
-    ch_s, ch_N = hip.ozaki_choose(0, 0, n, n, n, dA, dB)
-    ms = timed(lambda: hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, ch_N))
+    ch_s, ch_N = hip.ozaki_choose(0, 0, m, n, k, dA, dB)
+    ms = timed(lambda: hip.ozaki2_dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC, ch_N))
     e_ab, e_rel = errs()
     chooser = {"slices": ch_s, "moduli": ch_N, "crt_ms": round(ms, 3), "crt_max_err_over_absAabsB": e_ab,
                "crt_max_rel_err": e_rel, "no_worse_than_native": bool(e_ab <= nat_ab and e_rel <= nat_rel),
                "speedup_vs_native": round(nat_ms / ms, 3)}
     hip.set_sync(True)
-    out = {"inputs": "uniform[-0.5,0.5]*exp(0.5*N(0,1)), phi=0.5", "m": n, "n": n, "k": n, "samples": 4096,
+    out = {"inputs": f"uniform[-0.5,0.5]*exp({phi}*N(0,1)), phi={phi}", "m": m, "n": n, "k": k, "samples": 4096,
            "oracle": "double-double dot products", "native_ms": round(nat_ms, 3),
            "native_tflops": round(flops / nat_ms / 1e9, 3), "native_max_err_over_absAabsB": nat_ab,
            "native_max_rel_err": nat_rel, "slices": slices, "crt": crt, "chooser": chooser}
@@ -132,7 +137,7 @@ def main():
     ap.add_argument("--k", type=int, default=None, help="k of the ozaki-forms suite (default n)")
     ap.add_argument("--cases", default="NN,NT,TN,TT")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-tests", type=int, default=5, help="timed executions of the CPU baseline")
     ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--suite", default=None,
                     choices=["rank-share", "ozaki-sweep", "ozaki-forms", "grouped", "cholqr", "pmc-calib",
@@ -163,11 +168,21 @@ def main():
     dC = hip.DeviceArray.zeros((n, n))
     hip.sync()
 
-    hip.set_sync(False)
+    import ctypes as C
+
+    from ca3dmm_amd import engine as E
+    engines = []
+    for ta, tb, name in cases:
+        eng = E.CA3P()
+        E.lib.ca3dmm_engine_init(n, n, n, ta, tb, 0, n, 0, n, 0, n, 0, n, 0, n, 0, n, None, E.comm_self(),
+                                 E.DEV_TYPE_HIP, C.byref(eng), None)
+        if not eng:
+            raise RuntimeError(f"ca3dmm_engine_init failed for {name}")
+        engines.append(eng)
 
     def step():
-        for ta, tb, _ in cases:
-            hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
+        for eng in engines:
+            E.lib.ca3dmm_engine_exec(eng, dA.ptr, n, dB.ptr, n, dC.ptr, n)
 
     for w in range(args.warmup):
         step()
@@ -194,24 +209,46 @@ def main():
     except Exception as ex:  # the oracle is optional for timing
         log(f"bench: accuracy sample skipped: {ex}")
 
-    timer = hip.Timer()
     hip.sync()
     t0 = time.perf_counter()
-    timer.start()
     for s in range(args.steps):
         step()
-    gpu_ms = timer.stop()
     hip.sync()
     wall = time.perf_counter() - t0
     log(f"bench: {args.steps} timed steps done in {wall:.3f} s")
+    exec_stat = {}
+    for (_, _, name), eng in zip(cases, engines):
+        e = eng.contents
+        exec_stat[name] = round(e.exec_ms / max(1, e.n_exec), 4)
+        E.lib.ca3dmm_engine_free(C.byref(eng))
+
+    # the dominant kernel's launch duration: the same DGEMMs launched back to
+    # back on the library stream, bracketed by HIP events on that stream
+    timer = hip.Timer()
+    hip.set_sync(False)
+    for ta, tb, _ in cases:
+        hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
+    hip.sync()
+    timer.start()
+    for s in range(args.steps):
+        for ta, tb, _ in cases:
+            hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
+    gpu_ms = timer.stop()
     hip.set_sync(True)
+    log("bench: kernel timing done")
 
     # Ozaki at equal max error: phi = 0.5 inputs (uniform[-0.5,0.5]*exp(0.5 N(0,1))),
     # native error measured on sampled entries against double-double, then
     # for each Ozaki form the smallest slice / modulus count whose error is no worse
     ozaki = None
     if not args.no_ozaki:
-        ozaki = ozaki_at_equal_error(hip, n, rnd)
+        log("bench: Ozaki at equal error, 8192^3 phi = 0.5")
+        ozaki = ozaki_at_equal_error(hip, n, n, n, 0.5, rnd)
+        # the hardest measured row: tall-skinny k = 1024 with the widest
+        # exponent spread (phi = 4)
+        log("bench: Ozaki at equal error, 65536 x 65536 x 1024 phi = 4")
+        ozaki["tall_phi4"] = ozaki_at_equal_error(hip, 65536, 65536, 1024, 4.0, rnd, slice_counts=range(9, 12),
+                                                  crt_counts=range(17, 26))
     hip.set_sync(False)
 
     flops_per_launch = 2.0 * n * n * n
@@ -222,12 +259,18 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import cpu_baseline as CB
-        nb = 4096
-        log(f"bench: CPU baseline ({nb}^3 DGEMM, ~{args.cpu_seconds:.0f} s)")
-        tf, reps, el = CB.time_dgemm(nb, args.cpu_seconds)
-        cpu = {"value": round(tf, 4), "unit": "TFLOP/s", "cores": CB.threads(), "kind": "port",
-               "sample": f"{reps} x {nb}^3 NN DGEMM (reference CPU local DGEMM, cblas_dgemm "
-                         f"shape) via numpy/{CB.blas_name()}, {el:.1f} s"}
+        log("bench: CPU baseline (example_AB CPU flow, 4 simulated ranks, 2000^3 NN)")
+        t0c = time.perf_counter()
+        r = CB.run_example_ab(2000, 2000, 2000, 4, n_test=args.cpu_tests)
+        el = time.perf_counter() - t0c
+        cpu = {"value": round(r["tflops"], 4), "unit": "TFLOP/s", "cores": CB.threads(), "kind": "port",
+               "sample": f"example_AB CPU flow: 4 simulated ranks, M=N=K=2000 NN, uniform[-1,1], grid "
+                         f"{'x'.join(map(str, r['grid']))}, 1 warm-up + {r['n_test']} timed executions "
+                         f"({el:.1f} s with the linear-fill validation); local DGEMM through numpy/"
+                         f"{CB.blas_name()}",
+               "workload": "example_AB, 4 ranks, dev_type=0, 2000^3, transA=transB=0, 1D column layout",
+               "phases_ms": {k_: round(v, 3) for k_, v in r["ms"].items()},
+               "validation_errors": r["errors"], "blas": CB.blas_name()}
 
     out = {
         "metric": METRIC,
@@ -242,9 +285,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"hip_dgemm FP64-MFMA {n}^3 x {{{','.join(c for _, _, c in cases)}}}, "
-                               "uniform[-1,1] FP64, 1x1x1 grid local DGEMM",
-                   "m": n, "n": n, "k": n, "method": "dgemm-mfma"},
+        "config": {"workload": f"ca3dmm_engine_exec (DEV_TYPE_HIP, 1x1x1 grid) FP64-MFMA {n}^3 x "
+                               f"{{{','.join(c for _, _, c in cases)}}}, uniform[-1,1] FP64",
+                   "m": n, "n": n, "k": n, "method": "dgemm-mfma", "api": "ca3dmm_engine_exec",
+                   "engine_exec_ms": exec_stat},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": measured_traffic("dgemm_kernel", n), "kernel": "dgemm_kernel",
