@@ -412,12 +412,47 @@ size_t redist_bytes(const mat_redist_engine_p e)
     return e ? e->dt_size * ((size_t) e->send_cnt + (size_t) e->recv_cnt) : 0;
 }
 
+// 1: the mat_redist steps run staged through the engines' send and receive
+// buffers (pack, exchange, unpack: three passes over the data); 0 (default):
+// device-resident, one pass.
+bool redist_staged() { return env_int("CA3DMM_REDIST_STAGED", 0, 0, 1) != 0; }
+
+// The device-resident redistribution of a set of engines that together form
+// one group (all ranks of the simulated grid, or the single rank).  Every
+// rank's memory is on this GPU, so each intersection block goes straight
+// from the sender's source block to the receiver's destination block: the
+// receiver reads the sender's data where it lies, as a P2P read over xGMI
+// would, and the pack, exchange and unpack of src/mat_redist.c:317-416
+// collapse into one pass.  The blocks, their order and the bytes are those
+// of the staged path.  Descriptors are appended to d (one launch for
+// several matrices).
+void redist_direct(int n, const mat_redist_engine_p *eng, const void *const *src, const int *src_ld,
+                   void *const *dst, const int *dst_ld, std::vector<hip_blk_copy_t> &d)
+{
+    (void) n;
+    for (int r = 0; r < n; ++r) {
+        const mat_redist_engine_p e = eng[r];
+        const size_t dt = e->dt_size;
+        for (int i = 0; i < e->n_proc_recv; ++i) {
+            const int *b = e->rblk_sizes + 4 * i;
+            const int sr = e->recv_ranks[i];
+            const mat_redist_engine_p s = eng[sr];
+            hip_blk_copy_t c;
+            c.src = (const char *) src[sr] + dt * ((size_t) (b[0] - s->src_srow) * src_ld[sr] + (b[1] - s->src_scol));
+            c.dst = (char *) dst[r] + dt * ((size_t) (b[0] - e->req_srow) * dst_ld[r] + (b[1] - e->req_scol));
+            c.lds = src_ld[sr]; c.ldd = dst_ld[r]; c.nrow = b[2]; c.ncol = b[3];
+            d.push_back(c);
+        }
+    }
+}
+
 // Pack, exchange and unpack of a set of engines that together form one group
-// (all ranks of the simulated grid, or the single rank).  Each step is one
-// batched copy launch.  For a single rank the self exchange is the identity
-// and the unpack reads the send buffer directly.
-int redist_run(int n, const mat_redist_engine_p *eng, const void *const *src, const int *src_ld,
-               void *const *dst, const int *dst_ld)
+// (mat_redist.c:317-416), each step one batched copy launch, through the
+// engines' work buffers.  For a single rank the self exchange is the identity
+// and the unpack reads the send buffer directly.  The default path is
+// redist_direct (one pass); this one runs with CA3DMM_REDIST_STAGED=1.
+int redist_staged_run(int n, const mat_redist_engine_p *eng, const void *const *src, const int *src_ld,
+                      void *const *dst, const int *dst_ld)
 {
     std::vector<hip_blk_copy_t> d;
     const size_t dt = eng[0]->dt_size;
@@ -468,6 +503,15 @@ int redist_run(int n, const mat_redist_engine_p *eng, const void *const *src, co
         }
     }
     return hip_copy_matrix_blocks(dt, (int) d.size(), d.data());
+}
+
+int redist_run(int n, const mat_redist_engine_p *eng, const void *const *src, const int *src_ld,
+               void *const *dst, const int *dst_ld)
+{
+    if (redist_staged()) return redist_staged_run(n, eng, src, src_ld, dst, dst_ld);
+    std::vector<hip_blk_copy_t> d;
+    redist_direct(n, eng, src, src_ld, dst, dst_ld, d);
+    return d.empty() ? 0 : hip_copy_matrix_blocks(eng[0]->dt_size, (int) d.size(), d.data());
 }
 
 // ---------------------------------------------------------------------
@@ -1111,6 +1155,8 @@ struct ca3dmm_sim {
     int ozaki = 0;                // > 0: the local products through the CRT form with this many moduli
     ca3::grouped::Plan *plan = nullptr;
     ca3::grouped::OzPlan *ozplan = nullptr;
+    ca3::grouped::SeqGraph *seq = nullptr;  // the whole exec sequence, captured (use_graph)
+    std::vector<uintptr_t> seq_key;         // arguments and settings it was captured for
 };
 
 namespace {
@@ -1200,8 +1246,10 @@ int build_ozaki(ca3dmm_sim *S)
     return rc ? rc : ca3::grouped::build_ozaki(probs, folds, S->ozaki, &S->ozplan);
 }
 
-int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
-             const int *ldB, void *const *dst_C, const int *ldC)
+// The exec sequence of all ranks, enqueued on the library stream without
+// synchronising (so that it can be captured whole into a hipGraph).
+int sim_sequence(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
+                 const int *ldB, void *const *dst_C, const int *ldC)
 {
     const Problem &P = S->P;
     const int p = P.p;
@@ -1210,14 +1258,21 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     std::vector<void *> dp(p);
     std::vector<int> sl(p), dl(p);
     int rc = 0;
-    // 1. redistribute A and B
+    // 1. redistribute A and B: one pass, both matrices in one batched launch
+    //    (staged: pack, exchange and unpack per matrix)
+    const bool staged = redist_staged();
+    std::vector<hip_blk_copy_t> rd;
     if (!P.btb) {
         for (int r = 0; r < p; ++r) {
             re[r] = S->eng[r]->redist_A; sp[r] = src_A[r]; sl[r] = ldA[r];
             dp[r] = S->eng[r]->A_rd_recv_d;
             dl[r] = std::max(1, recv_ld(S->eng[r]->A_rd_nrow, S->eng[r]->A_rd_ncol, P.tA));
         }
-        if ((rc = redist_run(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data()))) return rc;
+        if (staged) {
+            if ((rc = redist_staged_run(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data()))) return rc;
+        } else {
+            redist_direct(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data(), rd);
+        }
     }
     for (int r = 0; r < p; ++r) {
         ca3dmm_engine_p e = S->eng[r];
@@ -1225,7 +1280,12 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
         dp[r] = P.btb ? e->B_2dmm_d : e->B_rd_recv_d;
         dl[r] = std::max(1, P.btb ? e->B_2dmm_nrow : recv_ld(e->B_rd_nrow, e->B_rd_ncol, P.tB));
     }
-    if ((rc = redist_run(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data()))) return rc;
+    if (staged) {
+        if ((rc = redist_staged_run(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data()))) return rc;
+    } else {
+        redist_direct(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data(), rd);
+        if (!rd.empty() && (rc = hip_copy_matrix_blocks(8, (int) rd.size(), rd.data()))) return rc;
+    }
 
     // 2. transposes and the allgather of the replicated operand
     if (!P.btb) {
@@ -1285,13 +1345,11 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     //    reconstructed values (products in one persistent launch when grouped)
     const bool fp64_rounds = !S->ozaki;
     if (S->ozaki) {
-        if (!S->ozplan && (rc = build_ozaki(S))) return rc;
-        if ((rc = ca3::grouped::run_ozaki(S->ozplan, S->grouped != 0, S->use_graph != 0))) return rc;
+        if ((rc = ca3::grouped::run_ozaki(S->ozplan, S->grouped != 0, false))) return rc;
     } else if (S->grouped) {
         // 3.+4. as one persistent grouped launch (rounds accumulated in
         //    registers, fold done by the last layer of each tile)
-        if (!S->plan && (rc = build_grouped(S))) return rc;
-        if ((rc = ca3::grouped::run(S->plan, S->use_graph != 0))) return rc;
+        if ((rc = ca3::grouped::run(S->plan, false))) return rc;
     }
 
     // 3. Cannon rounds: round s of rank (row, col) multiplies the A block of
@@ -1342,6 +1400,71 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
         }
         if ((rc = redist_run(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data()))) return rc;
     }
+    return 0;
+}
+
+// What a captured sequence depends on: every argument of the exec, the
+// mode, the tunings and the redistribution path.
+std::vector<uintptr_t> sequence_key(ca3dmm_sim *S, const void *const *src_A, const int *ldA,
+                                    const void *const *src_B, const int *ldB, void *const *dst_C,
+                                    const int *ldC)
+{
+    std::vector<uintptr_t> k;
+    const int p = S->P.p;
+    k.push_back((uintptr_t) S->grouped);
+    k.push_back((uintptr_t) S->ozaki);
+    k.push_back((uintptr_t) redist_staged());
+    if (S->plan) {
+        const ca3::grouped::Tuning &t = ca3::grouped::tuning(S->plan);
+        k.push_back((uintptr_t) t.group_rows);
+        k.push_back((uintptr_t) t.wg_per_cu);
+    }
+    if (S->ozplan) {
+        const ca3::grouped::OzTuning &t = ca3::grouped::tuning_ozaki(S->ozplan);
+        k.push_back((uintptr_t) t.tile_n);
+        k.push_back((uintptr_t) t.xcd_queue);
+    }
+    for (int r = 0; r < p; ++r) {
+        k.push_back(src_A ? (uintptr_t) src_A[r] : 0);
+        k.push_back(ldA ? (uintptr_t) ldA[r] : 0);
+        k.push_back((uintptr_t) src_B[r]);
+        k.push_back((uintptr_t) ldB[r]);
+        k.push_back(dst_C ? (uintptr_t) dst_C[r] : 0);
+        k.push_back(ldC ? (uintptr_t) ldC[r] : 0);
+    }
+    return k;
+}
+
+// One exec of the whole grid.  With use_graph the sequence -- the
+// redistributions of A and B, transposes, allgathers, the products and the
+// fold, the redistribution of C -- is captured into one hipGraph on the first
+// call (again when an argument or setting changes) and replayed after that.
+int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
+             const int *ldB, void *const *dst_C, const int *ldC)
+{
+    int rc = 0;
+    // the plans allocate and synchronise: build them outside any capture
+    if (S->ozaki && !S->ozplan && (rc = build_ozaki(S))) return rc;
+    if (!S->ozaki && S->grouped && !S->plan && (rc = build_grouped(S))) return rc;
+    const bool save = ca3::grouped::sync_mode();
+    hip_proxy_set_sync(0);
+    auto body = [&]() { return sim_sequence(S, src_A, ldA, src_B, ldB, dst_C, ldC); };
+    if (S->use_graph) {
+        std::vector<uintptr_t> key = sequence_key(S, src_A, ldA, src_B, ldB, dst_C, ldC);
+        if (!S->seq || key != S->seq_key) {
+            (void) hip_stream_sync(nullptr);
+            ca3::grouped::destroy_sequence(S->seq);
+            S->seq = nullptr;
+            S->seq_key.clear();
+            rc = ca3::grouped::capture_sequence(&S->seq, body);
+            if (!rc) S->seq_key = key;
+        }
+        if (!rc) rc = ca3::grouped::replay_sequence(S->seq);
+    } else {
+        rc = body();
+    }
+    hip_proxy_set_sync(save ? 1 : 0);
+    if (rc) return rc;
     return hip_stream_sync(nullptr);
 }
 
@@ -1496,6 +1619,8 @@ int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu)
 void ca3dmm_sim_free(ca3dmm_sim_p *sim)
 {
     if (!sim || !*sim) return;
+    (void) hip_stream_sync(nullptr);
+    ca3::grouped::destroy_sequence((*sim)->seq);
     if ((*sim)->plan) ca3::grouped::destroy((*sim)->plan);
     if ((*sim)->ozplan) {
         (void) hip_stream_sync(nullptr);

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace ca3 {
@@ -75,6 +76,16 @@ void set_tuning_ozaki(OzPlan *p, const OzTuning &t);
 // sequence from a captured hipGraph.
 int run_ozaki(OzPlan *p, bool grouped, bool use_graph);
 void destroy_ozaki(OzPlan *p);
+
+// A whole enqueued sequence (every kernel `body` launches on the library
+// stream) captured once into a hipGraph and replayed.  body must not
+// synchronise, allocate or copy through the host while it is captured.
+struct SeqGraph;
+int capture_sequence(SeqGraph **out, const std::function<int()> &body);
+// The library's sync mode (hip_proxy_set_sync), for code outside the HIP sources.
+bool sync_mode();
+int replay_sequence(SeqGraph *g);
+void destroy_sequence(SeqGraph *g);
 
 }  // namespace grouped
 }  // namespace ca3

@@ -291,5 +291,47 @@ void destroy(Plan *p)
     delete p;
 }
 
+struct SeqGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int capture_sequence(SeqGraph **out, const std::function<int()> &body)
+{
+    *out = nullptr;
+    hipStream_t st = ca3::stream();
+    CA3_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) { if (g) (void) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return ca3::fail(e, "hipStreamEndCapture");
+    auto *sg = new SeqGraph;
+    sg->graph = g;
+    e = hipGraphInstantiate(&sg->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        destroy_sequence(sg);
+        return ca3::fail(e, "hipGraphInstantiate");
+    }
+    *out = sg;
+    return 0;
+}
+
+bool sync_mode() { return ca3::sync_mode(); }
+
+int replay_sequence(SeqGraph *g)
+{
+    CA3_CHECK(hipGraphLaunch(g->exec, ca3::stream()));
+    return 0;
+}
+
+void destroy_sequence(SeqGraph *g)
+{
+    if (!g) return;
+    if (g->exec) (void) hipGraphExecDestroy(g->exec);
+    if (g->graph) (void) hipGraphDestroy(g->graph);
+    delete g;
+}
+
 }  // namespace grouped
 }  // namespace ca3

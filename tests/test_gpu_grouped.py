@@ -150,3 +150,51 @@ def test_grouped_ozaki_every_tuning_bit_exact(hip):
         sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
         assert np.array_equal(dC.numpy(), want)
     sim.free()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
+def test_redistribution_one_pass_equals_staged(hip, ta, tb, monkeypatch):
+    """The device-resident redistribution (each block straight from the
+    sender's source block to the receiver's destination, A and B in one
+    launch) gives the same bits as the staged pack / exchange / unpack of
+    src/mat_redist.c:317-416 (CA3DMM_REDIST_STAGED=1), on the 24-rank grid
+    and on the single-rank engine's forced redistribution."""
+    rnd = np.random.default_rng(50 + 2 * ta + tb)
+    m, n, k = 601, 433, 517
+    A = rnd.uniform(-1, 1, (k, m) if ta else (m, k))
+    B = rnd.uniform(-1, 1, (n, k) if tb else (k, n))
+    monkeypatch.setenv("CA3DMM_REDIST_STAGED", "1")
+    staged = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3])
+    monkeypatch.setenv("CA3DMM_REDIST_STAGED", "0")
+    direct = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3])
+    assert np.array_equal(staged, direct)
+    graph = run_sim(24, m, n, k, ta, tb, A, B, proc_grid=[4, 2, 3], grouped=True, graph=True, repeat=2)
+    bound = k * 2.0 ** -53 * (np.abs(A.T if ta else A) @ np.abs(B.T if tb else B))
+    assert np.all(np.abs(graph - (A.T if ta else A) @ (B.T if tb else B)) <= bound + 1e-300)
+
+
+def test_sequence_graph_recaptured_for_new_arguments(hip):
+    """The whole exec sequence replayed from its hipGraph follows new input
+    buffers (a new capture) and a new tuning, bit-exact on integers."""
+    from ca3dmm_amd import device as dev
+    from ca3dmm_amd import engine as E
+    from oracle import oracle as O
+    rnd = np.random.default_rng(60)
+    m, n, k, p, grid = 500, 460, 700, 24, [4, 2, 3]
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=grid)
+    sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=grid, allocate=True)
+    sim.set_mode(grouped=True, graph=True)
+    dC = dev.DeviceArray.filled((m, n), 0xFF)
+    Cp = [dC.at(r[2] * m) for r in w["dst_C"]]
+    for it in range(3):
+        A, B = _ints(rnd, (m, k)), _ints(rnd, (k, n))
+        dA, dB = dev.DeviceArray.from_numpy(A), dev.DeviceArray.from_numpy(B)     # new buffers each time
+        Ap = [dA.at(r[2] * m) for r in w["src_A"]]
+        Bp = [dB.at(r[2] * k) for r in w["src_B"]]
+        if it == 2:
+            sim.set_tuning(4, 1)
+        for _ in range(2):
+            sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
+            assert np.array_equal(dC.numpy(), A @ B), it
+        del dA, dB
+    sim.free()
