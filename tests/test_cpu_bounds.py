@@ -1,0 +1,259 @@
+"""CPU: every device address the data-movement and GEMM kernels form, checked
+against its allocation, by restating the kernels' index arithmetic in numpy
+for the shapes the tests and the measured configurations use.  A GPU fault
+from an out-of-range access shows up here first.
+
+* dgemm_kernel / grouped_kernel / gram_partial_kernel operand loaders
+  (dgemm_tile.h OperandLoader<XC>): which operand is read as contiguous
+  along m / n (XC = 1) or along k (XC = 0) is read from the kernel sources
+  themselves, and each case's extreme element offsets are compared with the
+  allocation its BLAS convention implies (A is lda x (transA ? m : k), B is
+  ldb x (transB ? k : n)).  The full-stage (unpredicated) path must stay
+  inside the operand for interior tiles, including the repeated last load.
+* tile_coords (XCD remap + groups of 16 tile-rows) and grouped_coords visit
+  every tile exactly once.
+* copy_blocks_kernel: the row of a flattened index from the double
+  reciprocal plus one correction is exact, and the tiles of a batch cover
+  every row of every block once.
+* the one-pass redistribution: each block lies inside the sender's source
+  block and the receiver's destination block, for the grids of the tests
+  and of the measured configurations (oracle plans).
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "ca3dmm_amd", "csrc")
+BM = BN = 128
+BK = 16
+
+
+def _src(name):
+    with open(os.path.join(CSRC, name)) as f:
+        return f.read()
+
+
+# --------------------------------------------------------------------------- loaders
+def loader_xc(text, var, flag):
+    """XC of `OperandLoader<...> var` in a kernel source: 1, 0, or for a
+    template flag ('TA'/'TB') a function of it."""
+    m = re.search(r"OperandLoader<\s*(!?)\s*(\w+)\s*>\s*" + var + r"\s*;", text)
+    assert m, f"no OperandLoader declaration for {var}"
+    neg, arg = m.group(1) == "!", m.group(2)
+    if arg in ("0", "1"):
+        return lambda t: int(arg)
+    assert arg == flag, (arg, flag)
+    return (lambda t: 1 - t) if neg else (lambda t: t)
+
+
+def operand_extremes(xc, xdim, K, ld, x0s, kspans):
+    """Largest element offset a loader touches: XC = 1: x + k ld, XC = 0:
+    k + x ld, for x in [x0, x0 + 128) clipped to xdim (the predicate) and the
+    k ranges of its stages."""
+    hi = 0
+    for x0, (k0, k1) in zip(x0s, kspans):
+        xl = min(xdim, x0 + 128) - 1
+        kl = k1 - 1
+        if xl < 0 or kl < 0:
+            continue
+        hi = max(hi, xl + kl * ld if xc else kl + xl * ld)
+    return hi
+
+
+def stage_spans(M, N, K, m0, n0):
+    """k ranges the loaders read for tile (m0, n0) of mainloop: interior
+    tiles read whole 16-deep stages unpredicated (the last one repeated),
+    then the predicated tail; edge tiles are predicated throughout."""
+    interior = m0 + BM <= M and n0 + BN <= N
+    nfull = K // BK if interior else 0
+    spans = []
+    if nfull:
+        spans.append((0, nfull * BK))                 # includes the repeated last stage
+    if nfull * BK < K:
+        spans.append((nfull * BK, K))                 # predicated: never past K
+    return spans
+
+
+DGEMM_SHAPES = [(1, 1, 1), (15, 17, 3), (129, 130, 17), (257, 64, 1000), (64, 511, 33), (777, 555, 1999),
+                (1000, 1000, 1000), (2000, 4000, 1334), (2000, 4000, 1333), (2501, 5003, 1668),
+                (8192, 8192, 8192), (8192, 128, 8192), (128, 8192, 8192)]
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_dgemm_loaders_stay_inside_the_operands(ta, tb):
+    text = _src("dgemm.hip")
+    xa, xb = loader_xc(text, "la", "TA")(ta), loader_xc(text, "lb", "TB")(tb)
+    for M, N, K in DGEMM_SHAPES:
+        lda, ldb = (K if ta else M), (N if tb else K)          # tight leading dims
+        alloc_a = lda * (M if ta else K)
+        alloc_b = ldb * (K if tb else N)
+        tiles = [(tm * BM, tn * BN) for tm in range(-(-M // BM)) for tn in range(-(-N // BN))]
+        for m0, n0 in tiles:
+            sp = stage_spans(M, N, K, m0, n0)
+            for k0, k1 in sp:
+                # full stages read every element of the 128 x 16 block: they
+                # must be whole and in range on both axes
+                if (k0, k1) == sp[0] and m0 + BM <= M and n0 + BN <= N and k1 - k0 >= BK:
+                    assert k1 <= K and m0 + BM <= M and n0 + BN <= N
+            ha = operand_extremes(xa, M, K, lda, [m0] * len(sp), sp)
+            hb = operand_extremes(xb, N, K, ldb, [n0] * len(sp), sp)
+            assert ha < alloc_a, (ta, tb, M, N, K, m0, ha, alloc_a)
+            assert hb < alloc_b, (ta, tb, M, N, K, n0, hb, alloc_b)
+
+
+def test_grouped_and_gram_loaders_read_their_blocks_as_stored():
+    """grouped_kernel reads each Cannon round's A block (M x k, lda = M) along
+    m and its B block (k x N, ldb = k) along k; the Gram reads A^T and A of
+    a k x n A along k.  Any other XC would leave the blocks."""
+    g = _src("grouped.hip")
+    assert loader_xc(g, "la", "-")(0) == 1 and loader_xc(g, "lb", "-")(0) == 0
+    for M, N, K in [(2501, 5003, 1668), (2502, 5004, 1667), (1000, 500, 334), (3, 5, 7)]:
+        for m0 in range(0, M, BM):
+            for n0 in range(0, N, BN):
+                sp = stage_spans(M, N, K, m0, n0)
+                assert operand_extremes(1, M, K, M, [m0] * len(sp), sp) < M * K
+                assert operand_extremes(0, N, K, K, [n0] * len(sp), sp) < K * N
+    gram = _src("gram.hip")
+    m = re.search(r"OperandLoader<(\d)>\s+la,\s*lb;", gram)
+    assert m and m.group(1) == "0"
+    for n, k in [(512, 2000000), (130, 1000), (5, 3)]:
+        for t0 in range(0, n, BM):
+            assert operand_extremes(0, n, k, k, [t0], [(0, k)]) < k * n
+
+
+# --------------------------------------------------------------------------- tile order
+def tile_coords(bid, nb, M, N):
+    """dgemm_tile.h tile_coords, restated."""
+    tiles_m, tiles_n = -(-M // BM), -(-N // BN)
+    if nb % 8 == 0:
+        bid = (bid & 7) * (nb >> 3) + (bid >> 3)
+    G = 16
+    per = G * tiles_n
+    grp = bid // per
+    first = grp * G
+    gsz = min(tiles_m - first, G)
+    r = bid - grp * per
+    return first + r % gsz, r // gsz
+
+
+@pytest.mark.parametrize("M,N", [(128, 128), (1000, 1000), (2000, 4000), (8192, 8192), (2501, 5003),
+                                 (130, 9000), (9000, 130), (17 * 128, 3 * 128), (100, 100)])
+def test_tile_order_is_a_bijection(M, N):
+    tm_n, tn_n = -(-M // BM), -(-N // BN)
+    nb = tm_n * tn_n
+    seen = {tile_coords(b, nb, M, N) for b in range(nb)}
+    assert seen == {(i, j) for i in range(tm_n) for j in range(tn_n)}
+
+
+@pytest.mark.parametrize("G", [1, 4, 8, 16])
+def test_grouped_tile_order_is_a_bijection(G):
+    for tiles_m, tiles_n in [(20, 40), (19, 39), (1, 7), (33, 2)]:
+        seen = set()
+        for t in range(tiles_m * tiles_n):
+            per, grp = G * tiles_n, t // (G * tiles_n)
+            first = grp * G
+            gsz = min(tiles_m - first, G)
+            r = t - grp * per
+            seen.add((first + r % gsz, r // gsz))
+        assert seen == {(i, j) for i in range(tiles_m) for j in range(tiles_n)}
+
+
+# --------------------------------------------------------------------------- copies
+def row_col(e, ncol):
+    """copy_kernels.hip row_col, restated on uint32 values."""
+    inv = 1.0 / np.float64(ncol)
+    r = (e.astype(np.float64) * inv).astype(np.int64)
+    c = e.astype(np.int64) - r * ncol
+    low = c < 0
+    r = np.where(low, r - 1, r)
+    c = np.where(low, c + ncol, c)
+    high = c >= ncol
+    r = np.where(high, r + 1, r)
+    c = np.where(high, c - ncol, c)
+    return r, c
+
+
+def test_copy_row_of_flattened_index_is_exact():
+    rnd = np.random.default_rng(1)
+    ncols = np.unique(np.concatenate([np.arange(1, 300), rnd.integers(1, 1 << 20, 400),
+                                      [4000, 4001, 8000, 10007, 65535, 65536, (1 << 20) - 1]]))
+    for ncol in ncols:
+        ncol = int(ncol)
+        rows = np.unique(np.concatenate([np.arange(0, 64), rnd.integers(0, (2 ** 31 - 1) // ncol, 200)]))
+        rows = rows[rows * ncol < 2 ** 31]
+        # every position around each row start
+        e = (rows[:, None] * ncol + np.array([-1, 0, 1, ncol - 1])[None, :]).ravel()
+        e = e[(e >= 0) & (e < 2 ** 31)]
+        r, c = row_col(e, ncol)
+        assert np.array_equal(r, e // ncol) and np.array_equal(c, e % ncol), ncol
+
+
+def copy_tiles(descs, words=8):
+    """Tile partition of one CopyBatch (launch_copy): rows_per_tile and the
+    first tile of each descriptor."""
+    tile_words = 256 * 4 * 4
+    out, tiles = [], 0
+    for nrow, ncol in descs:
+        ncw = ncol * 8 // words
+        rpt = max(1, tile_words // ncw)
+        rpt = min(rpt, (2 ** 31 - 1) // ncw)
+        out.append((tiles, rpt, nrow))
+        tiles += -(-nrow // rpt)
+    return out, tiles
+
+
+def test_copy_batch_tiles_cover_every_row_once():
+    rnd = np.random.default_rng(2)
+    for _ in range(50):
+        descs = [(int(rnd.integers(1, 5000)), int(rnd.integers(1, 9000))) for _ in range(int(rnd.integers(1, 48)))]
+        parts, total = copy_tiles(descs)
+        cover = [np.zeros(nrow, int) for nrow, _ in descs]
+        for t in range(total):
+            lo = max(i for i, (t0, _, _) in enumerate(parts) if t0 <= t)   # the kernel's binary search
+            t0, rpt, nrow = parts[lo]
+            r0 = (t - t0) * rpt
+            nr = min(rpt, nrow - r0)
+            assert nr > 0 and r0 + nr <= nrow
+            cover[lo][r0:r0 + nr] += 1
+            assert nr * descs[lo][1] < 2 ** 31                      # 32-bit flattened index
+        assert all(np.all(c == 1) for c in cover)
+
+
+# --------------------------------------------------------------------------- redistribution
+GRIDS = [(8000, 8000, 8000, 24, [4, 2, 3], 0, 0), (10007, 10007, 10007, 24, [4, 2, 3], 0, 0),
+         (2000, 2000, 2000, 4, None, 0, 0), (1001, 1001, 1001, 24, [4, 2, 3], 1, 1),
+         (601, 433, 517, 24, [4, 2, 3], 1, 0), (700, 500, 900, 12, [2, 2, 3], 0, 1),
+         (300, 200, 250, 5, [2, 2, 1], 0, 0)]
+
+
+@pytest.mark.parametrize("m,n,k,p,grid,ta,tb", GRIDS)
+def test_one_pass_redistribution_blocks_inside_both_buffers(m, n, k, p, grid, ta, tb):
+    """redist_direct copies block b (row-major rect of a receive entry) from
+    the sender's source block to the receiver's destination block: it must
+    lie inside both rects, with leading dimensions no smaller than the rows
+    (row-major: ncol) of either buffer."""
+    w = O.ca3dmm_world(m, n, k, ta, tb, p, proc_grid=grid)
+    for name, red in w["redist"].items():
+        src, req = red["src"], red["req"]
+        moved = 0
+        for r in range(p):
+            _, _, rl, rc = red["plans"][r]
+            for b in rl:
+                s = src[int(b[4])]
+                q = req[r]
+                for rect in (s, q):
+                    assert rect[0] <= b[0] and b[0] + b[2] <= rect[0] + rect[2], (name, r, b, rect)
+                    assert rect[1] <= b[1] and b[1] + b[3] <= rect[1] + rect[3], (name, r, b, rect)
+                moved += int(b[2]) * int(b[3])
+            assert moved >= 0
+        # every requested element arrives exactly once
+        for r in range(p):
+            q = req[r]
+            area = int(q[2]) * int(q[3])
+            got = sum(int(b[2]) * int(b[3]) for b in red["plans"][r][2])
+            assert got == area, (name, r, got, area)
