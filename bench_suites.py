@@ -250,9 +250,10 @@ def grouped(args):
     d = float(np.max(np.abs(dC.numpy() - ref_c)))
     emit({"suite": "grouped", "mode": "grouped, untuned (rows=1, 1 WG/CU)", "ms": round(ms, 3),
           "tflops": round(flops / ms / 1e9, 2), "max_abs_diff_vs_per_launch": d})
-    g, wg = sim.autotune(2)
+    g, wg, tile = sim.autotune(2)
     ms = timed(run, reps=3)
-    emit({"suite": "grouped", "mode": f"grouped, tuned (rows={g}, {wg} WG/CU)", "ms": round(ms, 3),
+    tname = {0: "128x128", 1: "256x128", 2: "128x256"}[tile]
+    emit({"suite": "grouped", "mode": f"grouped, tuned (tile {tname}, rows={g}, {wg} WG/CU)", "ms": round(ms, 3),
           "tflops": round(flops / ms / 1e9, 2)})
     sim.set_mode(grouped=True, graph=True)
     ms = timed(run, reps=3)

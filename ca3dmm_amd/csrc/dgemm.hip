@@ -25,6 +25,7 @@
 //    in groups of 16 tile-rows, so the tiles resident on one XCD share A and
 //    B panels in that XCD's L2.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "ca3dmm_ext.h"
@@ -35,20 +36,20 @@ namespace {
 
 using namespace ca3::gemm;
 
-template <int TA, int TB>
-__global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(GemmArgs g)
+template <int TA, int TB, class TL>
+__global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void dgemm_kernel(GemmArgs g)
 {
-    __shared__ double As[2][kBK * kBM];
-    __shared__ double Bs[2][kBK * kBN];
+    __shared__ double As[2][kBK * TL::BM];
+    __shared__ double Bs[2][kBK * TL::BN];
 
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn);
-    const int m0 = tm * kBM, n0 = tn * kBN;
+    tile_coords<TL>(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn);
+    const int m0 = tm * TL::BM, n0 = tn * TL::BN;
 
     // A as stored (TA = 0) is contiguous along m; B as stored (TB = 0) is
     // contiguous along k, B^T along n.
-    OperandLoader<!TA> la;
-    OperandLoader<TB> lb;
+    OperandLoader<!TA, TL::BM, TL::THREADS> la;
+    OperandLoader<TB, TL::BN, TL::THREADS> lb;
     la.init(g.A, g.lda, m0, g.M);
     lb.init(g.B, g.ldb, n0, g.N);
 
@@ -58,19 +59,41 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    mainloop(la, lb, 0, g.K, m0 + kBM <= g.M && n0 + kBN <= g.N, As, Bs, acc);
+    mainloop<TL>(la, lb, 0, g.K, m0 + TL::BM <= g.M && n0 + TL::BN <= g.N, As, Bs, acc);
 
-    epilogue(g, m0, n0, acc);
+    epilogue<TL>(g, m0, n0, acc);
+}
+
+template <int TA, int TB, class TL>
+int launch_tile(const GemmArgs &g)
+{
+    const int tiles = ((g.M + TL::BM - 1) / TL::BM) * ((g.N + TL::BN - 1) / TL::BN);
+    dgemm_kernel<TA, TB, TL><<<tiles, TL::THREADS, 0, ca3::stream()>>>(g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ca3::fail(e, "dgemm_kernel");
+    return HIP_PROXY_OK;
+}
+
+// Workgroup tile: 128 x 128 (4 waves, two workgroups per CU) unless
+// CA3DMM_DGEMM_TILE selects 256x128 or 128x256 (8 waves, one per CU).
+int tile_choice()
+{
+    static int choice = -1;
+    if (choice < 0) {
+        const char *v = getenv("CA3DMM_DGEMM_TILE");
+        choice = (v && !strcmp(v, "256x128")) ? 1 : (v && !strcmp(v, "128x256")) ? 2 : 0;
+    }
+    return choice;
 }
 
 template <int TA, int TB>
 int launch(const GemmArgs &g)
 {
-    const int tiles = ((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
-    dgemm_kernel<TA, TB><<<tiles, kThreads, 0, ca3::stream()>>>(g);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return ca3::fail(e, "dgemm_kernel");
-    return HIP_PROXY_OK;
+    switch (tile_choice()) {
+    case 1: return launch_tile<TA, TB, Tile256x128>(g);
+    case 2: return launch_tile<TA, TB, Tile128x256>(g);
+    default: return launch_tile<TA, TB, Tile128>(g);
+    }
 }
 
 int run(hip_proxy_enum_t transA, hip_proxy_enum_t transB, int m, int n, int k, double alpha,

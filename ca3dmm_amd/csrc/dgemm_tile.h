@@ -23,6 +23,20 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int kBM = 128, kBN = 128, kBK = 16, kThreads = 256;
 constexpr int kMaxFoldSrc = 8;
 
+// Workgroup tile BM x BN of C: waves of 64 x 64 (4 x 4 MFMA blocks each),
+// WM along m and WN along n.  128 x 128 (4 waves, two workgroups per CU)
+// is the default; 256 x 128 and 128 x 256 use 8 waves, one workgroup per CU.
+template <int BM_, int BN_>
+struct TileShape {
+    static constexpr int BM = BM_, BN = BN_;
+    static constexpr int WM = BM / 64, WN = BN / 64;
+    static constexpr int NW = WM * WN, THREADS = 64 * NW;
+    static constexpr int MIN_BLOCKS = THREADS == 256 ? 2 : 1;     // per CU, for __launch_bounds__
+};
+using Tile128 = TileShape<128, 128>;
+using Tile256x128 = TileShape<256, 128>;
+using Tile128x256 = TileShape<128, 256>;
+
 struct GemmArgs {
     int M, N, K;
     double alpha, beta;
@@ -45,9 +59,10 @@ __device__ __forceinline__ int swz(int kk, int x)
 
 // Linear workgroup id -> (tile row, tile col): XCD-contiguous ranges, then
 // groups of 16 tile-rows walked column by column.
+template <class TL = Tile128>
 __device__ __forceinline__ void tile_coords(int bid, int nb, int M, int N, int *tm, int *tn)
 {
-    const int tiles_m = (M + kBM - 1) / kBM, tiles_n = (N + kBN - 1) / kBN;
+    const int tiles_m = (M + TL::BM - 1) / TL::BM, tiles_n = (N + TL::BN - 1) / TL::BN;
     (void) tiles_n;
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
     constexpr int G = 16;
@@ -60,26 +75,32 @@ __device__ __forceinline__ void tile_coords(int bid, int nb, int M, int N, int *
     *tn = r / gsz;
 }
 
-// Loads one 16 x 128 stage of an operand into registers and writes it to
-// LDS as [k][x].  XC = 1: the operand is contiguous along x (m or n);
-// XC = 0: contiguous along k.  Out-of-range elements read as 0.
-template <int XC>
+// Loads one 16 x W stage of an operand (W = the tile's extent along m or
+// n) with T threads into registers and writes it to LDS as [k][x].  XC = 1:
+// the operand is contiguous along x (m or n); XC = 0: contiguous along k.
+// Out-of-range elements read as 0.  Each thread holds P = 16 W / T elements:
+// XC = 1 at k rows kk0, kk0 + T/W, ...; XC = 0 at columns x, x + T/16, ...
+template <int XC, int W = 128, int T = 256>
 struct OperandLoader {
+    static constexpr int P = W * kBK / T;
+    static constexpr int KSTEP = T / W;         // XC: k rows per pass of the workgroup
+    static constexpr int XSTEP = T / kBK;       // !XC: columns per pass
+    static_assert(P >= 1 && P <= 16 && KSTEP * P == kBK && XSTEP * P == W, "loader shape");
     const double *base;   // element (x = this thread's first x, k = 0)
     int64_t ld;
-    int kk0;              // XC: first k row of this thread (0/1); !XC: its k row
+    int kk0;              // XC: first k row of this thread; !XC: its k row
     unsigned xmask;       // bit p: x of element p is in range
-    int xcol;             // XC: this thread's x
-    double r[8];
+    int xcol;             // this thread's (first) x
+    double r[P];
 
     __device__ __forceinline__ void init(const double *M, int64_t ld_, int x0, int xdim)
     {
         const int t = threadIdx.x;
         ld = ld_;
         if (XC) {
-            xcol = t & 127;
-            kk0 = t >> 7;
-            xmask = (x0 + xcol < xdim) ? 0xffu : 0u;
+            xcol = t % W;
+            kk0 = t / W;
+            xmask = (x0 + xcol < xdim) ? 0xffffu : 0u;
             base = M + (x0 + xcol);
         } else {
             kk0 = t & 15;
@@ -87,8 +108,8 @@ struct OperandLoader {
             xcol = xt;
             xmask = 0;
 #pragma unroll
-            for (int p = 0; p < 8; ++p)
-                if (x0 + xt + 16 * p < xdim) xmask |= 1u << p;
+            for (int p = 0; p < P; ++p)
+                if (x0 + xt + XSTEP * p < xdim) xmask |= 1u << p;
             base = M + (int64_t) (x0 + xt) * ld;
         }
     }
@@ -98,17 +119,17 @@ struct OperandLoader {
         if (XC) {
             const double *q = base + (int64_t) (k0 + kk0) * ld;
 #pragma unroll
-            for (int p = 0; p < 8; ++p) {
-                const bool ok = xmask && (k0 + kk0 + 2 * p < kend);
-                r[p] = ok ? q[(int64_t) (2 * p) * ld] : 0.0;
+            for (int p = 0; p < P; ++p) {
+                const bool ok = xmask && (k0 + kk0 + KSTEP * p < kend);
+                r[p] = ok ? q[(int64_t) (KSTEP * p) * ld] : 0.0;
             }
         } else {
             const bool kok = (k0 + kk0 < kend);
             const double *q = base + (k0 + kk0);
 #pragma unroll
-            for (int p = 0; p < 8; ++p) {
+            for (int p = 0; p < P; ++p) {
                 const bool ok = kok && ((xmask >> p) & 1u);
-                r[p] = ok ? q[(int64_t) (16 * p) * ld] : 0.0;
+                r[p] = ok ? q[(int64_t) (XSTEP * p) * ld] : 0.0;
             }
         }
     }
@@ -120,11 +141,11 @@ struct OperandLoader {
         if (XC) {
             const double *q = base + (int64_t) (k0 + kk0) * ld;
 #pragma unroll
-            for (int p = 0; p < 8; ++p) r[p] = q[(int64_t) (2 * p) * ld];
+            for (int p = 0; p < P; ++p) r[p] = q[(int64_t) (KSTEP * p) * ld];
         } else {
             const double *q = base + (k0 + kk0);
 #pragma unroll
-            for (int p = 0; p < 8; ++p) r[p] = q[(int64_t) (16 * p) * ld];
+            for (int p = 0; p < P; ++p) r[p] = q[(int64_t) (XSTEP * p) * ld];
         }
     }
 
@@ -132,33 +153,33 @@ struct OperandLoader {
     {
         if (XC) {
 #pragma unroll
-            for (int p = 0; p < 8; ++p) {
-                const int kk = kk0 + 2 * p;
-                S[kk * 128 + swz(kk, xcol)] = r[p];
+            for (int p = 0; p < P; ++p) {
+                const int kk = kk0 + KSTEP * p;
+                S[kk * W + swz(kk, xcol)] = r[p];
             }
         } else {
 #pragma unroll
-            for (int p = 0; p < 8; ++p) S[kk0 * 128 + swz(kk0, xcol + 16 * p)] = r[p];
+            for (int p = 0; p < P; ++p) S[kk0 * W + swz(kk0, xcol + XSTEP * p)] = r[p];
         }
     }
 };
 
 // k-slices [KS0, KS1) of the 16 k-steps of MFMAs on the LDS stage (4 x
 // v_mfma_f64_16x16x4 k-slices in all).
-template <int KS0 = 0, int KS1 = kBK / 4>
+template <class TL = Tile128, int KS0 = 0, int KS1 = kBK / 4>
 __device__ __forceinline__ void stage_mfma(const double *As, const double *Bs, f64x4 (&acc)[4][4])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % TL::WM, wn = wave / TL::WM;
     const int li = lane & 15;
 #pragma unroll
     for (int ks = KS0; ks < KS1; ++ks) {
         const int kk = ks * 4 + (lane >> 4);
         double a[4], b[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = As[kk * kBM + swz(kk, wm * 64 + i * 16 + li)];
+        for (int i = 0; i < 4; ++i) a[i] = As[kk * TL::BM + swz(kk, wm * 64 + i * 16 + li)];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = Bs[kk * kBN + swz(kk, wn * 64 + j * 16 + li)];
+        for (int j = 0; j < 4; ++j) b[j] = Bs[kk * TL::BN + swz(kk, wn * 64 + j * 16 + li)];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -167,11 +188,11 @@ __device__ __forceinline__ void stage_mfma(const double *As, const double *Bs, f
     }
 }
 
-// Accumulate op(A)[m0:+128, kbeg:kend) * op(B)[kbeg:kend, n0:+128) into acc
+// Accumulate op(A)[m0:+BM, kbeg:kend) * op(B)[kbeg:kend, n0:+BN) into acc
 // with bounds checks on every element (edge tiles and partial k-stages).
-template <class LA, class LB>
+template <class TL = Tile128, class LA, class LB>
 __device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int kend,
-                                                 double (*As)[kBK * kBM], double (*Bs)[kBK * kBN],
+                                                 double (*As)[kBK * TL::BM], double (*Bs)[kBK * TL::BN],
                                                  f64x4 (&acc)[4][4])
 {
     if (kend <= kbeg) return;
@@ -188,7 +209,7 @@ __device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int k
             la.load(kbeg + (kt + 1) * kBK, kend);
             lb.load(kbeg + (kt + 1) * kBK, kend);
         }
-        stage_mfma(As[cur], Bs[cur], acc);
+        stage_mfma<TL>(As[cur], Bs[cur], acc);
         if (more) {
             la.store(As[cur ^ 1]);
             lb.store(Bs[cur ^ 1]);
@@ -198,14 +219,14 @@ __device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int k
 }
 
 // The same accumulation.  A tile that is interior in m and n (interior =
-// m0 + 128 <= M and n0 + 128 <= N) runs its whole k-stages through a loop
+// m0 + BM <= M and n0 + BN <= N) runs its whole k-stages through a loop
 // without predicates or data-dependent branches (one register stage in
 // flight, the last stage's load repeated in bounds and never consumed);
 // edge tiles and the k tail go through mainloop_checked.  Both add the
 // k-steps in the same order, so results do not depend on the path.
-template <class LA, class LB>
+template <class TL = Tile128, class LA, class LB>
 __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, bool interior,
-                                         double (*As)[kBK * kBM], double (*Bs)[kBK * kBN],
+                                         double (*As)[kBK * TL::BM], double (*Bs)[kBK * TL::BN],
                                          f64x4 (&acc)[4][4])
 {
     const int nfull = (interior && kend > kbeg) ? (kend - kbeg) / kBK : 0;
@@ -224,22 +245,23 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, boo
             // one's MFMAs (its loads have had half a stage to land), so the
             // LDS writes complete under the second half instead of before
             // the barrier
-            stage_mfma<0, kBK / 8>(As[cur], Bs[cur], acc);
+            stage_mfma<TL, 0, kBK / 8>(As[cur], Bs[cur], acc);
             la.store(As[cur ^ 1]);
             lb.store(Bs[cur ^ 1]);
-            stage_mfma<kBK / 8, kBK / 4>(As[cur], Bs[cur], acc);
+            stage_mfma<TL, kBK / 8, kBK / 4>(As[cur], Bs[cur], acc);
             __syncthreads();
         }
     }
     const int kt0 = kbeg + nfull * kBK;
-    if (kt0 < kend) mainloop_checked(la, lb, kt0, kend, As, Bs, acc);
+    if (kt0 < kend) mainloop_checked<TL>(la, lb, kt0, kend, As, Bs, acc);
 }
 
 // C := fold(P) + (alpha*acc + beta*C) for the tile at (m0, n0).
+template <class TL = Tile128>
 __device__ __forceinline__ void epilogue(const GemmArgs &g, int m0, int n0, const f64x4 (&acc)[4][4])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % TL::WM, wn = wave / TL::WM;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int row = m0 + wm * 64 + i * 16 + (lane & 15);

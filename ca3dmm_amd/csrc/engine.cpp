@@ -1416,6 +1416,7 @@ std::vector<uintptr_t> sequence_key(ca3dmm_sim *S, const void *const *src_A, con
     k.push_back((uintptr_t) redist_staged());
     if (S->plan) {
         const ca3::grouped::Tuning &t = ca3::grouped::tuning(S->plan);
+        k.push_back((uintptr_t) t.tile);
         k.push_back((uintptr_t) t.group_rows);
         k.push_back((uintptr_t) t.wg_per_cu);
     }
@@ -1590,7 +1591,7 @@ int ca3dmm_sim_set_tuning_ozaki(ca3dmm_sim_p sim, int tile_n, int xcd_queue)
     return HIP_PROXY_OK;
 }
 
-int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu)
+int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu, int *tile)
 {
     if (!sim) return HIP_PROXY_ERR_ARG;
     for (auto e : sim->eng)
@@ -1601,15 +1602,19 @@ int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per
     const ca3::grouped::Tuning &t = ca3::grouped::tuning(sim->plan);
     if (group_rows) *group_rows = t.group_rows;
     if (wg_per_cu) *wg_per_cu = t.wg_per_cu;
+    if (tile) *tile = t.tile;
     return hip_stream_sync(nullptr);
 }
 
-int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu)
+int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu, int tile)
 {
-    if (!sim || group_rows < 1 || wg_per_cu < 1 || wg_per_cu > 2) return HIP_PROXY_ERR_ARG;
+    if (!sim || group_rows < 1 || wg_per_cu < 1 || wg_per_cu > 2 || tile < 0 || tile > 2 ||
+        (tile != 0 && wg_per_cu != 1))
+        return HIP_PROXY_ERR_ARG;
     int rc = 0;
     if (!sim->plan && (rc = build_grouped(sim))) return rc;
     ca3::grouped::Tuning t;
+    t.tile = tile;
     t.group_rows = group_rows;
     t.wg_per_cu = wg_per_cu;
     ca3::grouped::set_tuning(sim->plan, t);

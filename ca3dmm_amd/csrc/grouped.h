@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <string>
 #include <vector>
 
 namespace ca3 {
@@ -38,8 +39,9 @@ struct Fold {                 // k-layer reduction of the C blocks of kp problem
 };
 
 struct Tuning {
+    int tile = 0;             // workgroup tile: 0 = 128 x 128, 1 = 256 x 128, 2 = 128 x 256
     int group_rows = 8;       // tile rows walked together in the queue order
-    int wg_per_cu = 2;        // persistent workgroups per CU
+    int wg_per_cu = 2;        // persistent workgroups per CU (1 for the 8-wave tiles)
 };
 
 struct Plan;
@@ -64,6 +66,12 @@ struct OzTuning {
     int xcd_queue = 1;        // 1: items in 8 per-XCD ranges; 0: one counter over all items
 };
 
+// The tuning table shared by both forms: in memory, and on disk at
+// CA3DMM_TUNE_CACHE (default ~/.cache/ca3dmm_amd/tuning.txt), one line per
+// problem signature; lookups read the file once per process.
+bool lookup_tuning(const char *form, const std::string &sig, Tuning *t);
+void store_tuning(const char *form, const std::string &sig, const Tuning &t);
+
 struct OzPlan;
 int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &folds, int nmod, OzPlan **out);
 // Time the candidate tunings of the grouped launch (tile 256 / 128 x queue
@@ -76,6 +84,8 @@ void set_tuning_ozaki(OzPlan *p, const OzTuning &t);
 // sequence from a captured hipGraph.
 int run_ozaki(OzPlan *p, bool grouped, bool use_graph);
 void destroy_ozaki(OzPlan *p);
+bool lookup_tuning_ozaki(const std::string &sig, OzTuning *t);
+void store_tuning_ozaki(const std::string &sig, const OzTuning &t);
 
 // A whole enqueued sequence (every kernel `body` launches on the library
 // stream) captured once into a hipGraph and replayed.  body must not

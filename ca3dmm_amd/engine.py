@@ -114,8 +114,8 @@ _SIGS = {
     "ca3dmm_sim_c_out": (P, [P, I]),
     "ca3dmm_sim_free": (None, [C.POINTER(P)]),
     "ca3dmm_sim_set_mode": (I, [P, I, I]),
-    "ca3dmm_sim_autotune": (I, [P, I, IP, IP]),
-    "ca3dmm_sim_set_tuning": (I, [P, I, I]),
+    "ca3dmm_sim_autotune": (I, [P, I, IP, IP, IP]),
+    "ca3dmm_sim_set_tuning": (I, [P, I, I, I]),
     "ca3dmm_sim_set_ozaki": (I, [P, I]),
     "ca3dmm_sim_autotune_ozaki": (I, [P, I, IP, IP]),
     "ca3dmm_sim_set_tuning_ozaki": (I, [P, I, I]),
@@ -168,9 +168,11 @@ class SimGrid:
                      "ca3dmm_sim_set_mode")
 
     def autotune(self, reps=3):
-        g, w = C.c_int(), C.c_int()
-        check(lib.ca3dmm_sim_autotune(self.h, reps, C.byref(g), C.byref(w)), "ca3dmm_sim_autotune")
-        return g.value, w.value
+        """Tune the FP64 grouped launch; returns (group_rows, wg_per_cu, tile)
+        with tile 0 = 128x128, 1 = 256x128, 2 = 128x256."""
+        g, w, t = C.c_int(), C.c_int(), C.c_int()
+        check(lib.ca3dmm_sim_autotune(self.h, reps, C.byref(g), C.byref(w), C.byref(t)), "ca3dmm_sim_autotune")
+        return g.value, w.value, t.value
 
     def set_ozaki(self, nmoduli):
         """nmoduli > 0: the local products through the CRT form (0: FP64 MFMA)."""
@@ -186,8 +188,8 @@ class SimGrid:
         return check(lib.ca3dmm_sim_set_tuning_ozaki(self.h, tile_n, 1 if xcd_queue else 0),
                      "ca3dmm_sim_set_tuning_ozaki")
 
-    def set_tuning(self, group_rows, wg_per_cu):
-        return check(lib.ca3dmm_sim_set_tuning(self.h, group_rows, wg_per_cu), "ca3dmm_sim_set_tuning")
+    def set_tuning(self, group_rows, wg_per_cu, tile=0):
+        return check(lib.ca3dmm_sim_set_tuning(self.h, group_rows, wg_per_cu, tile), "ca3dmm_sim_set_tuning")
 
     def c_out(self, r):
         return lib.ca3dmm_sim_c_out(self.h, r)

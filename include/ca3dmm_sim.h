@@ -54,19 +54,25 @@ int ca3dmm_sim_exec(ca3dmm_sim_p sim, const void *const *src_A, const int *ldA,
 void *ca3dmm_sim_c_out(ca3dmm_sim_p sim, int rank);
 
 /* grouped != 0: run the Cannon rounds of all ranks and the k-layer fold as
- * ONE persistent launch pulling 128 x 128 tiles from a device work queue
- * (rounds accumulated in registers; the last layer to finish a tile adds the
- * kp layer tiles in layer order).  use_graph != 0: replay that launch from a
- * captured hipGraph.  Default: one DGEMM launch per rank and round. */
+ * ONE persistent launch pulling tiles of every rank's C from a device work
+ * queue (rounds accumulated in registers; the last layer to finish a tile
+ * adds the kp layer tiles in layer order).  use_graph != 0: capture the whole
+ * exec -- redistributions, transposes, allgathers, products, fold -- into one
+ * hipGraph on the first call (again when an argument or setting changes)
+ * and replay it.  Default: one DGEMM launch per rank and round, eager. */
 int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph);
 
-/* Time the grouped launch over its tunings (tile rows walked together in
- * the queue order: 1, 4, 8, 16; persistent workgroups per CU: 1, 2), keep
- * the fastest for this grid's shape signature and report it. */
-int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu);
+/* Time the grouped launch over its tunings -- workgroup tile 128 x 128 (4
+ * waves), 256 x 128 or 128 x 256 (8 waves); tile rows walked together in the
+ * queue order: 1, 4, 8, 16; persistent workgroups per CU: 1, 2 (128 x 128
+ * only) -- keep the fastest for this grid's shape signature, in memory and
+ * in the tuning table on disk (CA3DMM_TUNE_CACHE, default
+ * ~/.cache/ca3dmm_amd/tuning.txt; a later plan of the same signature starts
+ * from it), and report it.  tile (may be NULL): 0, 1, 2 as above. */
+int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu, int *tile);
 
 /* Fix the grouped launch's tuning by hand (e.g. the untuned baseline). */
-int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu);
+int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu, int tile);
 
 /* nmoduli > 0 (4..25): every rank's local product -- all its Cannon rounds
  * -- runs through the CRT form of the Ozaki scheme (hip_ozaki2_dgemm's

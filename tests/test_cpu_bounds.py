@@ -42,7 +42,7 @@ def _src(name):
 def loader_xc(text, var, flag):
     """XC of `OperandLoader<...> var` in a kernel source: 1, 0, or for a
     template flag ('TA'/'TB') a function of it."""
-    m = re.search(r"OperandLoader<\s*(!?)\s*(\w+)\s*>\s*" + var + r"\s*;", text)
+    m = re.search(r"OperandLoader<\s*(!?)\s*(\w+)\s*(?:,[^>]*)?>\s*" + var + r"\s*;", text)
     assert m, f"no OperandLoader declaration for {var}"
     neg, arg = m.group(1) == "!", m.group(2)
     if arg in ("0", "1"):
@@ -51,13 +51,13 @@ def loader_xc(text, var, flag):
     return (lambda t: 1 - t) if neg else (lambda t: t)
 
 
-def operand_extremes(xc, xdim, K, ld, x0s, kspans):
+def operand_extremes(xc, xdim, K, ld, x0s, kspans, W=128):
     """Largest element offset a loader touches: XC = 1: x + k ld, XC = 0:
-    k + x ld, for x in [x0, x0 + 128) clipped to xdim (the predicate) and the
+    k + x ld, for x in [x0, x0 + W) clipped to xdim (the predicate) and the
     k ranges of its stages."""
     hi = 0
     for x0, (k0, k1) in zip(x0s, kspans):
-        xl = min(xdim, x0 + 128) - 1
+        xl = min(xdim, x0 + W) - 1
         kl = k1 - 1
         if xl < 0 or kl < 0:
             continue
@@ -65,11 +65,11 @@ def operand_extremes(xc, xdim, K, ld, x0s, kspans):
     return hi
 
 
-def stage_spans(M, N, K, m0, n0):
+def stage_spans(M, N, K, m0, n0, bm=BM, bn=BN):
     """k ranges the loaders read for tile (m0, n0) of mainloop: interior
     tiles read whole 16-deep stages unpredicated (the last one repeated),
     then the predicated tail; edge tiles are predicated throughout."""
-    interior = m0 + BM <= M and n0 + BN <= N
+    interior = m0 + bm <= M and n0 + bn <= N
     nfull = K // BK if interior else 0
     spans = []
     if nfull:
@@ -84,24 +84,45 @@ DGEMM_SHAPES = [(1, 1, 1), (15, 17, 3), (129, 130, 17), (257, 64, 1000), (64, 51
                 (8192, 8192, 8192), (8192, 128, 8192), (128, 8192, 8192)]
 
 
+TILES = [(128, 128), (256, 128), (128, 256)]      # TileShape instances of the DGEMM and grouped kernels
+
+
+def test_loader_thread_layouts_cover_each_stage_once():
+    """OperandLoader<XC, W, T>: the (k, x) elements of one 16 x W stage the T
+    threads load (P each; XC = 1 at k rows kk0 + KSTEP p, XC = 0 at columns
+    x + XSTEP p) cover the stage exactly once, for every W / T in use."""
+    for W, T in [(128, 256), (256, 512), (128, 512)]:
+        P, KSTEP, XSTEP = W * BK // T, T // W, T // BK
+        for xc in (0, 1):
+            seen = np.zeros((BK, W), int)
+            for t in range(T):
+                for p in range(P):
+                    if xc:
+                        seen[t // W + KSTEP * p, t % W] += 1
+                    else:
+                        seen[t & 15, (t >> 4) + XSTEP * p] += 1
+            assert np.all(seen == 1), (W, T, xc)
+
+
+@pytest.mark.parametrize("bm,bn", TILES)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_dgemm_loaders_stay_inside_the_operands(ta, tb):
+def test_dgemm_loaders_stay_inside_the_operands(ta, tb, bm, bn):
     text = _src("dgemm.hip")
     xa, xb = loader_xc(text, "la", "TA")(ta), loader_xc(text, "lb", "TB")(tb)
     for M, N, K in DGEMM_SHAPES:
         lda, ldb = (K if ta else M), (N if tb else K)          # tight leading dims
         alloc_a = lda * (M if ta else K)
         alloc_b = ldb * (K if tb else N)
-        tiles = [(tm * BM, tn * BN) for tm in range(-(-M // BM)) for tn in range(-(-N // BN))]
+        tiles = [(tm * bm, tn * bn) for tm in range(-(-M // bm)) for tn in range(-(-N // bn))]
         for m0, n0 in tiles:
-            sp = stage_spans(M, N, K, m0, n0)
+            sp = stage_spans(M, N, K, m0, n0, bm, bn)
             for k0, k1 in sp:
-                # full stages read every element of the 128 x 16 block: they
-                # must be whole and in range on both axes
-                if (k0, k1) == sp[0] and m0 + BM <= M and n0 + BN <= N and k1 - k0 >= BK:
-                    assert k1 <= K and m0 + BM <= M and n0 + BN <= N
-            ha = operand_extremes(xa, M, K, lda, [m0] * len(sp), sp)
-            hb = operand_extremes(xb, N, K, ldb, [n0] * len(sp), sp)
+                # full stages read every element of the bm (bn) x 16 block:
+                # they must be whole and in range on both axes
+                if (k0, k1) == sp[0] and m0 + bm <= M and n0 + bn <= N and k1 - k0 >= BK:
+                    assert k1 <= K and m0 + bm <= M and n0 + bn <= N
+            ha = operand_extremes(xa, M, K, lda, [m0] * len(sp), sp, bm)
+            hb = operand_extremes(xb, N, K, ldb, [n0] * len(sp), sp, bn)
             assert ha < alloc_a, (ta, tb, M, N, K, m0, ha, alloc_a)
             assert hb < alloc_b, (ta, tb, M, N, K, n0, hb, alloc_b)
 
@@ -112,12 +133,13 @@ def test_grouped_and_gram_loaders_read_their_blocks_as_stored():
     a k x n A along k.  Any other XC would leave the blocks."""
     g = _src("grouped.hip")
     assert loader_xc(g, "la", "-")(0) == 1 and loader_xc(g, "lb", "-")(0) == 0
-    for M, N, K in [(2501, 5003, 1668), (2502, 5004, 1667), (1000, 500, 334), (3, 5, 7)]:
-        for m0 in range(0, M, BM):
-            for n0 in range(0, N, BN):
-                sp = stage_spans(M, N, K, m0, n0)
-                assert operand_extremes(1, M, K, M, [m0] * len(sp), sp) < M * K
-                assert operand_extremes(0, N, K, K, [n0] * len(sp), sp) < K * N
+    for bm, bn in TILES:
+        for M, N, K in [(2501, 5003, 1668), (2502, 5004, 1667), (1000, 500, 334), (3, 5, 7)]:
+            for m0 in range(0, M, bm):
+                for n0 in range(0, N, bn):
+                    sp = stage_spans(M, N, K, m0, n0, bm, bn)
+                    assert operand_extremes(1, M, K, M, [m0] * len(sp), sp, bm) < M * K
+                    assert operand_extremes(0, N, K, K, [n0] * len(sp), sp, bn) < K * N
     gram = _src("gram.hip")
     m = re.search(r"OperandLoader<(\d)>\s+la,\s*lb;", gram)
     assert m and m.group(1) == "0"
@@ -127,9 +149,9 @@ def test_grouped_and_gram_loaders_read_their_blocks_as_stored():
 
 
 # --------------------------------------------------------------------------- tile order
-def tile_coords(bid, nb, M, N):
+def tile_coords(bid, nb, M, N, bm=BM, bn=BN):
     """dgemm_tile.h tile_coords, restated."""
-    tiles_m, tiles_n = -(-M // BM), -(-N // BN)
+    tiles_m, tiles_n = -(-M // bm), -(-N // bn)
     if nb % 8 == 0:
         bid = (bid & 7) * (nb >> 3) + (bid >> 3)
     G = 16
@@ -141,12 +163,13 @@ def tile_coords(bid, nb, M, N):
     return first + r % gsz, r // gsz
 
 
+@pytest.mark.parametrize("bm,bn", TILES)
 @pytest.mark.parametrize("M,N", [(128, 128), (1000, 1000), (2000, 4000), (8192, 8192), (2501, 5003),
                                  (130, 9000), (9000, 130), (17 * 128, 3 * 128), (100, 100)])
-def test_tile_order_is_a_bijection(M, N):
-    tm_n, tn_n = -(-M // BM), -(-N // BN)
+def test_tile_order_is_a_bijection(M, N, bm, bn):
+    tm_n, tn_n = -(-M // bm), -(-N // bn)
     nb = tm_n * tn_n
-    seen = {tile_coords(b, nb, M, N) for b in range(nb)}
+    seen = {tile_coords(b, nb, M, N, bm, bn) for b in range(nb)}
     assert seen == {(i, j) for i in range(tm_n) for j in range(tn_n)}
 
 

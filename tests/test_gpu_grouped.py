@@ -198,3 +198,44 @@ def test_sequence_graph_recaptured_for_new_arguments(hip):
             assert np.array_equal(dC.numpy(), A @ B), it
         del dA, dB
     sim.free()
+
+
+@pytest.mark.parametrize("tile,wg", [(0, 1), (0, 2), (1, 1), (2, 1)])
+def test_grouped_every_tile_shape_exact(hip, tile, wg):
+    """The grouped launch under each workgroup tile (128x128 with one or two
+    workgroups per CU, 256x128, 128x256): bit-exact on integers over a grid
+    with kp = 3 folds and uneven cuboids."""
+    from ca3dmm_amd import device as dev
+    from ca3dmm_amd import engine as E
+    from oracle import oracle as O
+    rnd = np.random.default_rng(80 + tile)
+    m, n, k, p, grid = 1001, 777, 905, 24, [4, 2, 3]
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=grid)
+    sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=grid, allocate=True)
+    A, B = _ints(rnd, (m, k)), _ints(rnd, (k, n))
+    dA, dB = dev.DeviceArray.from_numpy(A), dev.DeviceArray.from_numpy(B)
+    dC = dev.DeviceArray.filled((m, n), 0xFF)
+    Ap = [dA.at(r[2] * m) for r in w["src_A"]]
+    Bp = [dB.at(r[2] * k) for r in w["src_B"]]
+    Cp = [dC.at(r[2] * m) for r in w["dst_C"]]
+    sim.set_mode(grouped=True, graph=False)
+    sim.set_tuning(4, wg, tile)
+    sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
+    assert np.array_equal(dC.numpy(), A @ B)
+    sim.free()
+
+
+def test_autotune_table_persists(hip, tmp_path, monkeypatch):
+    """The tuner's pick is written to the tuning table and a later plan of the
+    same shape starts from it (here: a second simulated grid in the same
+    process reads the table)."""
+    monkeypatch.setenv("CA3DMM_TUNE_CACHE", str(tmp_path / "tuning.txt"))
+    from ca3dmm_amd import engine as E
+    from oracle import oracle as O
+    m, n, k, p, grid = 640, 520, 700, 12, [2, 2, 3]
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=grid)
+    sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=grid, allocate=True)
+    g, wg, tile = sim.autotune(1)
+    sim.free()
+    text = (tmp_path / "tuning.txt").read_text()
+    assert f"tile={tile} rows={g} wg={wg}" in text and text.startswith("fp64 ")
