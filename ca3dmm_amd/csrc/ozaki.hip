@@ -54,6 +54,20 @@ constexpr int kKb = 64;        // k bytes per MFMA step (one LDS sub-block)
 constexpr int kKp = 256;       // k padding and k-chunk granularity (whole pipeline rounds)
 constexpr int kMaxSlices = 16;
 constexpr int kExpEmpty = INT_MIN;
+// A line holding an Inf or a NaN: it is sliced as a zero line (no undefined
+// conversions), and every C entry of its row (op(A)) or column (op(B)) is
+// written as NaN.  Bits of |x| at or above this mark are Inf or NaN.
+constexpr int kExpNonFinite = INT_MIN + 1;
+constexpr unsigned long long kNonFiniteBits = 0x7ff0000000000000ull;
+
+// v * 2^(ea + eb) in one step (one rounding, no intermediate overflow): 0
+// for an empty line, NaN for a line with a non-finite element.
+__device__ __forceinline__ double oz_unscale(double v, int ea, int eb)
+{
+    if (ea == kExpNonFinite || eb == kExpNonFinite) return __builtin_nan("");
+    if (ea == kExpEmpty || eb == kExpEmpty) return 0.0;
+    return ldexp(v, ea + eb);
+}
 
 // ---------------------------------------------------------------------
 // 1. line exponents
@@ -61,6 +75,8 @@ constexpr int kExpEmpty = INT_MIN;
 // Line l of the operand: element (l, kk) at X[l*sl + kk*sk].  mx[l] holds
 // the bits of the largest |x| seen (non-negative doubles order like their
 // bit patterns, so an integer atomicMax is exact); 0 means an all-zero line.
+// The maximum is taken on the bits, so an Inf (0x7ff0...) or a NaN (above
+// it) is kept: bits >= kNonFiniteBits mark a non-finite line.
 // Block (bx, by) of a gx x gy grid: 64 lines x (K / gy chunk); the thread
 // layout follows the contiguous direction so loads coalesce.
 __device__ __forceinline__ void oz_max_body(int bx, int by, int gy, const double *__restrict__ X, int lines, int K,
@@ -80,26 +96,27 @@ __device__ __forceinline__ void oz_max_body(int bx, int by, int gy, const double
             const int l = l0 + w * 16 + r;
             if (l >= lines) break;
             const double *xl = X + (int64_t) l * sl;
-            double m = 0.0;
-            for (int kk = k0 + lane; kk < k1; kk += 64) m = fmax(m, fabs(xl[kk]));
+            unsigned long long m = 0;
+            for (int kk = k0 + lane; kk < k1; kk += 64)
+                m = max(m, (unsigned long long) __double_as_longlong(fabs(xl[kk])));
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-            if (lane == 0 && m > 0.0) atomicMax(&mx[l], (unsigned long long) __double_as_longlong(m));
+            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+            if (lane == 0 && m != 0) atomicMax(&mx[l], m);
         }
         return;
     }
-    __shared__ double red[256];
-    double m = 0.0;
+    __shared__ unsigned long long red[256];
+    unsigned long long m = 0;
     const int myline = l0 + (t & 63);
     for (int kk = k0 + (t >> 6); kk < k1; kk += 4)
-        if (myline < lines) m = fmax(m, fabs(X[myline + (int64_t) kk * sk]));
+        if (myline < lines) m = max(m, (unsigned long long) __double_as_longlong(fabs(X[myline + (int64_t) kk * sk])));
     red[t] = m;
     __syncthreads();
     if (t < 64) {
         const int l = l0 + t;
-        double v = 0.0;
-        for (int j = 0; j < 4; ++j) v = fmax(v, red[t + 64 * j]);
-        if (l < lines && v > 0.0) atomicMax(&mx[l], (unsigned long long) __double_as_longlong(v));
+        unsigned long long v = 0;
+        for (int j = 0; j < 4; ++j) v = max(v, red[t + 64 * j]);
+        if (l < lines && v != 0) atomicMax(&mx[l], v);
     }
 }
 
@@ -109,11 +126,16 @@ __global__ __launch_bounds__(256) void ozaki_max_kernel(const double *__restrict
     oz_max_body(blockIdx.x, blockIdx.y, gridDim.y, X, lines, K, sl, sk, mx);
 }
 
-// E[l] = kExpEmpty for a zero line, else e + 1 or e + 2 as above.
+// E[l] = kExpEmpty for a zero line, kExpNonFinite for a line with an Inf or
+// a NaN, else e + 1 or e + 2 as above.
 __global__ void ozaki_scale_kernel(const unsigned long long *__restrict__ mx, int n, int *__restrict__ E)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
+    if (mx[l] >= kNonFiniteBits) {
+        E[l] = kExpNonFinite;
+        return;
+    }
     const double v = __longlong_as_double((long long) mx[l]);
     if (v == 0.0) {
         E[l] = kExpEmpty;
@@ -187,8 +209,9 @@ __global__ __launch_bounds__(256) void ozaki_slice_kernel(const double *__restri
     __syncthreads();
     const int li = t >> 2, kq = (t & 3) * 16;
     const int l = l0 + li;
-    const int E = (l < lines && ex[l] != kExpEmpty) ? ex[l] : 0;
-    const bool zero_line = (l >= lines) || ex[min(l, lines - 1)] == kExpEmpty;
+    const int exl = ex[min(l, lines - 1)];
+    const bool zero_line = (l >= lines) || exl == kExpEmpty || exl == kExpNonFinite;
+    const int E = zero_line ? 0 : exl;
     uint64_t U[16];
     int off[16];
 #pragma unroll
@@ -456,8 +479,7 @@ __device__ __forceinline__ void oz_epilogue(const OzArgs &g, const i32x4 (&acc)[
                     *wp = v;
                     continue;
                 }
-                const int eb = g.eB[col];
-                v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
+                v = oz_unscale(v, ea, g.eB[col]);
                 double *cp = g.C + (int64_t) col * g.ldc + row;
                 v = ca3::mul_rn(g.alpha, v);
                 if (g.beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(g.beta, *cp));
@@ -506,9 +528,8 @@ __global__ __launch_bounds__(256) void ozaki_finalize_kernel(const double *__res
     const int i = blockIdx.x * 256 + threadIdx.x;
     for (int j = blockIdx.y; j < N; j += gridDim.y) {
         if (i >= M) return;
-        const int ea = eA[i], eb = eB[j];
-        double v = W[(int64_t) j * ldw + i];
-        v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(v, ea), eb) : 0.0;
+        const double v0 = W[(int64_t) j * ldw + i];
+        double v = oz_unscale(v0, eA[i], eB[j]);
         double *cp = C + (int64_t) j * ldc + i;
         v = ca3::mul_rn(alpha, v);
         if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
@@ -897,6 +918,7 @@ __device__ __forceinline__ void oz2_norm_body(int bx, int by, int gy, const doub
         for (int r = 0; r < 16; ++r) {
             const int l = l0 + w * 16 + r;
             if (l >= lines) break;
+            if (mx[l] >= kNonFiniteBits) continue;          // flagged by its maximum; no norm needed
             int E;
             (void) frexp(__longlong_as_double((long long) mx[l]), &E);
             const double *xl = X + (int64_t) l * sl;
@@ -911,7 +933,7 @@ __device__ __forceinline__ void oz2_norm_body(int bx, int by, int gy, const doub
     __shared__ unsigned long long red[256];
     unsigned long long acc = 0;
     const int myline = l0 + (t & 63);
-    if (myline < lines) {
+    if (myline < lines && mx[myline] < kNonFiniteBits) {
         int E;
         (void) frexp(__longlong_as_double((long long) mx[myline]), &E);
         for (int kk = k0 + (t >> 6); kk < k1; kk += 4) acc += crt_norm_term(X[myline + (int64_t) kk * sk], E);
@@ -938,7 +960,8 @@ __global__ __launch_bounds__(256) void ozaki2_norm_kernel(const double *__restri
 // S <= 4^h - ceil(2897 2^(2h - T)): then 4^(T-h) S <= (2^T - 2897/2)^2, so
 // 2^e ||line||_2 plus the rounding slack sqrt(k)/2 <= 2897/2 (k <= 2^23)
 // stays within 2^T.  The search starts at ceil(bitlen(S - 1) / 2) - 1,
-// below which 4^h < S.  kExpEmpty for a zero line.
+// below which 4^h < S.  kExpEmpty for a zero line, kExpNonFinite for a line
+// with an Inf or a NaN.
 constexpr int kCrtSlack = 2897;     // ceil(sqrt(kCrtKMax))
 static_assert((long long) kCrtSlack * kCrtSlack >= kCrtKMax &&
                   (long long) (kCrtSlack - 1) * (kCrtSlack - 1) < kCrtKMax,
@@ -948,6 +971,10 @@ __global__ void ozaki2_scale_kernel(const unsigned long long *__restrict__ mx, c
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
+    if (mx[l] >= kNonFiniteBits) {
+        e[l] = kExpNonFinite;
+        return;
+    }
     const double v = __longlong_as_double((long long) mx[l]);
     if (v == 0.0) {
         e[l] = kExpEmpty;
@@ -1022,7 +1049,7 @@ __device__ __forceinline__ void oz2_residue_tiles(int tau0, int step, int ntiles
         const int l0 = (tau % nbx) * 64, k0 = (tau / nbx) * 64;
         const int li = t >> 2, kq = (t & 3) * 16;
         const int l = l0 + li;
-        const bool zero_line = (l >= lines) || exc == kExpEmpty;
+        const bool zero_line = (l >= lines) || exc == kExpEmpty || exc == kExpNonFinite;
         const int e = zero_line ? 0 : exc;
         double ph[16], pm[16], pl[16];                  // NP = 2: (-, h, l); NP = 3: (h2, h1, l)
 #pragma unroll
@@ -1195,7 +1222,8 @@ __device__ __forceinline__ void oz_epilogue_crt(const OzArgs &g, const i32x4 (&a
 // 2^35 and the rounded quotient errs by far less than 1/(2 Q_i), so rint
 // finds the balanced remainder; floor with the offset 1/2 + 1/(2 Q_0) gives
 // [-Q_0/2, Q_0/2) for the even Q_0).  Then x = V_0 + Q_0 (V_1 + Q_1 (...))
-// in FP64 from the top digit, and 2^-eA 2^-eB x (0 for an empty line).
+// in FP64 from the top digit, and 2^-(eA + eB) x (0 for an empty line, NaN
+// for a line with a non-finite element).
 template <int NM>
 __device__ __forceinline__ void oz2_crt_block(const int8_t *__restrict__ R, int64_t rplane, int Mp, int rb, int cb,
                                               int lane, const int *__restrict__ eA, const int *__restrict__ eB,
@@ -1233,7 +1261,8 @@ __device__ __forceinline__ void oz2_crt_block(const int8_t *__restrict__ R, int6
 #pragma unroll
         for (int i = NC - 2; i >= 0; --i) x = ca3::add_rn(ca3::mul_rn(x, c_crt.q[i]), V[i]);
         const int eb = eB[cb * 16 + 4 * (lane >> 4) + q];
-        out[q] = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(x, -ea), -eb) : 0.0;
+        out[q] = (ea == kExpNonFinite || eb == kExpNonFinite) ? __builtin_nan("")
+                 : (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(x, -(ea + eb)) : 0.0;
     }
 }
 
@@ -1697,8 +1726,7 @@ __global__ __launch_bounds__(256) void ozaki_fold_kernel(const int *__restrict__
         const double w = ldexp(1.0, -8 * (d + 2));
         for (int c = 0; c < nchunks; ++c) acc = ca3::add_rn(acc, (double) p[(int64_t) (d * nchunks + c) * plane] * w);
     }
-    const int ea = eA[i], eb = eB[j];
-    double v = (ea != kExpEmpty && eb != kExpEmpty) ? ldexp(ldexp(acc, ea), eb) : 0.0;
+    double v = oz_unscale(acc, eA[i], eB[j]);
     double *cp = C + (int64_t) j * ldc + i;
     v = ca3::mul_rn(alpha, v);
     if (beta != 0.0) v = ca3::add_rn(v, ca3::mul_rn(beta, *cp));
@@ -2072,8 +2100,6 @@ void launch_crt_grouped_n(int nmod, std::integer_sequence<int, Ns...>, int block
                            : void()), ...);
 }
 
-std::map<std::string, OzTuning> g_oz_cache;
-std::mutex g_oz_cache_mu;
 
 int oz_sequence(OzPlan *p, const OzTuning &tn, bool grouped, hipStream_t st)
 {
@@ -2317,9 +2343,8 @@ int build_ozaki(const std::vector<Problem> &probs, const std::vector<Fold> &fold
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { destroy_ozaki(p); return ca3::fail(e, "grouped ozaki plan"); }
     {
-        std::lock_guard<std::mutex> lk(g_oz_cache_mu);
-        auto it = g_oz_cache.find(p->signature);
-        if (it != g_oz_cache.end()) p->tune = it->second;
+        OzTuning t;
+        if (lookup_tuning_ozaki(p->signature, &t)) p->tune = t;
     }
     *out = p;
     return 0;
@@ -2379,8 +2404,7 @@ int autotune_ozaki(OzPlan *p, int reps)
     (void) hipEventDestroy(e1);
     if (rc) return rc;
     p->tune = bt;
-    std::lock_guard<std::mutex> lk(g_oz_cache_mu);
-    g_oz_cache[p->signature] = bt;
+    store_tuning_ozaki(p->signature, bt);
     return 0;
 }
 

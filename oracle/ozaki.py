@@ -75,7 +75,7 @@ def ozaki_dgemm(opA, opB, s, alpha=1.0, beta=0.0, Cin=None):
     acc = np.zeros(D[0].shape)
     for d in range(s - 1, -1, -1):
         acc = acc + np.ldexp(D[d], -SLICE_BITS * (d + 2))
-    C = np.ldexp(np.ldexp(acc, EA.reshape(-1, 1)), EB.reshape(1, -1))
+    C = np.ldexp(acc, EA.reshape(-1, 1) + EB.reshape(1, -1))
     C = np.where(zA.reshape(-1, 1) | zB.reshape(1, -1), 0.0, C)
     C = alpha * C
     if beta != 0.0:
@@ -106,16 +106,25 @@ DIGIT_BITS = 8
 TOP_MANT = 253.0 / 256.0
 
 
+def nonfinite_lines(X, axis):
+    """Lines (axis=1: rows, axis=0: columns) holding an Inf or a NaN.  Both
+    forms slice such a line as a zero line and write NaN into every C entry
+    of its row (op(A)) or column (op(B))."""
+    return ~np.all(np.isfinite(X), axis=axis)
+
+
 def line_scale8(X, axis):
     """Per-line scale exponent of the 8-bit scheme.  With f, e = frexp(max |x|)
     of the line: E = e + 1 when f <= 253/256, else E = e + 2.  Then every
     scaled entry a' = x 2^-E has |a'| <= 253/512, so N = rint(a' 2^(8s)) has
     |N| / 256^(s-1) <= 126.5 and its top balanced digit stays in [-128, 127]
-    (the lower digits add at most 128/255 in magnitude).  Returns (E, zero mask)."""
-    mx = np.max(np.abs(X), axis=axis)
+    (the lower digits add at most 128/255 in magnitude).  Returns (E, zero
+    mask); a line with a non-finite element counts as zero here."""
+    bad = nonfinite_lines(X, axis)
+    mx = np.max(np.abs(np.where(np.isfinite(X), X, 0.0)), axis=axis)
     f, e = np.frexp(mx)
     E = np.where(f <= TOP_MANT, e + 1, e + 2).astype(np.int64)
-    return E, mx == 0
+    return E, (mx == 0) | bad
 
 
 def split8(X, s, axis):
@@ -123,8 +132,7 @@ def split8(X, s, axis):
     (line_scale8).  Returns (list of s int8 arrays, E, zero mask)."""
     E, zero = line_scale8(X, axis)
     shp = (-1, 1) if axis == 1 else (1, -1)
-    a = np.ldexp(X, -E.reshape(shp))
-    a = np.where(zero.reshape(shp), 0.0, a)
+    a = np.ldexp(np.where(zero.reshape(shp), 0.0, X), -E.reshape(shp))
     f, ex = np.frexp(a)                         # a = f * 2^ex, f in [0.5, 1)
     M = np.ldexp(f, 53).astype(np.int64)        # exact 53-bit signed significand
     z = ex.astype(np.int64) - 53 + 8 * s        # N = M * 2^z
@@ -185,8 +193,9 @@ def ozaki_dgemm8(opA, opB, s, alpha=1.0, beta=0.0, Cin=None):
     for d in range(s - 1, -1, -1):
         for D in Dc:
             acc = acc + np.ldexp(D[d], -DIGIT_BITS * (d + 2))
-    C = np.ldexp(np.ldexp(acc, EA.reshape(-1, 1)), EB.reshape(1, -1))
+    C = np.ldexp(acc, EA.reshape(-1, 1) + EB.reshape(1, -1))          # one rounding
     C = np.where(zA.reshape(-1, 1) | zB.reshape(1, -1), 0.0, C)
+    C = np.where(nonfinite_lines(opA, 1).reshape(-1, 1) | nonfinite_lines(opB, 0).reshape(1, -1), np.nan, C)
     C = alpha * C
     if beta != 0.0:
         C = C + beta * Cin
@@ -267,8 +276,12 @@ def crt_T(N):
 
 def crt_line_exponents(X, axis, N):
     """Scale exponents e (int64) and zero mask of the lines of X along
-    `axis` (axis=1: rows of op(A); axis=0: columns of op(B))."""
+    `axis` (axis=1: rows of op(A); axis=0: columns of op(B)); a line with a
+    non-finite element counts as zero here (nonfinite_lines)."""
     T = crt_T(N)
+    bad = nonfinite_lines(X, axis)
+    X = np.where(np.isfinite(X), X, 0.0)
+    X = np.where(bad.reshape((-1, 1) if axis == 1 else (1, -1)), 0.0, X)
     mx = np.max(np.abs(X), axis=axis)
     _, E = np.frexp(mx)
     E = E.astype(np.int64)
@@ -283,7 +296,7 @@ def crt_line_exponents(X, axis, N):
     S = np.sum(w, axis=axis, dtype=np.uint64).astype(object)
     h = np.array([crt_scale_h(int(v), T) for v in S.ravel()], dtype=np.int64)
     e = T + 20 - E - h
-    return e, mx == 0
+    return e, (mx == 0) | bad
 
 
 CRT_SLACK = 2897            # ceil(sqrt(CRT_KMAX)): 2 x the largest rounding slack of a line norm
@@ -304,7 +317,7 @@ def crt_scaled(X, axis, N):
     """Integer-valued doubles a' = rint(x 2^e) per line; zero lines 0."""
     e, zero = crt_line_exponents(X, axis, N)
     shp = (-1, 1) if axis == 1 else (1, -1)
-    a = np.rint(np.ldexp(X, np.where(zero, 0, e).reshape(shp)))
+    a = np.rint(np.ldexp(np.where(zero.reshape(shp), 0.0, X), np.where(zero, 0, e).reshape(shp)))
     return np.where(zero.reshape(shp), 0.0, a), e, zero
 
 
@@ -386,8 +399,9 @@ def ozaki2_dgemm(opA, opB, N, alpha=1.0, beta=0.0, Cin=None):
         P = RA[t].astype(np.float64) @ RB[t].astype(np.float64)
         Z.append(balanced_mod(P.astype(np.int64), p))
     x = crt_horner(crt_garner(Z, N), N)
-    C = np.ldexp(np.ldexp(x, -np.where(zA, 0, eA).reshape(-1, 1)), -np.where(zB, 0, eB).reshape(1, -1))
+    C = np.ldexp(x, -(np.where(zA, 0, eA).reshape(-1, 1) + np.where(zB, 0, eB).reshape(1, -1)))   # one rounding
     C = np.where(zA.reshape(-1, 1) | zB.reshape(1, -1), 0.0, C)
+    C = np.where(nonfinite_lines(opA, 1).reshape(-1, 1) | nonfinite_lines(opB, 0).reshape(1, -1), np.nan, C)
     C = alpha * C
     if beta != 0.0:
         C = C + beta * Cin
