@@ -18,6 +18,7 @@ Each suite prints JSON lines (one per measurement) on stdout:
   yardstick    rocBLAS dgemm and INT8 gemm_ex at the same size (never on the path).
 """
 import json
+import os
 import sys
 import time
 
@@ -120,12 +121,64 @@ def rank_share(args):
             unpack.append((recvs[r].at(e[5]), e[3], dsts[r].at((e[0] - q[0]) * q[3] + (e[1] - q[1])), q[3], e[2], e[3]))
     check_blocks(pack, [Afull] + sends)
     check_blocks(unpack, recvs + dsts)
-    for name, lst in (("pack A, 24 ranks", pack), ("unpack A, 24 ranks", unpack), ("pack A, rank 0", pack0)):
+    # the one-pass redistribution of A and B for all 24 ranks (what the
+    # simulated grid runs): each block from the sender's column block straight
+    # into the receiver's redistribution buffer, both matrices in one call
+    Bfull = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
+    direct, direct0 = [], []
+    bufs = [Afull, Bfull]
+    for name, full in (("A", Afull), ("B", Bfull)):
+        red = w["redist"][name]
+        dst = [hip.DeviceArray((max(1, red["req"][r][2] * red["req"][r][3]),)) for r in range(24)]
+        bufs += dst
+        for r in range(24):
+            q = [int(v) for v in red["req"][r]]
+            for e in red["plans"][r][2]:
+                e = [int(v) for v in e]
+                s0 = [int(v) for v in red["src"][e[4]]]
+                src = full.at(s0[0] * 8000) + 8 * ((e[0] - s0[0]) * 8000 + (e[1] - s0[1]))
+                d = (src, 8000, dst[r].at((e[0] - q[0]) * q[3] + (e[1] - q[1])), q[3], e[2], e[3])
+                direct.append(d)
+                if r == 0:
+                    direct0.append(d)
+    check_blocks(direct, bufs)
+    for name, lst in (("pack A, 24 ranks", pack), ("unpack A, 24 ranks", unpack), ("pack A, rank 0", pack0),
+                      ("one-pass redistribution of A and B, 24 ranks", direct),
+                      ("one-pass redistribution of A and B, rank 0", direct0)):
         elems = sum(d[4] * d[5] for d in lst)
         ms = timed(lambda: hip.copy_blocks(8, lst), reps=20)
         by = 16 * elems
         emit({"suite": "rank-share", "what": name, "blocks": len(lst), "elements": elems, "ms": round(ms, 5),
               "bytes": by, "gbs": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / HBM_GBS, 4)})
+    del bufs, direct, direct0
+    # the whole 8000^3 grid in one persistent grouped launch (all 24 ranks'
+    # Cannon rounds and the kp = 3 fold), autotuned, then the full exec
+    # (redistributions included) replayed from its hipGraph
+    from ca3dmm_amd import engine as E
+    A8 = rnd.uniform(-1, 1, (8000, 8000))
+    B8 = rnd.uniform(-1, 1, (8000, 8000))
+    dA8, dB8 = hip.DeviceArray.from_numpy(A8), hip.DeviceArray.from_numpy(B8)
+    dC8 = hip.DeviceArray.zeros((8000, 8000))
+    sim = E.SimGrid(24, 8000, 8000, 8000, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=[4, 2, 3])
+    Ap = [dA8.at(r[2] * 8000) for r in w["src_A"]]
+    Bp = [dB8.at(r[2] * 8000) for r in w["src_B"]]
+    Cp = [dC8.at(r[2] * 8000) for r in w["dst_C"]]
+    run = lambda: sim.exec(Ap, [8000] * 24, Bp, [8000] * 24, Cp, [8000] * 24)
+    fl = 2.0 * 8000 ** 3
+    ms = timed(run, reps=5)
+    emit({"suite": "rank-share", "what": "8000^3 on 4x2x3, per-rank per-round launches, whole exec",
+          "ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)})
+    sim.set_mode(grouped=True, graph=False)
+    g, wg, tile = sim.autotune(3)
+    ms = timed(run, reps=5)
+    tname = {0: "128x128", 1: "256x128", 2: "128x256"}[tile]
+    emit({"suite": "rank-share", "what": f"8000^3 on 4x2x3, grouped (tile {tname}, rows={g}, {wg} WG/CU), whole exec",
+          "ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)})
+    sim.set_mode(grouped=True, graph=True)
+    ms = timed(run, reps=5)
+    emit({"suite": "rank-share", "what": "8000^3 on 4x2x3, grouped, whole exec replayed from one hipGraph",
+          "ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)})
+    sim.free()
     hip.set_sync(True)
 
 
@@ -362,10 +415,45 @@ def pmc_calib(args):
     hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 7)
     hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 15)
     hip.sync()
+    # the same DGEMM with direct-to-LDS stages (NN)
+    os.environ["CA3DMM_DGEMM_LOADS"] = "dtl"
+    hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC)
+    hip.sync()
+    del os.environ["CA3DMM_DGEMM_LOADS"]
+    del dA, dB, dC, P, out, sends
+    # the kp = 3 fold over the whole 8000 x 8000 C
+    P8 = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000))) for _ in range(3)]
+    out8 = hip.DeviceArray.zeros((8000, 8000))
+    hip.fold_sum(P8, 8000, 8000, out8)
+    hip.sync()
+    del P8, out8
+    # the 8000^3 grid on 4x2x3: one exec per-rank (one-pass redistributions of
+    # A and B in one launch, 48 DGEMM rounds, 24 folds, C redistribution),
+    # then one exec through the grouped launch
+    from ca3dmm_amd import engine as E
+    dA8 = Afull
+    dB8 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
+    dC8 = hip.DeviceArray.zeros((8000, 8000))
+    sim = E.SimGrid(24, 8000, 8000, 8000, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=[4, 2, 3])
+    Ap = [dA8.at(r[2] * 8000) for r in w["src_A"]]
+    Bp = [dB8.at(r[2] * 8000) for r in w["src_B"]]
+    Cp = [dC8.at(r[2] * 8000) for r in w["dst_C"]]
+    sim.exec(Ap, [8000] * 24, Bp, [8000] * 24, Cp, [8000] * 24)
+    sim.set_mode(grouped=True, graph=False)
+    sim.set_tuning(8, 2, 0)
+    sim.exec(Ap, [8000] * 24, Bp, [8000] * 24, Cp, [8000] * 24)
+    sim.free()
+    del dA8, dB8, dC8, Afull
+    # the FP64 Gram of a 2e6 x 512 operand
+    dG = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (512, 2000000)).T)
+    dS = hip.DeviceArray.zeros((512, 512))
+    hip.gram(512, 2000000, dG, dS)
+    hip.sync()
     emit({"suite": "pmc-calib", "copy_bytes_read": nel * 8, "copy_bytes_written": nel * 8,
-          "dgemm_n": n, "dgemm_cases": "NN,NT,TN,TT", "fold": "3 x 2000x1334",
+          "dgemm_n": n, "dgemm_cases": "NN,NT,TN,TT, then NN direct-to-LDS", "fold": "3 x 2000x1334, 3 x 8000x8000",
           "pack_elements": sum(d[4] * d[5] for d in pack), "ozaki": f"{n}^3 s=7",
-          "ozaki_crt": f"{n}^3 moduli=15"})
+          "ozaki_crt": f"{n}^3 moduli=15", "grid": "8000^3 on 4x2x3: per-rank exec, then grouped exec",
+          "gram": "512 x 512 from 2e6 x 512"})
 
 
 def yardstick(args):

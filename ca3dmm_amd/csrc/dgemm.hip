@@ -64,6 +64,55 @@ __global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void dgemm_kernel(Gemm
     epilogue<TL>(g, m0, n0, acc);
 }
 
+// The 128 x 128 tile with direct-to-LDS stages for interior tiles (the k
+// tail and edge tiles through the register-staged checked path).  Same
+// products in the same order as dgemm_kernel<TA, TB, Tile128>.
+template <int TA, int TB>
+__global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(1024))) double As[2][kBK * kBM];
+    __shared__ __attribute__((aligned(1024))) double Bs[2][kBK * kBN];
+
+    int tm, tn;
+    tile_coords<Tile128>(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn);
+    const int m0 = tm * kBM, n0 = tn * kBN;
+
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    int kt0 = 0;
+    if (m0 + kBM <= g.M && n0 + kBN <= g.N && g.K >= kBK) {
+        DtlLoader<!TA> da;
+        DtlLoader<TB> db;
+        da.init(g.A, g.lda, m0);
+        db.init(g.B, g.ldb, n0);
+        const int nfull = g.K / kBK;
+        mainloop_dtl<!TA, TB>(da, db, g.lda, g.ldb, nfull, As, Bs, acc);
+        kt0 = nfull * kBK;
+    }
+    if (kt0 < g.K) {
+        OperandLoader<!TA> la;
+        OperandLoader<TB> lb;
+        la.init(g.A, g.lda, m0, g.M);
+        lb.init(g.B, g.ldb, n0, g.N);
+        mainloop_checked(la, lb, kt0, g.K, As, Bs, acc);
+    }
+    epilogue(g, m0, n0, acc);
+}
+
+template <int TA, int TB>
+int launch_dtl(const GemmArgs &g)
+{
+    const int tiles = ((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
+    dgemm_dtl_kernel<TA, TB><<<tiles, kThreads, 0, ca3::stream()>>>(g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ca3::fail(e, "dgemm_dtl_kernel");
+    return HIP_PROXY_OK;
+}
+
 template <int TA, int TB, class TL>
 int launch_tile(const GemmArgs &g)
 {
@@ -76,20 +125,27 @@ int launch_tile(const GemmArgs &g)
 
 // Workgroup tile: 128 x 128 (4 waves, two workgroups per CU) unless
 // CA3DMM_DGEMM_TILE selects 256x128 or 128x256 (8 waves, one per CU).
+// CA3DMM_DGEMM_LOADS=dtl: the 128 x 128 tile with direct-to-LDS stages, for
+// 16-byte aligned operands with even leading dimensions.
 int tile_choice()
 {
-    static int choice = -1;
-    if (choice < 0) {
-        const char *v = getenv("CA3DMM_DGEMM_TILE");
-        choice = (v && !strcmp(v, "256x128")) ? 1 : (v && !strcmp(v, "128x256")) ? 2 : 0;
-    }
-    return choice;
+    const char *v = getenv("CA3DMM_DGEMM_TILE");
+    return (v && !strcmp(v, "256x128")) ? 1 : (v && !strcmp(v, "128x256")) ? 2 : 0;
+}
+
+bool dtl_choice(const GemmArgs &g)
+{
+    const char *v = getenv("CA3DMM_DGEMM_LOADS");
+    if (!v || strcmp(v, "dtl")) return false;
+    return ((uintptr_t) g.A % 16) == 0 && ((uintptr_t) g.B % 16) == 0 && (g.lda % 2) == 0 && (g.ldb % 2) == 0;
 }
 
 template <int TA, int TB>
 int launch(const GemmArgs &g)
 {
-    switch (tile_choice()) {
+    const int tc = tile_choice();
+    if (tc == 0 && dtl_choice(g)) return launch_dtl<TA, TB>(g);
+    switch (tc) {
     case 1: return launch_tile<TA, TB, Tile256x128>(g);
     case 2: return launch_tile<TA, TB, Tile128x256>(g);
     default: return launch_tile<TA, TB, Tile128>(g);

@@ -256,6 +256,112 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, boo
     if (kt0 < kend) mainloop_checked<TL>(la, lb, kt0, kend, As, Bs, acc);
 }
 
+// ---------------------------------------------------------------------
+// Direct-to-LDS stages (interior tiles of the 128 x 128 DGEMM, 16-byte
+// aligned operands with even leading dimensions): global_load_lds_dwordx4
+// writes lane l's 16 bytes (two doubles) at M0 + 16 l, so one wave
+// instruction fills a 1 KB LDS block; a stage of an operand is 16 blocks
+// (16 k x 128 x), four per wave.  No staging registers and no LDS stores:
+// the loads of stage z + 1 are issued right after the barrier that opens
+// stage z and have the whole stage to land.  Layouts (d = one double):
+//   XC = 1 (contiguous along x): block = one k row of 128 x; 16-byte chunk
+//     c of row k holds x = 2 (c ^ fx(k)), +1 with fx(k) = 8 (k & 1);
+//   XC = 0 (contiguous along k): block = 8 x lines of 16 k; chunk c of line
+//     x holds k = 2 (c ^ hx(x)), +1 with hx(x) = (x >> 1) & 7.
+// Each lane fetches the global chunk its LDS slot holds, so the reads stay
+// coalesced, and the MFMA operand reads (16 consecutive x at 4 k) hit 64
+// distinct banks per 32 lanes.  The k-slices of the MFMAs are those of
+// stage_mfma, so both paths add the same products in the same order.
+// ---------------------------------------------------------------------
+template <int XC>
+__device__ __forceinline__ int dtl_index(int k, int x)
+{
+    if (XC) return k * 128 + 2 * ((x >> 1) ^ ((k & 1) << 3)) + (x & 1);
+    return x * 16 + 2 * ((k >> 1) ^ ((x >> 1) & 7)) + (k & 1);
+}
+
+template <int XC>
+struct DtlLoader {
+    const double *p[4];   // this lane's element for k0 = 0, one per block of the wave
+
+    __device__ __forceinline__ void init(const double *M, int64_t ld, int x0)
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int b = 4 * w + i;
+            if (XC) {
+                p[i] = M + x0 + 2 * (lane ^ ((b & 1) << 3)) + (int64_t) b * ld;
+            } else {
+                const int x = 8 * b + (lane >> 3);
+                p[i] = M + (int64_t) (x0 + x) * ld + 2 * ((lane & 7) ^ ((x >> 1) & 7));
+            }
+        }
+    }
+
+    // issue the stage starting at k0 into S (16 x 128 doubles)
+    __device__ __forceinline__ void issue(int64_t koff, double *S) const
+    {
+        const int w = threadIdx.x >> 6;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) (p[i] + koff),
+                                             (void __attribute__((address_space(3))) *) (S + (4 * w + i) * 128),
+                                             16, 0, 0);
+    }
+};
+
+template <int XCA, int XCB>
+__device__ __forceinline__ void stage_mfma_dtl(const double *As, const double *Bs, f64x4 (&acc)[4][4])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15;
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ++ks) {
+        const int kk = ks * 4 + (lane >> 4);
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[dtl_index<XCA>(kk, wm * 64 + i * 16 + li)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[dtl_index<XCB>(kk, wn * 64 + j * 16 + li)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+    }
+}
+
+// The whole 16-deep k-stages [0, nfull * 16) of an interior 128 x 128 tile
+// through the direct-to-LDS double buffer (koff(k) = k * ld for XC = 1, k
+// for XC = 0).  The caller runs the k tail, if any, through
+// mainloop_checked after a barrier.
+template <int XCA, int XCB>
+__device__ __forceinline__ void mainloop_dtl(const DtlLoader<XCA> &la, const DtlLoader<XCB> &lb, int64_t lda,
+                                             int64_t ldb, int nfull, double (*As)[kBK * kBM],
+                                             double (*Bs)[kBK * kBN], f64x4 (&acc)[4][4])
+{
+    auto koff = [](int xc, int64_t ld, int k0) -> int64_t { return xc ? (int64_t) k0 * ld : (int64_t) k0; };
+    la.issue(koff(XCA, lda, 0), As[0]);
+    lb.issue(koff(XCB, ldb, 0), Bs[0]);
+    for (int kt = 0; kt < nfull; ++kt) {
+        const int cur = kt & 1;
+        // own loads of stage kt have landed (vmcnt counts the LDS DMA), then
+        // every wave's; every wave has also finished reading stage kt - 1
+        __builtin_amdgcn_s_waitcnt(0x0f70);     // vmcnt(0); expcnt (6:4) and lgkmcnt (11:8) not waited on
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (kt + 1 < nfull) {
+            la.issue(koff(XCA, lda, (kt + 1) * kBK), As[cur ^ 1]);
+            lb.issue(koff(XCB, ldb, (kt + 1) * kBK), Bs[cur ^ 1]);
+        }
+        stage_mfma_dtl<XCA, XCB>(As[cur], Bs[cur], acc);
+    }
+    __syncthreads();                            // the buffers are free for the caller's tail
+}
+
 // C := fold(P) + (alpha*acc + beta*C) for the tile at (m0, n0).
 template <class TL = Tile128>
 __device__ __forceinline__ void epilogue(const GemmArgs &g, int m0, int n0, const f64x4 (&acc)[4][4])

@@ -280,3 +280,56 @@ def test_one_pass_redistribution_blocks_inside_both_buffers(m, n, k, p, grid, ta
             area = int(q[2]) * int(q[3])
             got = sum(int(b[2]) * int(b[3]) for b in red["plans"][r][2])
             assert got == area, (name, r, got, area)
+
+
+# --------------------------------------------------------------------------- direct-to-LDS stages
+def dtl_index(xc, k, x):
+    """dgemm_tile.h dtl_index, restated (in doubles)."""
+    if xc:
+        return k * 128 + 2 * ((x >> 1) ^ ((k & 1) << 3)) + (x & 1)
+    return x * 16 + 2 * ((k >> 1) ^ ((x >> 1) & 7)) + (k & 1)
+
+
+def dtl_writes(xc):
+    """(LDS double index, k, x) of every double the 4 waves' DtlLoader issues
+    write: lane l of wave w, block b = 4 w + i lands at b * 128 + 2 l (+1)."""
+    out = []
+    for w in range(4):
+        for i in range(4):
+            b = 4 * w + i
+            for lane in range(64):
+                for h in range(2):
+                    lds = b * 128 + 2 * lane + h
+                    if xc:
+                        k, x = b, 2 * (lane ^ ((b & 1) << 3)) + h
+                    else:
+                        x = 8 * b + (lane >> 3)
+                        k, x = 2 * ((lane & 7) ^ ((x >> 1) & 7)) + h, x
+                    out.append((lds, k, x))
+    return out
+
+
+@pytest.mark.parametrize("xc", [0, 1])
+def test_dtl_stage_layout_is_consistent_and_conflict_free(xc):
+    """Every (k, x) of a 16 x 128 stage is written exactly once, at the slot
+    the MFMA reads use (dtl_index); the global chunk of each lane is 16-byte
+    aligned and inside the stage's 16 x 128 block; and each 32-lane half of
+    an operand read (16 consecutive x at 2 k rows) touches 64 distinct banks
+    (4-byte banks, 8 bytes per lane)."""
+    wr = dtl_writes(xc)
+    seen = {}
+    for lds, k, x in wr:
+        assert 0 <= k < 16 and 0 <= x < 128
+        assert (k, x) not in seen
+        seen[(k, x)] = lds
+        assert dtl_index(xc, k, x) == lds
+    assert len(seen) == 16 * 128
+    for X0 in range(0, 128, 16):
+        for ks in range(4):
+            for half in range(2):
+                banks = set()
+                for lane in range(32 * half, 32 * half + 32):
+                    k, x = 4 * ks + (lane >> 4), X0 + (lane & 15)
+                    byte = 8 * dtl_index(xc, k, x)
+                    banks.update({(byte // 4) % 64, (byte // 4 + 1) % 64})
+                assert len(banks) == 64, (xc, X0, ks, half)
