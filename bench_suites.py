@@ -340,32 +340,37 @@ def grouped(args):
 
 # ---------------------------------------------------------------------------
 def cholqr(args):
-    """CholeskyQR2 of 2e6 x 512 at three condition numbers, with the Gram
-    through FP64 MFMA and through Ozaki INT8 (s = 7, 8)."""
+    """CholeskyQR of 2e6 x 512 at four condition numbers through
+    hip_cholesky_qr: CholeskyQR2 with the Gram through FP64 MFMA and through
+    Ozaki INT8 (s = 7, 8), shifted CholeskyQR3, and the automatic variant."""
     from ca3dmm_amd import cholqr as CQ
     m, n = 2000000, 512
     dU = None
-    for cond in (1e2, 1e5, 1e8):
+    for cond in (1e2, 1e5, 1e8, 1e12):
         t0 = time.perf_counter()
         dA, dU = CQ.test_matrix(m, n, cond, seed=2022, dU=dU)
         log(f"cholqr: matrix kappa={cond:g} built in {time.perf_counter() - t0:.1f} s")
         dG = hip.DeviceArray.zeros((n, n))
-        for gs in (None, 7, 8):
+        cases = [(hip.CHOLQR2, None), (hip.CHOLQR2, 7), (hip.CHOLQR2, 8), (hip.SCHOLQR3, None),
+                 (hip.CHOLQR_AUTO, None)]
+        for variant, gs in cases:
             hip.set_sync(False)
             if gs:
                 ms_gram = timed(lambda: hip.ozaki_gram(n, m, dA, dG, gs), reps=3)
             else:
                 ms_gram = timed(lambda: hip.gram(n, m, dA, dG), reps=3)
             hip.set_sync(True)
-            rec = {"suite": "cholqr", "m": m, "n": n, "cond": cond,
+            name = {hip.CHOLQR2: "cholqr2", hip.SCHOLQR3: "shifted cholqr3", hip.CHOLQR_AUTO: "auto"}[variant]
+            rec = {"suite": "cholqr", "m": m, "n": n, "cond": cond, "variant": name,
                    "gram": f"ozaki-int8 s={gs}" if gs else "fp64-mfma", "gram_ms": round(ms_gram, 3),
                    "gram_tflops": round(n * (n + 1) * m / ms_gram / 1e9, 2)}
             try:
                 hip.sync()
                 t = hip.Timer()
                 t.start()
-                dQ, dR = CQ.cholesky_qr2(dA, m, n, gram_slices=gs)
-                rec["cholqr2_ms"] = round(t.stop(), 2)
+                dQ, dR, passes = CQ.cholesky_qr(dA, m, n, variant, gram_slices=gs)
+                rec["ms"] = round(t.stop(), 2)
+                rec["passes"] = passes
                 orth, res = CQ.qr_errors(dA, dQ, dR, m, n)
                 rec.update({"orth_QtQ_minus_I_F": orth, "residual_rel_F": res})
                 del dQ, dR
