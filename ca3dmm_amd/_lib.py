@@ -73,6 +73,7 @@ SIGNATURES = {
     "hip_ozaki_choose": (I, [I, I, I, I, I, P, I, P, I, IP, IP]),
     "hip_ozaki_counts": (I, [D, D, I, IP, IP]),
     "hip_dsyrk_gram": (I, [I, I, D, P, I, D, P, I, I]),
+    "hip_dsyrk_gram_noalloc": (I, [I, I, D, P, I, D, P, I, I]),
     "hip_ozaki_gram": (I, [I, I, D, P, I, D, P, I, I, I]),
     "hip_dtrtri_upper": (I, [I, P, I, P, I]),
     "hip_cholesky_qr": (I, [I, I, P, I, P, I, I, I, IP]),

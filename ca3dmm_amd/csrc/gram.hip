@@ -17,6 +17,12 @@
 #include "dgemm_tile.h"
 #include "hip_common.h"
 
+// Partial tiles of hip_dsyrk_gram_noalloc: a fixed array in device memory,
+// 512 tiles of 128 x 128 doubles (64 MiB) per device, held until the process
+// ends.  Host code must use the address hipGetSymbolAddress returns for the
+// current device, never this name.
+static __device__ __align__(256) double scratch[(size_t) 512 * 128 * 128];
+
 namespace {
 
 using namespace ca3::gemm;
@@ -99,6 +105,43 @@ struct Ws {
 Ws g_gram_ws[16];
 std::mutex g_gram_mu;
 
+constexpr int kScratchTiles = 512;   // partial tiles the allocation-free entry can hold
+double *g_scratch_ptr[16];          // device address of `scratch` per device (0: not looked up yet)
+
+// Tiles, k-split and k-range per split for n x n from k.  Enough workgroups
+// for one round of the chip at two per CU (at most kScratchTiles partial
+// tiles on a 256-CU part), stages of >= 1024 k, whole 16-k stages per split.
+void plan(int n, int k, GramArgs *g)
+{
+    memset(g, 0, sizeof(*g));
+    const int ntile = (n + kBM - 1) / kBM, npairs = ntile * (ntile + 1) / 2;
+    const int target = 2 * ca3::cu_count();
+    int ksplit = std::max(1, std::min(target / npairs, (k + 1023) / 1024));
+    if (k == 0) ksplit = 1;
+    const int kstep = std::max(kBK, ((k + ksplit - 1) / ksplit + kBK - 1) / kBK * kBK);
+    ksplit = std::max(1, (k + kstep - 1) / kstep);
+    g->n = n; g->k = k; g->ntile = ntile; g->npairs = npairs; g->ksplit = ksplit; g->kstep = kstep;
+}
+
+int check_args(int n, int k, const double *A, int ldA, const double *C, int ldC)
+{
+    if (n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: negative size");
+    if (n > 0 && (!C || ldC < n)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: bad C / ldC");
+    if (n > 0 && k > 0 && (!A || ldA < std::max(1, k))) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: bad A / ldA");
+    return 0;
+}
+
+int launch(const GramArgs &g, double alpha, double beta, double *C, int ldC, int mirror)
+{
+    hipStream_t st = ca3::stream();
+    gram_partial_kernel<<<g.npairs * g.ksplit, kThreads, 0, st>>>(g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ca3::fail(e, "gram_partial_kernel");
+    gram_reduce_kernel<<<dim3(16, g.npairs), 256, 0, st>>>(g.P, g.n, g.ntile, g.npairs, g.ksplit, alpha, beta, C,
+                                                          ldC, mirror ? 1 : 0);
+    return ca3::finish();
+}
+
 }  // namespace
 
 extern "C" {
@@ -106,19 +149,11 @@ extern "C" {
 int hip_dsyrk_gram(const int n, const int k, const double alpha, const double *A, const int ldA,
                    const double beta, double *C, const int ldC, int mirror)
 {
-    if (n < 0 || k < 0) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: negative size");
-    if (n == 0) return HIP_PROXY_OK;
-    if (!C || ldC < n) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: bad C / ldC");
-    if (k > 0 && (!A || ldA < std::max(1, k))) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "gram: bad A / ldA");
-    const int ntile = (n + kBM - 1) / kBM, npairs = ntile * (ntile + 1) / 2;
-    // enough workgroups to fill the chip twice over, stages of >= 1024 k
-    const int target = 2 * 2 * ca3::cu_count();
-    int ksplit = std::max(1, std::min(target / npairs, (k + 1023) / 1024));
-    if (k == 0) ksplit = 1;
-    // k-range per split: ceil(k / ksplit) rounded up to whole stages
-    const int kstep = std::max(kBK, ((k + ksplit - 1) / ksplit + kBK - 1) / kBK * kBK);
-    ksplit = std::max(1, (k + kstep - 1) / kstep);
-    const size_t bytes = sizeof(double) * (size_t) ksplit * npairs * kBM * kBN;
+    int rc = check_args(n, k, A, ldA, C, ldC);
+    if (rc || n == 0) return rc;
+    GramArgs g;
+    plan(n, k, &g);
+    const size_t bytes = sizeof(double) * (size_t) g.ksplit * g.npairs * kBM * kBN;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     Ws &w = g_gram_ws[dev & 15];
@@ -135,17 +170,39 @@ int hip_dsyrk_gram(const int n, const int k, const double alpha, const double *A
             w.bytes = bytes;
         }
     }
-    GramArgs g;
-    memset(&g, 0, sizeof(g));
-    g.n = n; g.k = k; g.ntile = ntile; g.npairs = npairs; g.ksplit = ksplit; g.kstep = kstep;
     g.A = A; g.lda = ldA; g.P = (double *) w.p;
-    hipStream_t st = ca3::stream();
-    gram_partial_kernel<<<npairs * ksplit, kThreads, 0, st>>>(g);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return ca3::fail(e, "gram_partial_kernel");
-    gram_reduce_kernel<<<dim3(16, npairs), 256, 0, st>>>(g.P, n, ntile, npairs, ksplit, alpha, beta, C, ldC,
-                                                        mirror ? 1 : 0);
-    return ca3::finish();
+    return launch(g, alpha, beta, C, ldC, mirror);
+}
+
+int hip_dsyrk_gram_noalloc(const int n, const int k, const double alpha, const double *A, const int ldA,
+                           const double beta, double *C, const int ldC, int mirror)
+{
+    int rc = check_args(n, k, A, ldA, C, ldC);
+    if (rc || n == 0) return rc;
+    GramArgs g;
+    plan(n, k, &g);
+    static_assert(sizeof(scratch) == sizeof(double) * (size_t) kScratchTiles * kBM * kBN, "scratch size");
+    if ((int64_t) g.ksplit * g.npairs > kScratchTiles)
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG,
+                             "gram_noalloc: n = %d, k = %d needs %lld partial 128 x 128 tiles; the fixed scratch "
+                             "holds %d (n <= 3968)", n, k, (long long) g.ksplit * g.npairs, kScratchTiles);
+    int dev = 0;
+    CA3_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 16) return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "gram_noalloc: device %d >= 16", dev);
+    double *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_gram_mu);
+        if (!g_scratch_ptr[dev]) {
+            // once per device, with that device current: the host-side name
+            // `scratch` is not a device address
+            void *p = nullptr;
+            CA3_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[dev] = (double *) p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    g.A = A; g.lda = ldA; g.P = ptr;
+    return launch(g, alpha, beta, C, ldC, mirror);
 }
 
 }  // extern "C"

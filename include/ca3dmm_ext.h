@@ -127,6 +127,20 @@ int hip_ozaki_gram(const int n, const int k, const double alpha, const double *A
 int hip_dsyrk_gram(const int n, const int k, const double alpha, const double *A, const int ldA,
                    const double beta, double *C, const int ldC, int mirror);
 
+/* hip_dsyrk_gram without any allocation: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels, k-split and result, bit for bit, as hip_dsyrk_gram; the k-split
+ * partial tiles live in a fixed 64 MiB device array of the library (512
+ * tiles of 128 x 128 doubles, one per GPU the library runs on, held until
+ * the process ends) instead of a library allocation.  Shapes needing more
+ * than 512 partial tiles (n > 3968 on a 256-CU MI355X) are refused with
+ * HIP_PROXY_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (two streams, two host threads) race.  Each
+ * call writes every partial tile it reads, so nothing needs clearing. */
+int hip_dsyrk_gram_noalloc(const int n, const int k, const double alpha, const double *A, const int ldA,
+                           const double beta, double *C, const int ldC, int mirror);
+
 /* X := R^{-1} for upper-triangular R (n x n, n <= 8192); the strictly
  * lower part of X is set to zero (examples/example_CholeskyQR.c:137). */
 int hip_dtrtri_upper(const int n, const double *R, const int ldR, double *X, const int ldX);
