@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "ca3dmm_ext.h"
 #include "hip_common.h"
@@ -40,6 +41,7 @@ struct CopyDesc {
     int32_t nrow, ncol; // ncol in words
     int32_t rows_per_tile;
     int32_t tile_start;
+    int32_t chunks;     // > 1: rows longer than a tile, cut into this many tiles each
     double inv_ncol;    // 1 / ncol, for the row of a flattened index
 };
 
@@ -73,13 +75,24 @@ __global__ __launch_bounds__(kCopyThreads) void copy_blocks_kernel(const CopyBat
         if (b.d[mid].tile_start <= t) lo = mid; else hi = mid - 1;
     }
     const CopyDesc &d = b.d[lo];
-    const int r0 = (t - d.tile_start) * d.rows_per_tile;
-    const int nr = min(d.rows_per_tile, d.nrow - r0);
-    const unsigned ncol = (unsigned) d.ncol;
-    const double inv = d.inv_ncol;
+    int r0, nr;
+    int64_t c0 = 0;
+    unsigned ncol = (unsigned) d.ncol;
+    double inv = d.inv_ncol;
+    if (d.chunks > 1) {             // one tile-long piece of one long row
+        const int q = t - d.tile_start;
+        r0 = q / d.chunks;
+        c0 = (int64_t) (q - r0 * d.chunks) * kTileWords;
+        ncol = (unsigned) min((int64_t) kTileWords, (int64_t) d.ncol - c0);
+        nr = 1;
+        inv = 1.0 / (double) ncol;
+    } else {
+        r0 = (t - d.tile_start) * d.rows_per_tile;
+        nr = min(d.rows_per_tile, d.nrow - r0);
+    }
     const unsigned total = (unsigned) nr * ncol;
-    const W *__restrict__ s = reinterpret_cast<const W *>(d.src) + (int64_t) r0 * d.lds;
-    W *__restrict__ o = reinterpret_cast<W *>(d.dst) + (int64_t) r0 * d.ldd;
+    const W *__restrict__ s = reinterpret_cast<const W *>(d.src) + (int64_t) r0 * d.lds + c0;
+    W *__restrict__ o = reinterpret_cast<W *>(d.dst) + (int64_t) r0 * d.ldd + c0;
     for (unsigned e0 = threadIdx.x; e0 < total; e0 += kCopyThreads * kCopyUnroll) {
         W v[kCopyUnroll];
         unsigned rr[kCopyUnroll], cc[kCopyUnroll];
@@ -116,12 +129,19 @@ int launch_copy(const hip_blk_copy_t *descs, int nblk, size_t dt, size_t w)
             d.nrow = s.nrow;
             d.ncol = (int32_t) ((size_t) s.ncol * dt / w);
             d.inv_ncol = 1.0 / (double) d.ncol;
-            d.rows_per_tile = std::max(1, kTileWords / d.ncol);
-            // keep the per-tile flattened index inside 32 bits
-            int64_t max_rows = (int64_t) 0x7fffffff / d.ncol;
-            if (d.rows_per_tile > max_rows) d.rows_per_tile = (int32_t) max_rows;
             d.tile_start = tiles;
-            tiles += (d.nrow + d.rows_per_tile - 1) / d.rows_per_tile;
+            if (d.ncol > kTileWords) {
+                // a row longer than a tile is cut into tile-long pieces, so a
+                // single long row (the allgather's blocks) still spreads over
+                // the chip
+                d.chunks = (d.ncol + kTileWords - 1) / kTileWords;
+                d.rows_per_tile = 1;
+                tiles += d.nrow * d.chunks;
+            } else {
+                d.chunks = 1;
+                d.rows_per_tile = std::max(1, kTileWords / d.ncol);
+                tiles += (d.nrow + d.rows_per_tile - 1) / d.rows_per_tile;
+            }
             ++b.n;
         }
         if (b.n == 0) continue;
@@ -272,16 +292,23 @@ int hip_copy_matrix_blocks(size_t dt_size, int nblk, const hip_blk_copy_t *descs
     if (dt_size != 4 && dt_size != 8)
         return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: dt_size must be 4 or 8 (got %zu)", dt_size);
     if (nblk < 0 || (nblk > 0 && !descs)) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: bad list");
-    size_t w = 16;
+    // each block goes with the widest word its addresses and leading
+    // dimensions allow (16 bytes, else the element size): one launch list per
+    // word size
+    std::vector<hip_blk_copy_t> wide, narrow;
     for (int i = 0; i < nblk; ++i) {
         const hip_blk_copy_t &s = descs[i];
         if (s.nrow < 0 || s.ncol < 0 || (s.nrow > 0 && s.ncol > 0 && (!s.src || !s.dst)))
             return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: block %d invalid", i);
         if (s.nrow > 0 && s.ncol > 0 && (s.lds < s.ncol || s.ldd < s.ncol))
             return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: block %d leading dim < ncol", i);
-        if (s.nrow > 0 && s.ncol > 0 && !aligned_to(s, dt_size, w)) w = dt_size;
+        if (s.nrow <= 0 || s.ncol <= 0) continue;
+        if ((int64_t) s.ncol * (int64_t) dt_size / 4 > 0x7fffffffLL)
+            return ca3::fail_msg(HIP_PROXY_ERR_ARG, "copy: block %d row too long", i);
+        (aligned_to(s, dt_size, 16) ? wide : narrow).push_back(s);
     }
-    int rc = launch_copy(descs, nblk, dt_size, w);
+    int rc = wide.empty() ? 0 : launch_copy(wide.data(), (int) wide.size(), dt_size, 16);
+    if (!rc && !narrow.empty()) rc = launch_copy(narrow.data(), (int) narrow.size(), dt_size, dt_size);
     if (rc) return rc;
     return ca3::finish();
 }

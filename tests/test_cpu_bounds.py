@@ -217,33 +217,53 @@ def test_copy_row_of_flattened_index_is_exact():
 
 
 def copy_tiles(descs, words=8):
-    """Tile partition of one CopyBatch (launch_copy): rows_per_tile and the
-    first tile of each descriptor."""
+    """Tile partition of one CopyBatch (launch_copy): per descriptor its first
+    tile, rows per tile and pieces per row (a row longer than a tile is cut
+    into tile-long pieces, one tile each)."""
     tile_words = 256 * 4 * 4
     out, tiles = [], 0
     for nrow, ncol in descs:
         ncw = ncol * 8 // words
-        rpt = max(1, tile_words // ncw)
-        rpt = min(rpt, (2 ** 31 - 1) // ncw)
-        out.append((tiles, rpt, nrow))
-        tiles += -(-nrow // rpt)
+        if ncw > tile_words:
+            chunks = -(-ncw // tile_words)
+            out.append((tiles, 1, nrow, chunks, ncw))
+            tiles += nrow * chunks
+        else:
+            rpt = max(1, tile_words // ncw)
+            out.append((tiles, rpt, nrow, 1, ncw))
+            tiles += -(-nrow // rpt)
     return out, tiles
 
 
-def test_copy_batch_tiles_cover_every_row_once():
-    rnd = np.random.default_rng(2)
-    for _ in range(50):
-        descs = [(int(rnd.integers(1, 5000)), int(rnd.integers(1, 9000))) for _ in range(int(rnd.integers(1, 48)))]
-        parts, total = copy_tiles(descs)
-        cover = [np.zeros(nrow, int) for nrow, _ in descs]
+@pytest.mark.parametrize("long_rows", [False, True])
+def test_copy_batch_tiles_cover_every_word_once(long_rows):
+    """copy_blocks_kernel's tile -> (rows, words) mapping, restated: every word
+    of every block is copied by exactly one tile, and a tile's flattened
+    index stays inside 32 bits."""
+    rnd = np.random.default_rng(2 + long_rows)
+    tile_words = 4096
+    for _ in range(30):
+        if long_rows:
+            descs = [(int(rnd.integers(1, 4)), int(rnd.integers(1, 40000))) for _ in range(int(rnd.integers(1, 48)))]
+        else:
+            descs = [(int(rnd.integers(1, 5000)), int(rnd.integers(1, 9000))) for _ in range(int(rnd.integers(1, 48)))]
+        parts, total = copy_tiles(descs, words=16)
+        cover = [np.zeros((nrow, p[4]), np.int8) for (nrow, _), p in zip(descs, parts)]
+        starts = [p[0] for p in parts]
         for t in range(total):
-            lo = max(i for i, (t0, _, _) in enumerate(parts) if t0 <= t)   # the kernel's binary search
-            t0, rpt, nrow = parts[lo]
-            r0 = (t - t0) * rpt
-            nr = min(rpt, nrow - r0)
-            assert nr > 0 and r0 + nr <= nrow
-            cover[lo][r0:r0 + nr] += 1
-            assert nr * descs[lo][1] < 2 ** 31                      # 32-bit flattened index
+            lo = int(np.searchsorted(starts, t, side="right") - 1)      # the kernel's binary search
+            t0, rpt, nrow, chunks, ncw = parts[lo]
+            if chunks > 1:
+                q = t - t0
+                r0, c0 = q // chunks, (q % chunks) * tile_words
+                nc = min(tile_words, ncw - c0)
+                assert r0 < nrow and nc > 0
+                cover[lo][r0, c0:c0 + nc] += 1
+            else:
+                r0 = (t - t0) * rpt
+                nr = min(rpt, nrow - r0)
+                assert nr > 0 and r0 + nr <= nrow and nr * ncw < 2 ** 31
+                cover[lo][r0:r0 + nr, :] += 1
         assert all(np.all(c == 1) for c in cover)
 
 

@@ -125,3 +125,25 @@ def test_transpose_bit_exact(hip, nrow, ncol):
 def test_memset_fill(hip):
     d = hip.DeviceArray.filled((1003,), 0xAB, np.uint8)
     assert np.all(d.numpy() == 0xAB)
+
+
+def test_copy_long_rows_and_mixed_alignment(hip):
+    """Rows longer than a copy tile (the allgather's single-row blocks) are
+    cut into tile-long pieces, and a list mixing 16-byte aligned and
+    unaligned blocks goes out with the widest word each block allows:
+    bit-exact either way."""
+    rnd = np.random.default_rng(21)
+    src = rnd.standard_normal(700001)
+    dst = np.zeros(700001)
+    ds, dd = hip.DeviceArray.from_numpy(src), hip.DeviceArray.from_numpy(dst)
+    blocks = [(ds.at(0), 300000, dd.at(0), 300000, 1, 300000),          # one long aligned row
+              (ds.at(300001), 150000, dd.at(300001), 150000, 2, 149999),  # odd offsets: 8-byte words
+              (ds.at(600002), 33333, dd.at(600002), 33333, 3, 33333)]
+    hip.copy_blocks(8, blocks)
+    got = dd.numpy()
+    want = np.zeros(700001)
+    want[:300000] = src[:300000]
+    for r in range(2):
+        want[300001 + r * 150000:300001 + r * 150000 + 149999] = src[300001 + r * 150000:300001 + r * 150000 + 149999]
+    want[600002:600002 + 3 * 33333] = src[600002:600002 + 3 * 33333]
+    assert np.array_equal(got, want)
