@@ -42,6 +42,16 @@ def measured_traffic(kernel, n):
     except (OSError, ValueError, KeyError):
         return None
     return None
+def dgemm_kernel_name():
+    """The kernel hip_dgemm launches for aligned operands with even leading
+    dimensions (every bench operand): the 128 x 128 tile with direct-to-LDS
+    stages unless CA3DMM_DGEMM_TILE or CA3DMM_DGEMM_LOADS=reg select the
+    register-staged kernel."""
+    if os.environ.get("CA3DMM_DGEMM_TILE") in ("256x128", "128x256"):
+        return "dgemm_kernel"
+    return "dgemm_kernel" if os.environ.get("CA3DMM_DGEMM_LOADS") == "reg" else "dgemm_dtl_kernel"
+
+
 METRIC = "FP64-accurate GEMM TFLOP/s: DGEMM-MFMA vs INT8-Ozaki at equal max rel err"
 
 
@@ -291,7 +301,7 @@ def main():
                    "engine_exec_ms": exec_stat},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": measured_traffic("dgemm_kernel", n), "kernel": "dgemm_kernel",
+                     "traffic": measured_traffic(dgemm_kernel_name(), n), "kernel": dgemm_kernel_name(),
                      "avg_launch_ms": round(gpu_ms / launches, 4)},
         "accuracy": err_stats,
         "ozaki": ozaki,

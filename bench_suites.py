@@ -384,9 +384,12 @@ def cholqr(args):
 def pmc_calib(args):
     """Counter-run program: a plain copy of 1 GiB (the scale of FETCH_SIZE /
     WRITE_SIZE), then one launch of each hot kernel at its bench size: the
-    four DGEMM trans cases at n^3, the kp = 3 fold over 2000 x 1334, the pack
-    of A for the 24 ranks of the 4x2x3 grid on 8000^3, a 7-slice Ozaki
-    product n^3 and a 15-modulus CRT-form product n^3."""
+    four DGEMM trans cases at n^3 (direct-to-LDS stages, the default), the
+    kp = 3 fold over 2000 x 1334, the pack of A for the 24 ranks of the 4x2x3
+    grid on 8000^3, a 7-slice Ozaki product n^3 and a 15-modulus CRT-form
+    product n^3; then the NN DGEMM register-staged throughout, the fold over
+    8000 x 8000, one 8000^3 grid exec per-rank and one grouped, and the Gram
+    of a 2e6 x 512 operand."""
     from oracle import oracle as O
     nel = (1 << 30) // 8
     src = hip.DeviceArray.zeros((nel,))
@@ -420,8 +423,8 @@ def pmc_calib(args):
     hip.ozaki_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 7)
     hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 15)
     hip.sync()
-    # the same DGEMM with direct-to-LDS stages (NN)
-    os.environ["CA3DMM_DGEMM_LOADS"] = "dtl"
+    # the same DGEMM with register-staged stages throughout (NN)
+    os.environ["CA3DMM_DGEMM_LOADS"] = "reg"
     hip.dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC)
     hip.sync()
     del os.environ["CA3DMM_DGEMM_LOADS"]
@@ -455,7 +458,7 @@ def pmc_calib(args):
     hip.gram(512, 2000000, dG, dS)
     hip.sync()
     emit({"suite": "pmc-calib", "copy_bytes_read": nel * 8, "copy_bytes_written": nel * 8,
-          "dgemm_n": n, "dgemm_cases": "NN,NT,TN,TT, then NN direct-to-LDS", "fold": "3 x 2000x1334, 3 x 8000x8000",
+          "dgemm_n": n, "dgemm_cases": "NN,NT,TN,TT (direct-to-LDS stages), then NN register-staged", "fold": "3 x 2000x1334, 3 x 8000x8000",
           "pack_elements": sum(d[4] * d[5] for d in pack), "ozaki": f"{n}^3 s=7",
           "ozaki_crt": f"{n}^3 moduli=15", "grid": "8000^3 on 4x2x3: per-rank exec, then grouped exec",
           "gram": "512 x 512 from 2e6 x 512"})

@@ -188,11 +188,18 @@ __device__ __forceinline__ void stage_mfma(const double *As, const double *Bs, f
     }
 }
 
+// The two LDS stage buffers of an operand held as two separate arrays (the
+// direct-to-LDS loop needs them so); As[i] as for a double[2][...] array.
+struct LdsPair {
+    double *b0, *b1;
+    __device__ __forceinline__ double *operator[](int i) const { return i ? b1 : b0; }
+};
+
 // Accumulate op(A)[m0:+BM, kbeg:kend) * op(B)[kbeg:kend, n0:+BN) into acc
 // with bounds checks on every element (edge tiles and partial k-stages).
-template <class TL = Tile128, class LA, class LB>
-__device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int kend,
-                                                 double (*As)[kBK * TL::BM], double (*Bs)[kBK * TL::BN],
+// As / Bs: double[2][16 * BM] (or LdsPair), As[i] = stage buffer i.
+template <class TL = Tile128, class LA, class LB, class BA, class BB>
+__device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int kend, BA As, BB Bs,
                                                  f64x4 (&acc)[4][4])
 {
     if (kend <= kbeg) return;
@@ -224,9 +231,8 @@ __device__ __forceinline__ void mainloop_checked(LA &la, LB &lb, int kbeg, int k
 // flight, the last stage's load repeated in bounds and never consumed);
 // edge tiles and the k tail go through mainloop_checked.  Both add the
 // k-steps in the same order, so results do not depend on the path.
-template <class TL = Tile128, class LA, class LB>
-__device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, bool interior,
-                                         double (*As)[kBK * TL::BM], double (*Bs)[kBK * TL::BN],
+template <class TL = Tile128, class LA, class LB, class BA, class BB>
+__device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, bool interior, BA As, BB Bs,
                                          f64x4 (&acc)[4][4])
 {
     const int nfull = (interior && kend > kbeg) ? (kend - kbeg) / kBK : 0;
@@ -262,8 +268,8 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, boo
 // writes lane l's 16 bytes (two doubles) at M0 + 16 l, so one wave
 // instruction fills a 1 KB LDS block; a stage of an operand is 16 blocks
 // (16 k x 128 x), four per wave.  No staging registers and no LDS stores:
-// the loads of stage z + 1 are issued right after the barrier that opens
-// stage z and have the whole stage to land.  Layouts (d = one double):
+// the loads of stage z + 1 are issued once stage z's operands are read
+// from LDS, and have stage z's MFMAs to land.  Layouts (d = one double):
 //   XC = 1 (contiguous along x): block = one k row of 128 x; 16-byte chunk
 //     c of row k holds x = 2 (c ^ fx(k)), +1 with fx(k) = 8 (k & 1);
 //   XC = 0 (contiguous along k): block = 8 x lines of 16 k; chunk c of line
@@ -335,29 +341,47 @@ __device__ __forceinline__ void stage_mfma_dtl(const double *As, const double *B
 
 // The whole 16-deep k-stages [0, nfull * 16) of an interior 128 x 128 tile
 // through the direct-to-LDS double buffer (koff(k) = k * ld for XC = 1, k
-// for XC = 0).  The caller runs the k tail, if any, through
-// mainloop_checked after a barrier.
+// for XC = 0); nothing is loaded for nfull = 0.  Ends with a barrier, after
+// which the caller runs the k tail, if any, through mainloop_checked.
+//
+// The two buffers of each operand must be distinct __shared__ arrays and
+// the loop is unrolled by two so that every access names its array at
+// compile time: the compiler then knows the LDS reads of stage z do not
+// touch the buffer the DMA of stage z + 1 writes.  With one array indexed
+// by the stage's parity it cannot tell, and makes the first LDS read after
+// each DMA wait for it, which exposes the whole load latency every stage.
 template <int XCA, int XCB>
 __device__ __forceinline__ void mainloop_dtl(const DtlLoader<XCA> &la, const DtlLoader<XCB> &lb, int64_t lda,
-                                             int64_t ldb, int nfull, double (*As)[kBK * kBM],
-                                             double (*Bs)[kBK * kBN], f64x4 (&acc)[4][4])
+                                             int64_t ldb, int nfull, double *A0, double *A1, double *B0,
+                                             double *B1, f64x4 (&acc)[4][4])
 {
     auto koff = [](int xc, int64_t ld, int k0) -> int64_t { return xc ? (int64_t) k0 * ld : (int64_t) k0; };
-    la.issue(koff(XCA, lda, 0), As[0]);
-    lb.issue(koff(XCB, ldb, 0), Bs[0]);
-    for (int kt = 0; kt < nfull; ++kt) {
-        const int cur = kt & 1;
-        // own loads of stage kt have landed (vmcnt counts the LDS DMA), then
-        // every wave's; every wave has also finished reading stage kt - 1
+    // own loads of the stage have landed (vmcnt counts the LDS DMA), then
+    // every wave's; every wave has also finished reading the previous stage
+    auto publish = []() __attribute__((always_inline)) {
         __builtin_amdgcn_s_waitcnt(0x0f70);     // vmcnt(0); expcnt (6:4) and lgkmcnt (11:8) not waited on
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
+    };
+    if (nfull > 0) {
+        la.issue(koff(XCA, lda, 0), A0);
+        lb.issue(koff(XCB, ldb, 0), B0);
+    }
+    for (int kt = 0; kt < nfull; kt += 2) {
+        publish();                              // stage kt in buffer 0
         if (kt + 1 < nfull) {
-            la.issue(koff(XCA, lda, (kt + 1) * kBK), As[cur ^ 1]);
-            lb.issue(koff(XCB, ldb, (kt + 1) * kBK), Bs[cur ^ 1]);
+            la.issue(koff(XCA, lda, (kt + 1) * kBK), A1);
+            lb.issue(koff(XCB, ldb, (kt + 1) * kBK), B1);
         }
-        stage_mfma_dtl<XCA, XCB>(As[cur], Bs[cur], acc);
+        stage_mfma_dtl<XCA, XCB>(A0, B0, acc);
+        if (kt + 1 >= nfull) break;
+        publish();                              // stage kt + 1 in buffer 1
+        if (kt + 2 < nfull) {
+            la.issue(koff(XCA, lda, (kt + 2) * kBK), A0);
+            lb.issue(koff(XCB, ldb, (kt + 2) * kBK), B0);
+        }
+        stage_mfma_dtl<XCA, XCB>(A1, B1, acc);
     }
     __syncthreads();                            // the buffers are free for the caller's tail
 }

@@ -45,9 +45,15 @@ __device__ __forceinline__ void pair_of(int p, int nt, int *ti, int *tj)
 
 __global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
 {
-    __shared__ double As[2][kBK * kBM];
-    __shared__ double Bs[2][kBK * kBN];
-    const int pair = blockIdx.x % g.npairs, ks = blockIdx.x / g.npairs;
+    __shared__ __attribute__((aligned(1024))) double A0[kBK * kBM], A1[kBK * kBM];
+    __shared__ __attribute__((aligned(1024))) double B0[kBK * kBN], B1[kBK * kBN];
+    // Workgroups are dealt round-robin over the 8 XCDs; the logical ids are
+    // made XCD-contiguous (the grid is a multiple of 8) so that the pairs of
+    // one k-range, which share their panels, run behind the same L2.
+    const int nb = gridDim.x, total = g.npairs * g.ksplit;
+    const int bid = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+    if (bid >= total) return;
+    const int pair = bid % g.npairs, ks = bid / g.npairs;
     int ti, tj;
     pair_of(pair, g.ntile, &ti, &tj);
     const int kbeg = ks * g.kstep, kend = min(g.k, kbeg + g.kstep);
@@ -60,7 +66,19 @@ __global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-    mainloop(la, lb, kbeg, kend, (ti + 1) * kBM <= g.n && (tj + 1) * kBN <= g.n, As, Bs, acc);
+    const bool interior = (ti + 1) * kBM <= g.n && (tj + 1) * kBN <= g.n;
+    int kt0 = kbeg;
+    if (interior && kend - kbeg >= kBK && ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0) {
+        // whole stages direct-to-LDS (kbeg is a multiple of 16, so the panels
+        // stay 16-byte aligned); the same products in the same order
+        DtlLoader<0> da, db;
+        da.init(g.A + kbeg, g.lda, ti * kBM);
+        db.init(g.A + kbeg, g.lda, tj * kBN);
+        const int nfull = (kend - kbeg) / kBK;
+        mainloop_dtl<0, 0>(da, db, g.lda, g.lda, nfull, A0, A1, B0, B1, acc);
+        kt0 = kbeg + nfull * kBK;
+    }
+    if (kt0 < kend) mainloop(la, lb, kt0, kend, interior, LdsPair{A0, A1}, LdsPair{B0, B1}, acc);
     // store the full 128 x 128 partial (rows fastest)
     double *out = g.P + ((int64_t) ks * g.npairs + pair) * (kBM * kBN);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -134,7 +152,7 @@ int check_args(int n, int k, const double *A, int ldA, const double *C, int ldC)
 int launch(const GramArgs &g, double alpha, double beta, double *C, int ldC, int mirror)
 {
     hipStream_t st = ca3::stream();
-    gram_partial_kernel<<<g.npairs * g.ksplit, kThreads, 0, st>>>(g);
+    gram_partial_kernel<<<(g.npairs * g.ksplit + 7) / 8 * 8, kThreads, 0, st>>>(g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return ca3::fail(e, "gram_partial_kernel");
     gram_reduce_kernel<<<dim3(16, g.npairs), 256, 0, st>>>(g.P, g.n, g.ntile, g.npairs, g.ksplit, alpha, beta, C,

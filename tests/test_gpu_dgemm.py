@@ -115,15 +115,17 @@ def test_dgemm_rejects_bad_arguments(hip):
         hip.dgemm(0, 0, 4, 4, 4, 1.0, dA, dA, 0.0, dA, lda=2)
 
 
-@pytest.mark.parametrize("variant", [("CA3DMM_DGEMM_LOADS", "dtl"), ("CA3DMM_DGEMM_TILE", "256x128"),
+@pytest.mark.parametrize("variant", [("CA3DMM_DGEMM_LOADS", "reg"), ("CA3DMM_DGEMM_TILE", "256x128"),
                                      ("CA3DMM_DGEMM_TILE", "128x256")])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_dgemm_variants_same_bits(hip, monkeypatch, variant, ta, tb):
-    """The direct-to-LDS 128 x 128 form and the 8-wave tile shapes add the
-    same products in the same order as the default kernel: the same bits on
-    random inputs (interior and edge tiles, a k tail), and exact integers."""
+    """The register-staged 128 x 128 form (the default runs interior stages
+    direct-to-LDS) and the 8-wave tile shapes add the same products in the
+    same order as the default kernel: the same bits on random inputs
+    (interior and edge tiles, odd and even stage counts, a k tail), and
+    exact integers."""
     rnd = np.random.default_rng(90 + 2 * ta + tb)
-    for m, n, k in [(640, 520, 700), (1000, 1000, 1000), (256, 384, 32), (130, 129, 17)]:
+    for m, n, k in [(640, 520, 700), (1000, 1000, 1000), (256, 384, 32), (384, 256, 48), (130, 129, 17)]:
         A, B = _mats(rnd, m, n, k, ta, tb, "f")
         dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
         ref = hip.DeviceArray.zeros((m, n))
