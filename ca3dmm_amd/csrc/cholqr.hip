@@ -115,6 +115,7 @@ double *workspace(size_t bytes, int *rc)
 // orthogonality once kappa(A) nears u^-1/2 = 2^26.5.  Above 2^23 the pass
 // is redone with the shift.
 constexpr double kAutoDiagRatio = 0x1p23;
+constexpr int kAutoMaxPasses = 6;
 
 // One pass on Q (m x n): G = Q^T Q (+ shift), R_p = chol(G), Q := Q R_p^{-1}.
 // Returns the Cholesky's info (> 0: breakdown, Q untouched) or a status < 0.
@@ -175,10 +176,14 @@ int hip_cholesky_qr(const int m, const int n, double *A, const int ldA, double *
     const int save = ca3::sync_mode();
     hip_proxy_set_sync(0);
     // plain passes still to do after the last shifted one; a plain pass
-    // that breaks down is redone shifted when the variant allows it
+    // that breaks down is redone shifted when the variant allows it.  One
+    // shifted pass divides kappa by about sqrt(11 (m n + n (n + 1)) u)
+    // ||Q||_F / ||Q||_2 only, so near kappa = 1e12 at m = 2e6 the automatic
+    // variant may need two or three of them before its two plain passes.
+    const int max_passes = (variant == HIP_CHOLQR_AUTO) ? kAutoMaxPasses : 3;
     int plain_left = 2, passes = 0;
     bool shift_next = (variant == HIP_SCHOLQR3);
-    while (plain_left > 0 && passes < 4) {
+    while (plain_left > 0 && passes < max_passes) {
         const bool autoplain = (variant == HIP_CHOLQR_AUTO && !shift_next);
         rc = one_pass(m, n, A, ldA, G, dscal, shift_next, gram_slices, autoplain);
         if (rc > 0 && autoplain) {

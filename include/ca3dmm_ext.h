@@ -153,13 +153,17 @@ int hip_dtrtri_upper(const int n, const double *R, const int ldR, double *X, con
  *   variant HIP_CHOLQR2:  two plain passes (CholeskyQR2);
  *   variant HIP_SCHOLQR3: one shifted pass, G + s I with
  *                         s = 11 (m n + n (n + 1)) 2^-53 ||A||_F^2, then two
- *                         plain passes (shifted CholeskyQR3, for condition
- *                         numbers up to about 2^53);
+ *                         plain passes (shifted CholeskyQR3: the shifted
+ *                         pass divides kappa by about
+ *                         sqrt(11 (m n + n (n + 1)) 2^-53), and the plain
+ *                         passes need what is left below about 2^26; at
+ *                         m = 2e6, n = 512 that is kappa(A) up to about 1e11);
  *   variant HIP_CHOLQR_AUTO: CholeskyQR2, but a plain pass whose Cholesky
  *                         breaks down, or whose max / min of diag(R_p)
  *                         exceeds 2^23 (kappa(A) too close to u^-1/2 for
- *                         CholeskyQR2), is redone shifted and followed by two
- *                         plain passes.
+ *                         CholeskyQR2), is redone shifted, as often as that
+ *                         happens, and followed by two plain passes (at most
+ *                         six passes in all).
  * gram_slices > 0 computes each Gram through hip_ozaki_gram with that many
  * slices, 0 through hip_dsyrk_gram.  Returns 0 on success, LAPACK's info
  * (> 0, the order of the leading minor that is not positive definite) when
