@@ -64,9 +64,13 @@ __global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void dgemm_kernel(Gemm
     epilogue<TL>(g, m0, n0, acc);
 }
 
-// The 128 x 128 tile with direct-to-LDS stages for interior tiles (the k
-// tail and edge tiles through the register-staged checked path).  Same
-// products in the same order as dgemm_kernel<TA, TB, Tile128>.
+// The 128 x 128 tile with direct-to-LDS stages for its whole k-stages (the
+// k tail through the register-staged checked path).  Edge tiles load
+// copies of their last row / column in place of the missing ones, which
+// needs an even count of existing rows (columns) where the operand is
+// contiguous along m (n); otherwise the tile runs the checked path
+// throughout.  Same products in the same order as dgemm_kernel<TA, TB,
+// Tile128>.
 template <int TA, int TB>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
 {
@@ -84,11 +88,12 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
     int kt0 = 0;
-    if (m0 + kBM <= g.M && n0 + kBN <= g.N && g.K >= kBK) {
+    const int mn = min(kBM, g.M - m0), nn = min(kBN, g.N - n0);
+    if ((TA || (mn & 1) == 0) && (!TB || (nn & 1) == 0) && g.K >= kBK) {
         DtlLoader<!TA> da;
         DtlLoader<TB> db;
-        da.init(g.A, g.lda, m0);
-        db.init(g.B, g.ldb, n0);
+        da.init(g.A, g.lda, m0, mn);
+        db.init(g.B, g.ldb, n0, nn);
         const int nfull = g.K / kBK;
         mainloop_dtl<!TA, TB>(da, db, g.lda, g.ldb, nfull, A0, A1, B0, B1, acc);
         kt0 = nfull * kBK;
@@ -139,7 +144,8 @@ bool dtl_choice(const GemmArgs &g)
 {
     const char *v = getenv("CA3DMM_DGEMM_LOADS");
     if (v && !strcmp(v, "reg")) return false;
-    return ((uintptr_t) g.A % 16) == 0 && ((uintptr_t) g.B % 16) == 0 && (g.lda % 2) == 0 && (g.ldb % 2) == 0;
+    return ((uintptr_t) g.A % 16) == 0 && ((uintptr_t) g.B % 16) == 0 && (g.lda % 2) == 0 && (g.ldb % 2) == 0 &&
+           dtl_ld_ok(g.lda) && dtl_ld_ok(g.ldb);
 }
 
 template <int TA, int TB>

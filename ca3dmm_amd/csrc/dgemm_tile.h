@@ -263,7 +263,7 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int kbeg, int kend, boo
 }
 
 // ---------------------------------------------------------------------
-// Direct-to-LDS stages (interior tiles of the 128 x 128 DGEMM, 16-byte
+// Direct-to-LDS stages (whole k-stages of the 128 x 128 DGEMM, 16-byte
 // aligned operands with even leading dimensions): global_load_lds_dwordx4
 // writes lane l's 16 bytes (two doubles) at M0 + 16 l, so one wave
 // instruction fills a 1 KB LDS block; a stage of an operand is 16 blocks
@@ -286,34 +286,52 @@ __device__ __forceinline__ int dtl_index(int k, int x)
     return x * 16 + 2 * ((k >> 1) ^ ((x >> 1) & 7)) + (k & 1);
 }
 
+// Edge tiles: only xn <= 128 x of the tile exist (xn even for XC = 1).  A
+// lane whose slot holds an x >= xn loads the last existing x (XC = 0) or x
+// pair (XC = 1) instead, so every load stays inside the operand; those LDS
+// rows hold copies whose products land in C rows or columns the epilogue
+// never stores.  Addresses are a wave-uniform base per block (scalar
+// registers) plus a 32-bit lane offset, which requires 7 * ld * 8 + 128 <
+// 2^32 bytes for XC = 0 (dtl_ld_ok).
+__host__ __device__ constexpr bool dtl_ld_ok(int64_t ld) { return ld >= 0 && ld <= (int64_t) 1 << 26; }
+
 template <int XC>
 struct DtlLoader {
-    const double *p[4];   // this lane's element for k0 = 0, one per block of the wave
+    const double *base[4];   // block i of this wave for k0 = 0 (wave-uniform)
+    uint32_t off[4];         // this lane's byte offset from base[i]
 
-    __device__ __forceinline__ void init(const double *M, int64_t ld, int x0)
+    __device__ __forceinline__ void init(const double *M, int64_t ld, int x0, int xn = 128)
     {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int b = 4 * w + i;
             if (XC) {
-                p[i] = M + x0 + 2 * (lane ^ ((b & 1) << 3)) + (int64_t) b * ld;
+                // k row b of the stage; chunk c of the row holds x = 2c, 2c + 1
+                base[i] = M + x0 + (int64_t) b * ld;
+                off[i] = 16u * (uint32_t) min(lane ^ ((b & 1) << 3), (xn >> 1) - 1);
             } else {
-                const int x = 8 * b + (lane >> 3);
-                p[i] = M + (int64_t) (x0 + x) * ld + 2 * ((lane & 7) ^ ((x >> 1) & 7));
+                // lines x = 8b .. 8b + 7 of the tile; the lane's k pair is
+                // fixed by its slot (line xl), the line read is x <= xn - 1
+                const int xb = min(8 * b, xn - 1), xl = 8 * b + (lane >> 3), x = min(xl, xn - 1);
+                base[i] = M + (int64_t) (x0 + xb) * ld;
+                off[i] = (uint32_t) ((int64_t) (x - xb) * ld * 8) + 16u * (uint32_t) ((lane & 7) ^ ((xl >> 1) & 7));
             }
         }
     }
 
-    // issue the stage starting at k0 into S (16 x 128 doubles)
+    // issue the stage starting at koff (elements) into S (16 x 128 doubles)
     __device__ __forceinline__ void issue(int64_t koff, double *S) const
     {
-        const int w = threadIdx.x >> 6;
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) (p[i] + koff),
+        for (int i = 0; i < 4; ++i) {
+            const char *g = reinterpret_cast<const char *>(base[i] + koff) + (uint64_t) off[i];
+            __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *) g,
                                              (void __attribute__((address_space(3))) *) (S + (4 * w + i) * 128),
                                              16, 0, 0);
+        }
     }
 };
 

@@ -68,12 +68,13 @@ __global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
     const bool interior = (ti + 1) * kBM <= g.n && (tj + 1) * kBN <= g.n;
     int kt0 = kbeg;
-    if (interior && kend - kbeg >= kBK && ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0) {
+    if (kend - kbeg >= kBK && ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0 && dtl_ld_ok(g.lda)) {
         // whole stages direct-to-LDS (kbeg is a multiple of 16, so the panels
-        // stay 16-byte aligned); the same products in the same order
+        // stay 16-byte aligned; edge tiles load copies of their last line in
+        // place of the missing ones); the same products in the same order
         DtlLoader<0> da, db;
-        da.init(g.A + kbeg, g.lda, ti * kBM);
-        db.init(g.A + kbeg, g.lda, tj * kBN);
+        da.init(g.A + kbeg, g.lda, ti * kBM, min(kBM, g.n - ti * kBM));
+        db.init(g.A + kbeg, g.lda, tj * kBN, min(kBN, g.n - tj * kBN));
         const int nfull = (kend - kbeg) / kBK;
         mainloop_dtl<0, 0>(da, db, g.lda, g.lda, nfull, A0, A1, B0, B1, acc);
         kt0 = kbeg + nfull * kBK;

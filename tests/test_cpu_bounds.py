@@ -353,3 +353,51 @@ def test_dtl_stage_layout_is_consistent_and_conflict_free(xc):
                     byte = 8 * dtl_index(xc, k, x)
                     banks.update({(byte // 4) % 64, (byte // 4 + 1) % 64})
                 assert len(banks) == 64, (xc, X0, ks, half)
+
+
+def dtl_loads(xc, xn, ld):
+    """DtlLoader::init / issue (dgemm_tile.h), restated for one stage at
+    koff = 0 of a tile at x0 = 0 with xn existing x: (LDS double index, k, x
+    of the element loaded there, element offset from the operand origin) of
+    both doubles of every lane's 16-byte load."""
+    out = []
+    for w in range(4):
+        for i in range(4):
+            b = 4 * w + i
+            for lane in range(64):
+                if xc:
+                    base = b * ld
+                    off = 16 * min(lane ^ ((b & 1) << 3), (xn >> 1) - 1)
+                else:
+                    xb, xl = min(8 * b, xn - 1), 8 * b + (lane >> 3)
+                    x = min(xl, xn - 1)
+                    base = xb * ld
+                    off = (x - xb) * ld * 8 + 16 * ((lane & 7) ^ ((xl >> 1) & 7))
+                    assert 0 <= off < 2 ** 32
+                assert off % 16 == 0
+                for h in range(2):
+                    e = base + off // 8 + h
+                    k, x = (b, e - b * ld) if xc else (e % ld, e // ld)
+                    out.append((b * 128 + 2 * lane + h, k, x, e))
+    return out
+
+
+@pytest.mark.parametrize("xc", [0, 1])
+@pytest.mark.parametrize("xn", [128, 127, 120, 104, 64, 9, 8, 2, 1])
+def test_dtl_edge_tile_loads_stay_inside_and_fill_existing_slots(xc, xn):
+    """Edge tiles through the direct-to-LDS stages: every load reads an
+    element of the 16 x xn block that exists (x < xn, k < 16), and every LDS
+    slot of an existing (k, x) holds exactly that element, as for an
+    interior tile; the remaining slots hold copies of existing elements.
+    XC = 1 needs xn even (odd counts take the checked path)."""
+    if xc and xn % 2:
+        pytest.skip("XC = 1 edge tiles need an even count")
+    ld = 1000 if xc else 2048                          # ld >= 128 (XC = 1) or >= 16 (XC = 0)
+    slot = {dtl_index(xc, kk, xx): (kk, xx) for kk in range(16) for xx in range(128)}
+    for lds, k, x, e in dtl_loads(xc, xn, ld):
+        assert 0 <= k < 16 and 0 <= x < xn, (xc, xn, lds, k, x)
+        want_k, want_x = slot[lds]
+        if want_x < xn:
+            assert (k, x) == (want_k, want_x), (xc, xn, lds)
+        else:
+            assert k == want_k, (xc, xn, lds)
