@@ -65,13 +65,14 @@ __global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void dgemm_kernel(Gemm
 }
 
 // The 128 x 128 tile with direct-to-LDS stages for its whole k-stages (the
-// k tail through the register-staged checked path).  Edge tiles load
-// copies of their last row / column in place of the missing ones, which
-// needs an even count of existing rows (columns) where the operand is
-// contiguous along m (n); otherwise the tile runs the checked path
-// throughout.  Same products in the same order as dgemm_kernel<TA, TB,
-// Tile128>.
-template <int TA, int TB>
+// k tail through the register-staged checked path).  WA / WB: 4 for 16-byte
+// loads of an operand (16-byte aligned, even ld), 1 for 4-byte loads.  Edge
+// tiles load copies of their last row / column in place of the missing
+// ones; with 16-byte loads that needs an even count of existing rows
+// (columns) where the operand is contiguous along m (n), otherwise the tile
+// runs the checked path throughout.  Same products in the same order as
+// dgemm_kernel<TA, TB, Tile128>.
+template <int TA, int TB, int WA, int WB>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
 {
     __shared__ __attribute__((aligned(1024))) double A0[kBK * kBM], A1[kBK * kBM];
@@ -89,9 +90,9 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
 
     int kt0 = 0;
     const int mn = min(kBM, g.M - m0), nn = min(kBN, g.N - n0);
-    if ((TA || (mn & 1) == 0) && (!TB || (nn & 1) == 0) && g.K >= kBK) {
-        DtlLoader<!TA> da;
-        DtlLoader<TB> db;
+    if ((TA || WA == 1 || (mn & 1) == 0) && (!TB || WB == 1 || (nn & 1) == 0) && g.K >= kBK) {
+        DtlLoader<!TA, WA> da;
+        DtlLoader<TB, WB> db;
         da.init(g.A, g.lda, m0, mn);
         db.init(g.B, g.ldb, n0, nn);
         const int nfull = g.K / kBK;
@@ -108,11 +109,11 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
     epilogue(g, m0, n0, acc);
 }
 
-template <int TA, int TB>
+template <int TA, int TB, int WA, int WB>
 int launch_dtl(const GemmArgs &g)
 {
     const int tiles = ((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
-    dgemm_dtl_kernel<TA, TB><<<tiles, kThreads, 0, ca3::stream()>>>(g);
+    dgemm_dtl_kernel<TA, TB, WA, WB><<<tiles, kThreads, 0, ca3::stream()>>>(g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return ca3::fail(e, "dgemm_dtl_kernel");
     return HIP_PROXY_OK;
@@ -130,29 +131,46 @@ int launch_tile(const GemmArgs &g)
 
 // Workgroup tile: 128 x 128 (4 waves, two workgroups per CU) unless
 // CA3DMM_DGEMM_TILE selects 256x128 or 128x256 (8 waves, one per CU).
-// The 128 x 128 tile runs its interior stages direct-to-LDS when both
-// operands are 16-byte aligned with even leading dimensions (6 % faster on
-// average over the four 8192^3 cases, same bits); CA3DMM_DGEMM_LOADS=reg
-// keeps the register-staged stages throughout.
+// The 128 x 128 tile loads its whole k-stages direct-to-LDS (6-16 % faster
+// than register staging, same bits) when each operand allows it: 16-byte
+// loads for an operand that is 16-byte aligned with an even leading
+// dimension; 4-byte loads only for an operand contiguous along m or n (XC =
+// 1).  A k-contiguous operand with 4-byte loads (two lines per load) made
+// the 8192^3 products 14-31 % slower than 16-byte loads, and slower than
+// register staging, so such a product stays register-staged.
+// CA3DMM_DGEMM_LOADS=reg keeps the register-staged stages throughout;
+// CA3DMM_DGEMM_LOADS=dword takes 4-byte loads for both operands (a test
+// form).
 int tile_choice()
 {
     const char *v = getenv("CA3DMM_DGEMM_TILE");
     return (v && !strcmp(v, "256x128")) ? 1 : (v && !strcmp(v, "128x256")) ? 2 : 0;
 }
 
-bool dtl_choice(const GemmArgs &g)
+// The direct-to-LDS load width of an operand (4 or 1 words), 0 for none
+// (the product is then register-staged).  xc: the operand is contiguous
+// along m or n.
+int dtl_words(const double *P, int64_t ld, bool xc, bool force_dword)
 {
-    const char *v = getenv("CA3DMM_DGEMM_LOADS");
-    if (v && !strcmp(v, "reg")) return false;
-    return ((uintptr_t) g.A % 16) == 0 && ((uintptr_t) g.B % 16) == 0 && (g.lda % 2) == 0 && (g.ldb % 2) == 0 &&
-           dtl_ld_ok(g.lda) && dtl_ld_ok(g.ldb);
+    if (!dtl_ld_ok(ld)) return 0;
+    if (force_dword) return 1;
+    if (((uintptr_t) P % 16) == 0 && (ld % 2) == 0) return 4;
+    return xc ? 1 : 0;
 }
 
 template <int TA, int TB>
 int launch(const GemmArgs &g)
 {
     const int tc = tile_choice();
-    if (tc == 0 && dtl_choice(g)) return launch_dtl<TA, TB>(g);
+    const char *v = getenv("CA3DMM_DGEMM_LOADS");
+    const bool reg = v && !strcmp(v, "reg"), dword = v && !strcmp(v, "dword");
+    if (tc == 0 && !reg) {
+        const int wa = dtl_words(g.A, g.lda, !TA, dword), wb = dtl_words(g.B, g.ldb, TB, dword);
+        if (wa == 4 && wb == 4) return launch_dtl<TA, TB, 4, 4>(g);
+        if (wa == 4 && wb == 1) return launch_dtl<TA, TB, 4, 1>(g);
+        if (wa == 1 && wb == 4) return launch_dtl<TA, TB, 1, 4>(g);
+        if (wa == 1 && wb == 1) return launch_dtl<TA, TB, 1, 1>(g);
+    }
     switch (tc) {
     case 1: return launch_tile<TA, TB, Tile256x128>(g);
     case 2: return launch_tile<TA, TB, Tile128x256>(g);

@@ -286,17 +286,26 @@ __device__ __forceinline__ int dtl_index(int k, int x)
     return x * 16 + 2 * ((k >> 1) ^ ((x >> 1) & 7)) + (k & 1);
 }
 
-// Edge tiles: only xn <= 128 x of the tile exist (xn even for XC = 1).  A
-// lane whose slot holds an x >= xn loads the last existing x (XC = 0) or x
-// pair (XC = 1) instead, so every load stays inside the operand; those LDS
-// rows hold copies whose products land in C rows or columns the epilogue
-// never stores.  Addresses are a wave-uniform base per block (scalar
-// registers) plus a 32-bit lane offset, which requires 7 * ld * 8 + 128 <
-// 2^32 bytes for XC = 0 (dtl_ld_ok).
+// Edge tiles: only xn <= 128 x of the tile exist.  A lane whose slot holds
+// an x >= xn loads the last existing x instead, so every load stays inside
+// the operand; those LDS rows hold copies whose products land in C rows or
+// columns the epilogue never stores.  Addresses are a wave-uniform base per
+// block (scalar registers) plus a 32-bit lane offset, which requires
+// 7 * ld * 8 + 128 < 2^32 bytes for XC = 0 (dtl_ld_ok).
+//
+// WORDS = 4: one 16-byte load per lane and block (global_load_lds_dwordx4);
+// needs 16-byte aligned chunks (operand 16-byte aligned, even ld) and, for
+// XC = 1, an even xn (a chunk holds an x pair).  WORDS = 1: four 4-byte
+// loads per lane and block (global_load_lds_dword, lane l of load q lands
+// at M0 + 256 q + 4 l), for operands with odd ld or 8-byte alignment and
+// any xn; the LDS image is the same.
 __host__ __device__ constexpr bool dtl_ld_ok(int64_t ld) { return ld >= 0 && ld <= (int64_t) 1 << 26; }
 
+template <int XC, int WORDS = 4>
+struct DtlLoader;
+
 template <int XC>
-struct DtlLoader {
+struct DtlLoader<XC, 4> {
     const double *base[4];   // block i of this wave for k0 = 0 (wave-uniform)
     uint32_t off[4];         // this lane's byte offset from base[i]
 
@@ -335,6 +344,59 @@ struct DtlLoader {
     }
 };
 
+// Load q (0..3) of a block fills LDS bytes [256 q, 256 q + 256) of it: lane
+// l writes the 4-byte half h = l & 1 of double j = (l >> 1) & 1 of 16-byte
+// chunk c = 16 q + (l >> 2).  The offsets are rebuilt per stage from a few
+// lane constants (the vector ALU is idle under the MFMAs), so the loader
+// holds as many registers as the 16-byte one.
+template <int XC>
+struct DtlLoader<XC, 1> {
+    const double *M;
+    int64_t ld;
+    int x0, xn;
+    uint32_t t;     // l >> 2: the chunk within the load's 16
+    uint32_t jh;    // 8 j + 4 h
+
+    __device__ __forceinline__ void init(const double *M_, int64_t ld_, int x0_, int xn_ = 128)
+    {
+        const int lane = threadIdx.x & 63;
+        M = M_; ld = ld_; x0 = x0_; xn = xn_;
+        t = (uint32_t) (lane >> 2);
+        jh = 8u * ((lane >> 1) & 1) + 4u * (lane & 1);
+    }
+
+    __device__ __forceinline__ void issue(int64_t koff, double *S) const
+    {
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int b = 4 * w + i;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double *base;
+                uint32_t off;
+                if (XC) {
+                    // k row b; chunk c = 16 q + t holds x pair p = c ^ 8 (b & 1)
+                    base = M + x0 + (int64_t) b * ld + koff;
+                    const uint32_t x = 2u * (uint32_t) ((16 * q + (int) t) ^ ((b & 1) << 3)) + (jh >> 3);
+                    off = 8u * min(x, (uint32_t) xn - 1u) + (jh & 4u);
+                } else {
+                    // line xl = 8b + 2q + (t >> 3), k pair (t & 7) ^ ((xl >> 1) & 7)
+                    const int xb = min(8 * b + 2 * q, xn - 1);
+                    base = M + (int64_t) (x0 + xb) * ld + koff;
+                    const int xl = 8 * b + 2 * q + (int) (t >> 3), x = min(xl, xn - 1);
+                    off = (uint32_t) ((x - xb) * (int) ld) * 8u +
+                          16u * (uint32_t) ((t & 7) ^ (uint32_t) ((xl >> 1) & 7)) + jh;
+                }
+                auto src = (const void __attribute__((address_space(1))) *) (reinterpret_cast<const char *>(base) +
+                                                                             (uint64_t) off);
+                auto dst = (void __attribute__((address_space(3))) *) (S + (4 * w + i) * 128 + 32 * q);
+                __builtin_amdgcn_global_load_lds(src, dst, 4, 0, 0);
+            }
+        }
+    }
+};
+
 template <int XCA, int XCB>
 __device__ __forceinline__ void stage_mfma_dtl(const double *As, const double *Bs, f64x4 (&acc)[4][4])
 {
@@ -368,8 +430,8 @@ __device__ __forceinline__ void stage_mfma_dtl(const double *As, const double *B
 // touch the buffer the DMA of stage z + 1 writes.  With one array indexed
 // by the stage's parity it cannot tell, and makes the first LDS read after
 // each DMA wait for it, which exposes the whole load latency every stage.
-template <int XCA, int XCB>
-__device__ __forceinline__ void mainloop_dtl(const DtlLoader<XCA> &la, const DtlLoader<XCB> &lb, int64_t lda,
+template <int XCA, int XCB, int WA, int WB>
+__device__ __forceinline__ void mainloop_dtl(const DtlLoader<XCA, WA> &la, const DtlLoader<XCB, WB> &lb, int64_t lda,
                                              int64_t ldb, int nfull, double *A0, double *A1, double *B0,
                                              double *B1, f64x4 (&acc)[4][4])
 {

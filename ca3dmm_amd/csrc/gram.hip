@@ -43,6 +43,10 @@ __device__ __forceinline__ void pair_of(int p, int nt, int *ti, int *tj)
     *tj = i + rem;
 }
 
+// W: 4 for 16-byte direct-to-LDS loads (A 16-byte aligned, even lda), 0 for
+// register-staged stages (both panels are k-contiguous, where 4-byte
+// direct-to-LDS loads lose to register staging).
+template <int W>
 __global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
 {
     __shared__ __attribute__((aligned(1024))) double A0[kBK * kBM], A1[kBK * kBM];
@@ -68,11 +72,11 @@ __global__ __launch_bounds__(kThreads, 2) void gram_partial_kernel(GramArgs g)
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
     const bool interior = (ti + 1) * kBM <= g.n && (tj + 1) * kBN <= g.n;
     int kt0 = kbeg;
-    if (kend - kbeg >= kBK && ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0 && dtl_ld_ok(g.lda)) {
+    if (W > 0 && kend - kbeg >= kBK) {
         // whole stages direct-to-LDS (kbeg is a multiple of 16, so the panels
-        // stay 16-byte aligned; edge tiles load copies of their last line in
+        // keep A's alignment; edge tiles load copies of their last line in
         // place of the missing ones); the same products in the same order
-        DtlLoader<0> da, db;
+        DtlLoader<0, 4> da, db;
         da.init(g.A + kbeg, g.lda, ti * kBM, min(kBM, g.n - ti * kBM));
         db.init(g.A + kbeg, g.lda, tj * kBN, min(kBN, g.n - tj * kBN));
         const int nfull = (kend - kbeg) / kBK;
@@ -153,7 +157,11 @@ int check_args(int n, int k, const double *A, int ldA, const double *C, int ldC)
 int launch(const GramArgs &g, double alpha, double beta, double *C, int ldC, int mirror)
 {
     hipStream_t st = ca3::stream();
-    gram_partial_kernel<<<(g.npairs * g.ksplit + 7) / 8 * 8, kThreads, 0, st>>>(g);
+    const int grid = (g.npairs * g.ksplit + 7) / 8 * 8;
+    if (dtl_ld_ok(g.lda) && ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0)
+        gram_partial_kernel<4><<<grid, kThreads, 0, st>>>(g);
+    else
+        gram_partial_kernel<0><<<grid, kThreads, 0, st>>>(g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return ca3::fail(e, "gram_partial_kernel");
     gram_reduce_kernel<<<dim3(16, g.npairs), 256, 0, st>>>(g.P, g.n, g.ntile, g.npairs, g.ksplit, alpha, beta, C,

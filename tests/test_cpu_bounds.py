@@ -401,3 +401,59 @@ def test_dtl_edge_tile_loads_stay_inside_and_fill_existing_slots(xc, xn):
             assert (k, x) == (want_k, want_x), (xc, xn, lds)
         else:
             assert k == want_k, (xc, xn, lds)
+
+
+def dtl_loads_dword(xc, xn, ld):
+    """DtlLoader<XC, 1>::issue (dgemm_tile.h), restated for one stage at koff
+    = 0 of a tile at x0 = 0: (LDS byte, k, x, byte offset from the operand
+    origin) of every 4-byte load (lane l of load q of block b lands at byte
+    1024 b + 256 q + 4 l)."""
+    out = []
+    for w in range(4):
+        for i in range(4):
+            b = 4 * w + i
+            for q in range(4):
+                for lane in range(64):
+                    t, j, h = lane >> 2, (lane >> 1) & 1, lane & 1
+                    if xc:
+                        base = b * ld * 8
+                        x = min(2 * ((16 * q + t) ^ ((b & 1) << 3)) + j, xn - 1)
+                        off = 8 * x + 4 * h
+                    else:
+                        xb = min(8 * b + 2 * q, xn - 1)
+                        base = xb * ld * 8
+                        xl = 8 * b + 2 * q + (t >> 3)
+                        x = min(xl, xn - 1)
+                        off = (x - xb) * ld * 8 + 16 * ((t & 7) ^ ((xl >> 1) & 7)) + 8 * j + 4 * h
+                    assert 0 <= off < 2 ** 32
+                    e8 = base + off                            # byte offset
+                    assert e8 % 4 == 0
+                    e = e8 // 8
+                    k, x = (b, e - b * ld) if xc else (e % ld, e // ld)
+                    out.append((1024 * b + 256 * q + 4 * lane, k, x, e8))
+    return out
+
+
+@pytest.mark.parametrize("xc", [0, 1])
+@pytest.mark.parametrize("xn", [128, 127, 104, 9, 1])
+@pytest.mark.parametrize("ld", [1001, 2048])
+def test_dtl_dword_loads_fill_the_same_slots(xc, xn, ld):
+    """The 4-byte loads (operands with odd ld or 8-byte alignment): every
+    load reads 4 bytes of an existing element of the stage's block, both
+    halves of a double come from the same element, and every LDS slot of an
+    existing (k, x) receives that element -- the same image as the 16-byte
+    loads."""
+    slot = {dtl_index(xc, kk, xx): (kk, xx) for kk in range(16) for xx in range(128)}
+    halves = {}
+    for byte, k, x, e8 in dtl_loads_dword(xc, xn, ld):
+        assert 0 <= k < 16 and 0 <= x < xn, (xc, xn, byte, k, x)
+        d = byte // 8
+        halves.setdefault(d, set()).add((e8 // 8, (byte % 8) == (e8 % 8)))
+        want_k, want_x = slot[d]
+        if want_x < xn:
+            assert (k, x) == (want_k, want_x), (xc, xn, ld, byte)
+        else:
+            assert k == want_k, (xc, xn, ld, byte)
+    assert len(halves) == 16 * 128
+    for d, hs in halves.items():
+        assert len(hs) == 1 and all(same for _, same in hs), d   # one element, halves in place
