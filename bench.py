@@ -149,6 +149,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tests", type=int, default=5, help="timed executions of the CPU baseline")
     ap.add_argument("--no-ozaki", action="store_true")
+    ap.add_argument("--cond", type=float, default=None, help="one condition number for the cholqr suite")
     ap.add_argument("--suite", default=None,
                     choices=["rank-share", "ozaki-sweep", "ozaki-forms", "grouped", "cholqr", "pmc-calib",
                              "yardstick"],

@@ -13,7 +13,7 @@ Each suite prints JSON lines (one per measurement) on stdout:
                (phi = 0.5), three timed calls each: the kernel-trace workload;
   grouped      the simulated 4x2x3 grid on 10007^3: one launch per rank and
                round vs the persistent grouped launch (untuned / tuned / graph);
-  cholqr       CholeskyQR2 of a 2e6 x 512 matrix with condition 1e2, 1e5, 1e8;
+  cholqr       CholeskyQR of a 2e6 x 512 matrix at condition 1e2, 1e5, 1e8, 1e12 (or --cond): each variant;
   pmc-calib    a plain 1 GiB copy, then one of each hot kernel, for counter runs;
   yardstick    rocBLAS dgemm and INT8 gemm_ex at the same size (never on the path).
 """
@@ -340,13 +340,13 @@ def grouped(args):
 
 # ---------------------------------------------------------------------------
 def cholqr(args):
-    """CholeskyQR of 2e6 x 512 at four condition numbers through
+    """CholeskyQR of 2e6 x 512 at four condition numbers (or --cond) through
     hip_cholesky_qr: CholeskyQR2 with the Gram through FP64 MFMA and through
     Ozaki INT8 (s = 7, 8), shifted CholeskyQR3, and the automatic variant."""
     from ca3dmm_amd import cholqr as CQ
     m, n = 2000000, 512
     dU = None
-    for cond in (1e2, 1e5, 1e8, 1e12):
+    for cond in ((args.cond,) if args.cond else (1e2, 1e5, 1e8, 1e12)):
         t0 = time.perf_counter()
         dA, dU = CQ.test_matrix(m, n, cond, seed=2022, dU=dU)
         log(f"cholqr: matrix kappa={cond:g} built in {time.perf_counter() - t0:.1f} s")

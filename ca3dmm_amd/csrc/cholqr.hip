@@ -20,6 +20,7 @@
 // ||A||_F): one shifted pass makes Q well enough conditioned (kappa(Q) of
 // order u^-1/2) for two plain passes to reach orthogonality of order u.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -117,14 +118,15 @@ double *workspace(size_t bytes, int *rc)
 constexpr double kAutoDiagRatio = 0x1p23;
 constexpr int kAutoMaxPasses = 6;
 
-// One pass on Q (m x n): G = Q^T Q (+ shift), R_p = chol(G), Q := Q R_p^{-1}.
-// Returns the Cholesky's info (> 0: breakdown, Q untouched) or a status < 0.
-// With check_ratio, max and min of diag(R_p) are read back and a ratio above
-// kAutoDiagRatio stops the pass before Q is touched (returns kIllConditioned).
+// One pass on Q (m x n): G = Q^T Q (+ shift), R_p = chol(G), Q' = Q R_p^{-1}
+// into Qo (out of place; Qo == Q: in place through hip_dtrsm).  Returns the
+// Cholesky's info (> 0: breakdown, Q untouched) or a status < 0.  With
+// check_ratio, max and min of diag(R_p) are read back and a ratio above
+// kAutoDiagRatio stops the pass before Q' is formed (returns kIllConditioned).
 constexpr int kIllConditioned = 1 << 30;
 
-int one_pass(int m, int n, double *Q, int ldQ, double *G, double *dscal, bool shifted, int gram_slices,
-             bool check_ratio)
+int one_pass(int m, int n, double *Q, int ldQ, double *Qo, int ldQo, double *G, double *dscal, bool shifted,
+             int gram_slices, bool check_ratio)
 {
     int rc = gram_slices > 0 ? hip_ozaki_gram(n, m, 1.0, Q, ldQ, 0.0, G, n, 0, gram_slices)
                              : hip_dsyrk_gram(n, m, 1.0, Q, ldQ, 0.0, G, n, 0);
@@ -149,7 +151,35 @@ int one_pass(int m, int n, double *Q, int ldQ, double *G, double *dscal, bool sh
     zero_lower_kernel<<<n, 256, 0, ca3::stream()>>>(G, n, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return ca3::fail(e, "zero_lower_kernel");
-    return hip_dtrsm(HipRight, HipUpper, HipNoTrans, HipNonUnit, m, n, 1.0, G, n, Q, ldQ);
+    if (Qo == Q) return hip_dtrsm(HipRight, HipUpper, HipNoTrans, HipNonUnit, m, n, 1.0, G, n, Q, ldQ);
+    return ca3::trsm_right_upper_out(m, n, G, n, Q, ldQ, Qo, ldQo);
+}
+
+// The out-of-place Q buffer: cached like the workspace; nullptr (and the
+// in-place solve) when it cannot be allocated.
+Ws g_qbuf[16];
+
+double *qbuffer(size_t bytes)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    Ws &w = g_qbuf[dev & 15];
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (w.bytes < bytes) {
+        if (w.p) {
+            (void) hipStreamSynchronize(ca3::stream());
+            (void) hipFree(w.p);
+            w.p = nullptr;
+            w.bytes = 0;
+        }
+        if (hipMalloc(&w.p, bytes) != hipSuccess) {
+            (void) hipGetLastError();          // the failed allocation is not an error of the call
+            w.p = nullptr;
+            return nullptr;
+        }
+        w.bytes = bytes;
+    }
+    return (double *) w.p;
 }
 
 }  // namespace
@@ -173,6 +203,13 @@ int hip_cholesky_qr(const int m, const int n, double *A, const int ldA, double *
     double *G = workspace((2 * nn + 2) * sizeof(double), &rc);
     if (rc) return rc;
     double *T = G + nn, *dscal = T + nn;
+    // an m x n buffer (when the device has room for it) lets each pass write
+    // its Q out of place: the passes alternate between A and the buffer, and
+    // an odd count ends with one copy back into A
+    const char *ip = getenv("CA3DMM_CHOLQR_INPLACE");
+    double *W = (ip && !strcmp(ip, "1")) ? nullptr : qbuffer((size_t) m * n * sizeof(double));
+    double *Q = A, *Qo = W ? W : A;
+    int ldQ = ldA, ldQo = W ? m : ldA;
     const int save = ca3::sync_mode();
     hip_proxy_set_sync(0);
     // plain passes still to do after the last shifted one; a plain pass
@@ -185,7 +222,7 @@ int hip_cholesky_qr(const int m, const int n, double *A, const int ldA, double *
     bool shift_next = (variant == HIP_SCHOLQR3);
     while (plain_left > 0 && passes < max_passes) {
         const bool autoplain = (variant == HIP_CHOLQR_AUTO && !shift_next);
-        rc = one_pass(m, n, A, ldA, G, dscal, shift_next, gram_slices, autoplain);
+        rc = one_pass(m, n, Q, ldQ, Qo, ldQo, G, dscal, shift_next, gram_slices, autoplain);
         if (rc > 0 && autoplain) {
             shift_next = true;                          // Q untouched: redo this pass shifted
             continue;
@@ -198,10 +235,14 @@ int hip_cholesky_qr(const int m, const int n, double *A, const int ldA, double *
             if (!rc) rc = hip_copy_matrix_block(8, n, n, T, n, R, ldR);
         }
         if (rc) break;
+        std::swap(Q, Qo);
+        std::swap(ldQ, ldQo);
         ++passes;
         plain_left = shift_next ? 2 : plain_left - 1;
         shift_next = false;
     }
+    // Q holds the last pass's result: in the buffer after an odd count
+    if (!rc && Q != A) rc = hip_copy_matrix_block(8, n, m, Q, ldQ, A, ldA);
     hip_proxy_set_sync(save);
     if (passes_run) *passes_run = passes;
     if (rc) return rc;

@@ -30,6 +30,11 @@ int finish();
 // Number of CUs of the current device (cached).
 int cu_count();
 
+// X := B R^{-1} (B, X m x n, R n x n upper triangular, non-unit) with X not
+// overlapping B: R^{-1} and one FP64 MFMA DGEMM per 128-column block of X
+// straight into X (the in-place hip_dtrsm needs a copy per block).
+int trsm_right_upper_out(int m, int n, const double *R, int ldR, const double *B, int ldB, double *X, int ldX);
+
 // FP64 multiply and add rounded separately.  hipcc contracts a * b + c into
 // one fused multiply-add by default (also through the __dmul_rn / __dadd_rn
 // wrappers, which are plain operators); the epilogues whose rounding order

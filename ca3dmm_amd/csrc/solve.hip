@@ -4,10 +4,14 @@
 // cuda_cublas_dtrsm (:254-275), and the host LAPACKE_dpotrf / LAPACKE_dtrtri
 // calls of examples/example_CholeskyQR.c:120-139.
 //
-//   hip_dpotrf(U)   right-looking Cholesky A = R^T R in one workgroup: the
-//                   n x n matrix (512 x 512 in CholeskyQR) stays L2 resident;
-//                   row k of R is staged in LDS for each rank-1 update.
-//                   Returns LAPACK's info (k+1 when the pivot k is not > 0).
+//   hip_dpotrf(U)   blocked right-looking Cholesky A = R^T R over 64-wide
+//                   block columns: the diagonal block factored in LDS by one
+//                   workgroup, the block row R_jj^{-T} A_j,rest by forward
+//                   substitution (one wave per column), the trailing upper
+//                   triangle updated by the FP64 MFMA Gram (hip_dsyrk_gram,
+//                   alpha = -1, beta = 1).  The strictly lower part of A is
+//                   not touched.  Returns LAPACK's info (k+1 when the pivot k
+//                   is not > 0).
 //   hip_dtrtri_upper  R^{-1} for upper-triangular R: one workgroup per
 //                   column j solves R x = e_j by a column sweep in LDS.
 //   hip_dtrsm(R,U,N,N)  B := alpha * B * R^{-1}: R^{-1} (trtri) followed by
@@ -24,44 +28,78 @@
 
 namespace {
 
-constexpr int kPotrfThreads = 1024;
 constexpr int kMaxN = 8192;            // potrf / trtri keep one column in LDS
 
-__global__ __launch_bounds__(kPotrfThreads) void potrf_upper_kernel(double *A, int n, int64_t lda, int *info)
+constexpr int kPb = 64;                // block width of the blocked Cholesky
+
+// The jb x jb diagonal block at A(j0, j0) (its upper triangle) factored in
+// LDS: R_jj with R_jj^T R_jj = A_jj.  info: 0 written by the first block;
+// a later block does nothing once info is set, and writes j0 + k + 1 when
+// its pivot k is not > 0.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int64_t lda, int j0, int jb, int *info)
 {
-    __shared__ double rowk[kMaxN];
+    __shared__ double L[kPb][kPb + 1];     // L[j][i] = A(j0 + i, j0 + j), i <= j
     __shared__ int bad;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = kPotrfThreads / 64;
+    const int t = threadIdx.x;
+    if (j0 > 0 && *info) return;
     if (t == 0) bad = 0;
+    double *D = A + (int64_t) j0 * lda + j0;
+    for (int e = t; e < jb * jb; e += 256) {
+        const int i = e % jb, j = e / jb;
+        if (i <= j) L[j][i] = D[(int64_t) j * lda + i];
+    }
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
+    for (int k = 0; k < jb; ++k) {
         if (t == 0) {
-            const double d = A[(int64_t) k * lda + k];
-            if (!(d > 0.0)) {
-                bad = k + 1;
-            } else {
-                A[(int64_t) k * lda + k] = sqrt(d);
-            }
+            const double d = L[k][k];
+            if (!(d > 0.0)) bad = k + 1;
+            else L[k][k] = sqrt(d);
         }
         __syncthreads();
         if (bad) break;
-        const double piv = A[(int64_t) k * lda + k];
-        // row k of R: R[k][j] = A[k][j] / R[k][k], j > k
-        for (int j = k + 1 + t; j < n; j += kPotrfThreads) {
-            const double v = A[(int64_t) j * lda + k] / piv;
-            A[(int64_t) j * lda + k] = v;
-            rowk[j] = v;
-        }
+        const double piv = L[k][k];
+        for (int j = k + 1 + t; j < jb; j += 256) L[j][k] = L[j][k] / piv;      // row k of R
         __syncthreads();
-        // trailing update of the upper triangle: A[i][j] -= R[k][i] R[k][j], k < i <= j
-        for (int j = k + 1 + wave; j < n; j += nw) {
-            const double rj = rowk[j];
-            double *col = A + (int64_t) j * lda;
-            for (int i = k + 1 + lane; i <= j; i += 64) col[i] = fma(-rowk[i], rj, col[i]);
+        // A(i, j) -= R(k, i) R(k, j), k < i <= j
+        const int r = jb - k - 1;
+        for (int e = t; e < r * r; e += 256) {
+            const int i = k + 1 + e % r, j = k + 1 + e / r;
+            if (i <= j) L[j][i] = fma(-L[i][k], L[j][k], L[j][i]);
         }
         __syncthreads();
     }
-    if (t == 0) *info = bad;
+    for (int e = t; e < jb * jb; e += 256) {
+        const int i = e % jb, j = e / jb;
+        if (i <= j) D[(int64_t) j * lda + i] = L[j][i];
+    }
+    if (t == 0 && (j0 == 0 || bad)) *info = bad ? j0 + bad : 0;
+}
+
+// Block row: X = R_jj^{-T} A(j0:j0+jb, c) for the columns c >= j0 + jb, in
+// place, by forward substitution; one wave per column, lane i holds row i.
+__global__ __launch_bounds__(256) void potrf_panel_kernel(double *A, int64_t lda, int j0, int jb, int n,
+                                                          const int *info)
+{
+    __shared__ double R[kPb][kPb + 1];     // R[j][i] = R_jj(i, j)
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (*info) return;
+    const double *D = A + (int64_t) j0 * lda + j0;
+    for (int e = t; e < jb * jb; e += 256) {
+        const int i = e % jb, j = e / jb;
+        R[j][i] = (i <= j) ? D[(int64_t) j * lda + i] : 0.0;
+    }
+    __syncthreads();
+    const int c = j0 + jb + blockIdx.x * 4 + wave;
+    if (c >= n) return;
+    double *col = A + (int64_t) c * lda + j0;
+    double x = lane < jb ? col[lane] : 0.0;
+    for (int i = 0; i < jb; ++i) {
+        // x_i is final once the rows above it are eliminated
+        const double xi = __shfl(x, i) / R[i][i];
+        if (lane == i) x = xi;
+        if (lane > i) x = fma(-R[lane][i], xi, x);
+    }
+    if (lane < jb) col[lane] = x;
 }
 
 // X = R^{-1}, both n x n upper triangular (ldx, ldr); one block per column.
@@ -123,9 +161,23 @@ int hip_dpotrf(const char uplo, const int n, double *A, const int ldA)
     int rc = 0;
     int *dinfo = (int *) scratch(256, &rc);
     if (rc) return rc;
-    potrf_upper_kernel<<<1, kPotrfThreads, 0, ca3::stream()>>>(A, n, ldA, dinfo);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return ca3::fail(e, "potrf_upper_kernel");
+    const int save = ca3::sync_mode();
+    hip_proxy_set_sync(0);
+    for (int j0 = 0; j0 < n && !rc; j0 += kPb) {
+        const int jb = std::min(kPb, n - j0), nr = n - j0 - jb;
+        potrf_diag_kernel<<<1, 256, 0, ca3::stream()>>>(A, ldA, j0, jb, dinfo);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { rc = ca3::fail(e, "potrf_diag_kernel"); break; }
+        if (nr == 0) break;
+        potrf_panel_kernel<<<(nr + 3) / 4, 256, 0, ca3::stream()>>>(A, ldA, j0, jb, n, dinfo);
+        e = hipGetLastError();
+        if (e != hipSuccess) { rc = ca3::fail(e, "potrf_panel_kernel"); break; }
+        // trailing upper triangle: A(r, r) -= X^T X, X = A(j0:j0+jb, r)
+        double *X = A + (int64_t) (j0 + jb) * ldA + j0;
+        rc = hip_dsyrk_gram(nr, jb, -1.0, X, ldA, 1.0, A + (int64_t) (j0 + jb) * ldA + j0 + jb, ldA, 0);
+    }
+    hip_proxy_set_sync(save);
+    if (rc) return rc;
     int info = 0;
     CA3_CHECK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, ca3::stream()));
     CA3_CHECK(hipStreamSynchronize(ca3::stream()));
@@ -176,3 +228,27 @@ int hip_dtrsm(hip_proxy_enum_t side, hip_proxy_enum_t uplo, hip_proxy_enum_t tra
 }
 
 }  // extern "C"
+
+int ca3::trsm_right_upper_out(int m, int n, const double *R, int ldR, const double *B, int ldB, double *X, int ldX)
+{
+    if (m < 0 || n < 0 || ldR < std::max(1, n) || ldB < std::max(1, m) || ldX < std::max(1, m))
+        return ca3::fail_msg(HIP_PROXY_ERR_ARG, "trsm_right_upper_out: bad arguments");
+    if (m == 0 || n == 0) return HIP_PROXY_OK;
+    if (n > kMaxN) return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "trsm_right_upper_out: n > %d", kMaxN);
+    int rc = 0;
+    double *Rinv = (double *) scratch(sizeof(double) * (size_t) n * n, &rc);
+    if (rc) return rc;
+    const int save = ca3::sync_mode();
+    hip_proxy_set_sync(0);
+    rc = hip_dtrtri_upper(n, R, ldR, Rinv, n);
+    // column block jb of X = B R^{-1} needs rows 0 .. end(jb) of R^{-1}
+    const int nb = 128;
+    for (int j0 = 0; !rc && j0 < n; j0 += nb) {
+        const int w = std::min(nb, n - j0), kk = j0 + w;
+        rc = hip_dgemm(HipNoTrans, HipNoTrans, m, w, kk, 1.0, B, ldB, Rinv + (size_t) j0 * n, n, 0.0,
+                       X + (size_t) j0 * ldX, ldX);
+    }
+    hip_proxy_set_sync(save);
+    if (rc) return rc;
+    return ca3::finish();
+}

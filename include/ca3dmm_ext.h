@@ -169,6 +169,9 @@ int hip_dtrtri_upper(const int n, const double *R, const int ldR, double *X, con
  * (> 0, the order of the leading minor that is not positive definite) when
  * a Cholesky breaks down and the variant does not recover, < 0 on errors.
  * *passes_run (may be NULL) receives the number of completed passes.
+ * When it can, the call holds a device buffer of m x n doubles (cached
+ * across calls) so that each pass writes Q out of place; otherwise, or with
+ * CA3DMM_CHOLQR_INPLACE=1, the passes solve in place (same result).
  * Synchronous (one read-back of the Cholesky's info per pass).  The
  * reference runs one pass and only prints a warning on breakdown
  * (examples/example_CholeskyQR.c:125-129). */

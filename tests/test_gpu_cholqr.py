@@ -54,6 +54,31 @@ def test_potrf_matches_scipy_and_reports_info(hip):
     assert hip.potrf_upper(6, dB) == 3
 
 
+def test_potrf_blocked_info_and_lower_part(hip):
+    """The blocked factorization (64-wide block columns): the strictly lower
+    part of A is not touched, R matches scipy across several blocks, and a
+    pivot that fails in a later block reports LAPACK's info."""
+    rnd = np.random.default_rng(14)
+    for n in (64, 65, 200, 513):
+        X = rnd.standard_normal((2 * n, n))
+        S = X.T @ X
+        L0 = rnd.standard_normal((n, n))
+        A = np.triu(S) + np.tril(L0, -1)            # junk below the diagonal
+        dA = hip.DeviceArray.from_numpy(A)
+        assert hip.potrf_upper(n, dA) == 0
+        got = dA.numpy()
+        assert np.array_equal(np.tril(got, -1), np.tril(L0, -1)), n
+        R = np.triu(got)
+        assert np.allclose(R, sla.cholesky(S, lower=False), rtol=1e-10, atol=1e-12), n
+    n = 300
+    X = rnd.standard_normal((2 * n, n))
+    S = X.T @ X
+    S[199, :] = 0.0
+    S[:, 199] = 0.0                                 # leading minor of order 200 singular
+    dS = hip.DeviceArray.from_numpy(S)
+    assert hip.potrf_upper(n, dS) == 200
+
+
 def _chol_factor(rnd, n):
     """Upper R of a well-conditioned SPD matrix (kappa(R) of a few units)."""
     X = rnd.standard_normal((4 * n, n))
@@ -159,3 +184,20 @@ def test_cholesky_qr_rejects_bad_arguments(hip):
         hip.cholesky_qr(10, 20, dA, dR)            # m < n
     with pytest.raises(HipError):
         hip.cholesky_qr(20, 10, dA, dR, variant=5, lda=20)
+
+
+def test_cholesky_qr_in_place_equals_out_of_place(hip, monkeypatch):
+    """Passes written out of place through the m x n buffer (default) and
+    solved in place (CA3DMM_CHOLQR_INPLACE=1) give the same bits, for an even
+    (CholeskyQR2) and an odd (shifted CholeskyQR3) number of passes."""
+    m, n = 30000, 200
+    for variant in (hip.CHOLQR2, hip.SCHOLQR3):
+        dA, _ = cholqr.test_matrix(m, n, 1e6, seed=21)
+        monkeypatch.delenv("CA3DMM_CHOLQR_INPLACE", raising=False)
+        dQ1, dR1, p1 = cholqr.cholesky_qr(dA, m, n, variant)
+        monkeypatch.setenv("CA3DMM_CHOLQR_INPLACE", "1")
+        dQ2, dR2, p2 = cholqr.cholesky_qr(dA, m, n, variant)
+        assert p1 == p2
+        assert np.array_equal(dQ1.numpy(), dQ2.numpy()), variant
+        assert np.array_equal(dR1.numpy(), dR2.numpy()), variant
+
