@@ -43,7 +43,7 @@ __global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void dgemm_kernel(Gemm
     __shared__ double Bs[2][kBK * TL::BN];
 
     int tm, tn;
-    tile_coords<TL>(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn);
+    tile_coords<TL>(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn, g.group);
     const int m0 = tm * TL::BM, n0 = tn * TL::BN;
 
     // A as stored (TA = 0) is contiguous along m; B as stored (TB = 0) is
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
     __shared__ __attribute__((aligned(1024))) double B0[kBK * kBN], B1[kBK * kBN];
 
     int tm, tn;
-    tile_coords<Tile128>(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn);
+    tile_coords<Tile128>(blockIdx.x, gridDim.x, g.M, g.N, &tm, &tn, g.group);
     const int m0 = tm * kBM, n0 = tn * kBN;
 
     f64x4 acc[4][4];
@@ -197,6 +197,11 @@ int run(hip_proxy_enum_t transA, hip_proxy_enum_t transB, int m, int n, int k, d
     g.alpha = alpha; g.beta = beta;
     g.A = A; g.lda = ldA; g.B = B; g.ldb = ldB; g.C = C; g.ldc = ldC;
     g.nsrc = nsrc;
+    {
+        const char *v = getenv("CA3DMM_DGEMM_GROUP");
+        const int G = v ? atoi(v) : 16;
+        g.group = (G >= 1 && G <= 256) ? G : 16;
+    }
     for (int t = 0; t < nsrc; ++t) {
         if (!P || !ldp || !P[t] || ldp[t] < m) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm: partial %d", t);
         g.P[t] = P[t];

@@ -46,6 +46,7 @@ struct GemmArgs {
     int64_t ldb;
     double *C;
     int64_t ldc;
+    int group;                      // tile rows walked together (tile_coords)
     int nsrc;                       // partial-C fold in the epilogue
     const double *P[kMaxFoldSrc];
     int64_t ldp[kMaxFoldSrc];
@@ -58,14 +59,14 @@ __device__ __forceinline__ int swz(int kk, int x)
 }
 
 // Linear workgroup id -> (tile row, tile col): XCD-contiguous ranges, then
-// groups of 16 tile-rows walked column by column.
+// groups of G tile-rows (16 unless CA3DMM_DGEMM_GROUP says otherwise)
+// walked column by column.
 template <class TL = Tile128>
-__device__ __forceinline__ void tile_coords(int bid, int nb, int M, int N, int *tm, int *tn)
+__device__ __forceinline__ void tile_coords(int bid, int nb, int M, int N, int *tm, int *tn, int G = 16)
 {
     const int tiles_m = (M + TL::BM - 1) / TL::BM, tiles_n = (N + TL::BN - 1) / TL::BN;
     (void) tiles_n;
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    constexpr int G = 16;
     const int per = G * tiles_n;
     const int grp = bid / per;
     const int first = grp * G;

@@ -149,12 +149,11 @@ def test_grouped_and_gram_loaders_read_their_blocks_as_stored():
 
 
 # --------------------------------------------------------------------------- tile order
-def tile_coords(bid, nb, M, N, bm=BM, bn=BN):
+def tile_coords(bid, nb, M, N, bm=BM, bn=BN, G=16):
     """dgemm_tile.h tile_coords, restated."""
     tiles_m, tiles_n = -(-M // bm), -(-N // bn)
     if nb % 8 == 0:
         bid = (bid & 7) * (nb >> 3) + (bid >> 3)
-    G = 16
     per = G * tiles_n
     grp = bid // per
     first = grp * G
@@ -169,8 +168,9 @@ def tile_coords(bid, nb, M, N, bm=BM, bn=BN):
 def test_tile_order_is_a_bijection(M, N, bm, bn):
     tm_n, tn_n = -(-M // bm), -(-N // bn)
     nb = tm_n * tn_n
-    seen = {tile_coords(b, nb, M, N, bm, bn) for b in range(nb)}
-    assert seen == {(i, j) for i in range(tm_n) for j in range(tn_n)}
+    for G in (1, 4, 8, 16, 32, 256):                  # CA3DMM_DGEMM_GROUP
+        seen = {tile_coords(b, nb, M, N, bm, bn, G) for b in range(nb)}
+        assert seen == {(i, j) for i in range(tm_n) for j in range(tn_n)}, G
 
 
 @pytest.mark.parametrize("G", [1, 4, 8, 16])
