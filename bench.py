@@ -42,6 +42,52 @@ def measured_traffic(kernel, n):
     except (OSError, ValueError, KeyError):
         return None
     return None
+def grid_8000(hip, dA, dB, dC):
+    """The rank-share configuration measured whole: 8000^3 NN on the
+    simulated 4x2x3 grid (24 ranks, 1D column layouts in and out), one
+    ca3dmm_sim exec = redistributions, allgather, 48 Cannon rounds, 24
+    kp = 3 folds and the C redistribution; per-rank launches, then the
+    persistent grouped launch (tile 128 x 128, rows 16, 2 workgroups per
+    CU) replayed from one hipGraph.  The first 8000 x 8000 of the bench
+    operands are the inputs (leading dimension 8192)."""
+    from ca3dmm_amd import engine as E
+    n, ld, p = 8000, dA.shape[0], 24
+
+    # example_AB's 1D column layout: rank r owns columns [scol, scol + ncol)
+    # of all n rows, the first n % p ranks one column more
+    # (calc_block_spos_size, src/utils.c:26-49)
+    def col1d(r):
+        q, rem = divmod(n, p)
+        return (0, n, r * q + min(r, rem), q + (1 if r < rem else 0))
+
+    rects = [col1d(r) for r in range(p)]
+    sim = E.SimGrid(p, n, n, n, 0, 0, rects, rects, rects, proc_grid=[4, 2, 3])
+    Ap = [dA.at(r[2] * ld) for r in rects]
+    Bp = [dB.at(r[2] * ld) for r in rects]
+    Cp = [dC.at(r[2] * ld) for r in rects]
+    run = lambda: sim.exec(Ap, [ld] * p, Bp, [ld] * p, Cp, [ld] * p)
+    fl = 2.0 * n ** 3
+    out = {"workload": "8000^3 NN on the simulated 4x2x3 grid (24 ranks), whole ca3dmm_sim exec"}
+    hip.set_sync(False)
+    try:
+        for mode, grouped, graph in (("per_rank", False, False), ("grouped_graph", True, True)):
+            sim.set_mode(grouped=grouped, graph=graph)
+            if grouped:
+                sim.set_tuning(16, 2, 0)
+            run()
+            hip.sync()
+            t = hip.Timer()
+            t.start()
+            for _ in range(5):
+                run()
+            ms = t.stop() / 5
+            out[mode] = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)}
+    finally:
+        hip.set_sync(True)
+        sim.free()
+    return out
+
+
 def dgemm_kernel_name():
     """The kernel hip_dgemm launches for aligned operands with even leading
     dimensions (every bench operand): the 128 x 128 tile with direct-to-LDS
@@ -53,6 +99,55 @@ def dgemm_kernel_name():
 
 
 METRIC = "FP64-accurate GEMM TFLOP/s: DGEMM-MFMA vs INT8-Ozaki at equal max rel err"
+
+
+def phi_matrix(rnd, shape, phi):
+    """Inputs of the precision comparison: uniform[-0.5, 0.5] * exp(phi N(0,1))."""
+    import numpy as np
+    return rnd.uniform(-0.5, 0.5, shape) * np.exp(phi * rnd.standard_normal(shape))
+
+
+def dd_sample(transA, transB, A, B, ii, jj, chunk=128):
+    """The error yardstick of the bench: C(i, j) = sum_l op(A)(i, l) op(B)(l, j)
+    for the sampled (ii, jj) in double-double arithmetic (products exact by
+    Dekker's split, the k terms summed by a pairwise tree of double-double
+    additions, Knuth's two-sum), rounded to double; and sum_l |op(A)||op(B)|.
+    Returns (ref, ab).  A, B: host arrays as stored (op applied here)."""
+    import numpy as np
+
+    def split(a):
+        t = a * 134217729.0                     # 2^27 + 1
+        hi = t - (t - a)
+        return hi, a - hi
+
+    def two_sum(a, b):
+        s = a + b
+        bb = s - a
+        return s, (a - (s - bb)) + (b - bb)
+
+    opA = A.T if transA else A
+    opB = B.T if transB else B
+    ref = np.empty(len(ii))
+    ab = np.empty(len(ii))
+    for c0 in range(0, len(ii), chunk):
+        x = np.ascontiguousarray(opA[ii[c0:c0 + chunk], :])
+        y = np.ascontiguousarray(opB[:, jj[c0:c0 + chunk]].T)
+        p = x * y
+        xh, xl = split(x)
+        yh, yl = split(y)
+        e = ((xh * yh - p) + xh * yl + xl * yh) + xl * yl     # p + e == x y exactly
+        hi, lo = p, e
+        while hi.shape[1] > 1:
+            if hi.shape[1] % 2:
+                z = np.zeros((hi.shape[0], 1))
+                hi, lo = np.hstack([hi, z]), np.hstack([lo, z])
+            s, t = two_sum(hi[:, 0::2], hi[:, 1::2])
+            t = t + (lo[:, 0::2] + lo[:, 1::2])
+            hi = s + t
+            lo = t - (hi - s)
+        ref[c0:c0 + chunk] = (hi + lo)[:, 0]
+        ab[c0:c0 + chunk] = np.sum(np.abs(x) * np.abs(y), axis=1)
+    return ref, ab
 
 
 def log(msg):
@@ -69,14 +164,12 @@ def ozaki_at_equal_error(hip, m, n, k, phi, rnd, slice_counts=range(6, 12), crt_
     hip_ozaki_choose picks without the search."""
     import numpy as np
 
-    from oracle import oracle as O
-    from oracle import ozaki as Z
-    A = np.asfortranarray(Z.phi_matrix(rnd, (m, k), phi))
-    B = np.asfortranarray(Z.phi_matrix(rnd, (k, n), phi))
+    A = np.asfortranarray(phi_matrix(rnd, (m, k), phi))
+    B = np.asfortranarray(phi_matrix(rnd, (k, n), phi))
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
     dC = hip.DeviceArray.zeros((m, n))
     ii, jj = rnd.integers(0, m, 4096), rnd.integers(0, n, 4096)
-    ref, ab = O.dd_sample(0, 0, m, n, k, A, m, B, k, ii, jj)
+    ref, ab = dd_sample(0, 0, A, B, ii, jj)
     del A, B
     timer = hip.Timer()
     flops = 2.0 * m * n * k
@@ -150,8 +243,9 @@ def main():
     ap.add_argument("--cpu-tests", type=int, default=5, help="timed executions of the CPU baseline")
     ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--cond", type=float, default=None, help="one condition number for the cholqr suite")
+    ap.add_argument("--no-grid", action="store_true", help="skip the 8000^3 simulated-grid line")
     ap.add_argument("--suite", default=None,
-                    choices=["rank-share", "ozaki-sweep", "ozaki-forms", "grouped", "cholqr", "pmc-calib",
+                    choices=["rank-share", "ozaki-sweep", "ozaki-forms", "grouped", "cholqr", "pmc-calib", "dgemm-group",
                              "yardstick"],
                     help="run one of the secondary measurements instead (JSON lines)")
     args = ap.parse_args()
@@ -202,23 +296,17 @@ def main():
 
     # accuracy of the last product of the warm-up against the double-double oracle
     err_stats = None
-    try:
-        from oracle import oracle as O
-        ta, tb, _ = cases[-1]
-        hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
-        hip.sync()
-        got = dC.numpy()
-        ii = rnd.integers(0, n, 512)
-        jj = rnd.integers(0, n, 512)
-        Af, Bf = np.asfortranarray(A), np.asfortranarray(B)
-        ref, ab = O.dd_sample(ta, tb, n, n, n, Af, n, Bf, n, ii, jj)
-        e = np.abs(got[ii, jj] - ref)
-        err_stats = {"max_err_over_absAabsB": float(np.max(e / ab)),
-                     "max_rel_err": float(np.max(e / np.abs(ref))),
-                     "samples": 512, "oracle": "double-double dot products"}
-        del Af, Bf
-    except Exception as ex:  # the oracle is optional for timing
-        log(f"bench: accuracy sample skipped: {ex}")
+    ta, tb, _ = cases[-1]
+    hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC)
+    hip.sync()
+    ii = rnd.integers(0, n, 512)
+    jj = rnd.integers(0, n, 512)
+    got = dC.sample(ii, jj)
+    ref, ab = dd_sample(ta, tb, A, B, ii, jj)
+    e = np.abs(got - ref)
+    err_stats = {"max_err_over_absAabsB": float(np.max(e / ab)),
+                 "max_rel_err": float(np.max(e / np.abs(ref))),
+                 "samples": 512, "oracle": "double-double dot products"}
 
     hip.sync()
     t0 = time.perf_counter()
@@ -247,6 +335,11 @@ def main():
     gpu_ms = timer.stop()
     hip.set_sync(True)
     log("bench: kernel timing done")
+
+    grid = None
+    if not args.no_grid:
+        log("bench: the 8000^3 grid on 4x2x3 (24 simulated ranks)")
+        grid = grid_8000(hip, dA, dB, dC)
 
     # Ozaki at equal max error: phi = 0.5 inputs (uniform[-0.5,0.5]*exp(0.5 N(0,1))),
     # native error measured on sampled entries against double-double, then
@@ -306,6 +399,7 @@ def main():
                      "avg_launch_ms": round(gpu_ms / launches, 4)},
         "accuracy": err_stats,
         "ozaki": ozaki,
+        "grid": grid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -15,6 +15,7 @@ Each suite prints JSON lines (one per measurement) on stdout:
                round vs the persistent grouped launch (untuned / tuned / graph);
   cholqr       CholeskyQR of a 2e6 x 512 matrix at condition 1e2, 1e5, 1e8, 1e12 (or --cond): each variant;
   pmc-calib    a plain 1 GiB copy, then one of each hot kernel, for counter runs;
+  dgemm-group  hip_dgemm at n^3 per tile-row group size (CA3DMM_DGEMM_GROUP);
   yardstick    rocBLAS dgemm and INT8 gemm_ex at the same size (never on the path).
 """
 import json
@@ -37,6 +38,53 @@ def emit(d):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def phi_matrix(rnd, shape, phi):
+    return _bench().phi_matrix(rnd, shape, phi)
+
+
+def dd_sample(transA, transB, A, B, ii, jj):
+    """(ref, ab) of the sampled entries in double-double (bench.dd_sample)."""
+    return _bench().dd_sample(transA, transB, A, B, ii, jj)
+
+
+def _bench():
+    import bench
+    return bench
+
+
+def col1d(nrow, ncol, p):
+    """example_AB's 1D column layout (srow, nrow, scol, ncol) per rank: the
+    first ncol % p ranks own one column more (calc_block_spos_size,
+    src/utils.c:26-49)."""
+    q, rem = divmod(ncol, p)
+    return [(0, nrow, r * q + min(r, rem), q + (1 if r < rem else 0)) for r in range(p)]
+
+
+def grid_redist(m, n, k, p, proc_grid=None):
+    """The redistribution plans of A and B that the library builds for the
+    p-rank grid on m x n x k with the 1D column layout, read back from its
+    engines: {name: {"src", "req", "plans"}} with plans[r] = (send list,
+    send count, receive list, receive count), list rows (srow, scol, nrow,
+    ncol, peer, displ) in the engines' row-major view (rows = columns)."""
+    from ca3dmm_amd import engine as E
+    sim = E.SimGrid(p, m, n, k, 0, 0, col1d(m, k, p), col1d(k, n, p), col1d(m, n, p), proc_grid=proc_grid,
+                    allocate=False)
+    out = {}
+    try:
+        for name in ("A", "B"):
+            src, req, plans = [], [], []
+            for r in range(p):
+                e = sim.engine(r)
+                rd = (e.redist_A if name == "A" else e.redist_B).contents
+                src.append((rd.src_srow, rd.src_scol, rd.src_nrow, rd.src_ncol))
+                req.append((rd.req_srow, rd.req_scol, rd.req_nrow, rd.req_ncol))
+                plans.append((rd.send_list().tolist(), rd.send_cnt, rd.recv_list().tolist(), rd.recv_cnt))
+            out[name] = {"src": src, "req": req, "plans": plans}
+    finally:
+        sim.free()
+    return out
 
 
 def check_blocks(blocks, bufs, dt=8):
@@ -66,7 +114,6 @@ def timed(f, reps=5, warm=1):
 
 # ---------------------------------------------------------------------------
 def rank_share(args):
-    from oracle import oracle as O
     hip.set_sync(False)
     rnd = np.random.default_rng(1)
     # two Cannon rounds of rank 0: C(2000 x 4000) = A0 B0 + A1 B1, k 1334 + 1333
@@ -98,8 +145,8 @@ def rank_share(args):
           "gbs": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / HBM_GBS, 4)})
     del P8, out8
     # pack / unpack of A for all 24 ranks (example_AB 1D column layout -> cuboid layout)
-    w = O.ca3dmm_world(8000, 8000, 8000, 0, 0, 24)
-    red = w["redist"]["A"]
+    plans = grid_redist(8000, 8000, 8000, 24)
+    red = plans["A"]
     Afull = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
     sends = [hip.DeviceArray((max(1, red["plans"][r][1]),)) for r in range(24)]
     recvs = [hip.DeviceArray((max(1, red["plans"][r][3]),)) for r in range(24)]
@@ -128,7 +175,7 @@ def rank_share(args):
     direct, direct0 = [], []
     bufs = [Afull, Bfull]
     for name, full in (("A", Afull), ("B", Bfull)):
-        red = w["redist"][name]
+        red = plans[name]
         dst = [hip.DeviceArray((max(1, red["req"][r][2] * red["req"][r][3]),)) for r in range(24)]
         bufs += dst
         for r in range(24):
@@ -159,10 +206,11 @@ def rank_share(args):
     B8 = rnd.uniform(-1, 1, (8000, 8000))
     dA8, dB8 = hip.DeviceArray.from_numpy(A8), hip.DeviceArray.from_numpy(B8)
     dC8 = hip.DeviceArray.zeros((8000, 8000))
-    sim = E.SimGrid(24, 8000, 8000, 8000, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=[4, 2, 3])
-    Ap = [dA8.at(r[2] * 8000) for r in w["src_A"]]
-    Bp = [dB8.at(r[2] * 8000) for r in w["src_B"]]
-    Cp = [dC8.at(r[2] * 8000) for r in w["dst_C"]]
+    rects = col1d(8000, 8000, 24)
+    sim = E.SimGrid(24, 8000, 8000, 8000, 0, 0, rects, rects, rects, proc_grid=[4, 2, 3])
+    Ap = [dA8.at(r[2] * 8000) for r in rects]
+    Bp = [dB8.at(r[2] * 8000) for r in rects]
+    Cp = [dC8.at(r[2] * 8000) for r in rects]
     run = lambda: sim.exec(Ap, [8000] * 24, Bp, [8000] * 24, Cp, [8000] * 24)
     fl = 2.0 * 8000 ** 3
     ms = timed(run, reps=5)
@@ -188,20 +236,18 @@ def ozaki_sweep(args):
     form, count): the slice form for s = 3..11 and the CRT form for 10..25
     moduli; `no_worse_than_native` needs both error measures at or below the
     native product's."""
-    from oracle import oracle as O
-    from oracle import ozaki as Z
     rnd = np.random.default_rng(7)
     shapes = [(args.n, args.n, args.n), (65536, 65536, 1024)]
-    forms = [("ozaki-int8", "s", range(3, 12), hip.ozaki_dgemm, Z.products),
+    forms = [("ozaki-int8", "s", range(3, 12), hip.ozaki_dgemm, lambda s: s * (s + 1) // 2),
              ("ozaki-int8-crt", "moduli", range(10, 26), hip.ozaki2_dgemm, lambda c: c)]
     for m, n, k in shapes:
         flops = 2.0 * m * n * k
         for phi in (0.5, 2.0, 4.0):
-            A, B = Z.phi_matrix(rnd, (m, k), phi), Z.phi_matrix(rnd, (k, n), phi)
+            A, B = phi_matrix(rnd, (m, k), phi), phi_matrix(rnd, (k, n), phi)
             dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
             dC = hip.DeviceArray.zeros((m, n))
             ii, jj = rnd.integers(0, m, 4096), rnd.integers(0, n, 4096)
-            ref, ab = O.dd_sample(0, 0, m, n, k, np.asfortranarray(A), m, np.asfortranarray(B), k, ii, jj)
+            ref, ab = dd_sample(0, 0, A, B, ii, jj)
             del A, B
             hip.set_sync(False)
             nat_ms = ms = timed(lambda: hip.dgemm(0, 0, m, n, k, 1.0, dA, dB, 0.0, dC), reps=2)
@@ -233,16 +279,14 @@ def ozaki_forms(args):
     form at 15 moduli, n x n x k (k = --k, default n) with phi = 0.5 inputs;
     one warm-up and three timed calls each, error on 1024 sampled entries
     against double-double."""
-    from oracle import oracle as O
-    from oracle import ozaki as Z
     n = args.n
     k = args.k or n
     rnd = np.random.default_rng(1)
-    A, B = Z.phi_matrix(rnd, (n, k), 0.5), Z.phi_matrix(rnd, (k, n), 0.5)
+    A, B = phi_matrix(rnd, (n, k), 0.5), phi_matrix(rnd, (k, n), 0.5)
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
     dC = hip.DeviceArray.zeros((n, n))
     ii, jj = rnd.integers(0, n, 1024), rnd.integers(0, n, 1024)
-    ref, ab = O.dd_sample(0, 0, n, n, k, np.asfortranarray(A), n, np.asfortranarray(B), k, ii, jj)
+    ref, ab = dd_sample(0, 0, A, B, ii, jj)
     del A, B
     flops = 2.0 * n * n * k
     hip.set_sync(False)
@@ -269,16 +313,15 @@ def grouped(args):
     FP64 form per launch / grouped untuned / tuned / hipGraph, then the
     Ozaki CRT form per rank / grouped / hipGraph, with the error of every
     form on 512 sampled entries against double-double."""
-    from oracle import oracle as O
     m = n = k = 10007
     p = 24
     rnd = np.random.default_rng(3)
-    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=[4, 2, 3])
+    lay_A, lay_B, lay_C = col1d(m, k, p), col1d(k, n, p), col1d(m, n, p)
     A = rnd.uniform(-1, 1, (m, k))
     B = rnd.uniform(-1, 1, (k, n))
     dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
     ii, jj = rnd.integers(0, m, 512), rnd.integers(0, n, 512)
-    ref, ab = O.dd_sample(0, 0, m, n, k, np.asfortranarray(A), m, np.asfortranarray(B), k, ii, jj)
+    ref, ab = dd_sample(0, 0, A, B, ii, jj)
     del A, B
 
     def errs():
@@ -286,10 +329,10 @@ def grouped(args):
         e = np.abs(got - ref)
         return {"max_err_over_absAabsB": float(np.max(e / ab)), "max_rel_err": float(np.max(e / np.abs(ref)))}
     dC = hip.DeviceArray.zeros((m, n))
-    Ap = [dA.at(r[2] * m) for r in w["src_A"]]
-    Bp = [dB.at(r[2] * k) for r in w["src_B"]]
-    Cp = [dC.at(r[2] * m) for r in w["dst_C"]]
-    sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=[4, 2, 3])
+    Ap = [dA.at(r[2] * m) for r in lay_A]
+    Bp = [dB.at(r[2] * k) for r in lay_B]
+    Cp = [dC.at(r[2] * m) for r in lay_C]
+    sim = E.SimGrid(p, m, n, k, 0, 0, lay_A, lay_B, lay_C, proc_grid=[4, 2, 3])
     flops = 2.0 * m * n * k
     run = lambda: sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
     hip.set_sync(False)
@@ -390,7 +433,6 @@ def pmc_calib(args):
     product n^3; then the NN DGEMM register-staged throughout, the fold over
     8000 x 8000, one 8000^3 grid exec per-rank and one grouped, and the Gram
     of a 2e6 x 512 operand."""
-    from oracle import oracle as O
     nel = (1 << 30) // 8
     src = hip.DeviceArray.zeros((nel,))
     dst = hip.DeviceArray.zeros((nel,))
@@ -407,8 +449,8 @@ def pmc_calib(args):
     P = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (2000, 1334))) for _ in range(3)]
     out = hip.DeviceArray.zeros((2000, 1334))
     hip.fold_sum(P, 2000, 1334, out)
-    w = O.ca3dmm_world(8000, 8000, 8000, 0, 0, 24)
-    red = w["redist"]["A"]
+    red = grid_redist(8000, 8000, 8000, 24)["A"]
+    rects = col1d(8000, 8000, 24)
     Afull = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
     sends = [hip.DeviceArray((max(1, red["plans"][r][1]),)) for r in range(24)]
     pack = []
@@ -442,10 +484,10 @@ def pmc_calib(args):
     dA8 = Afull
     dB8 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (8000, 8000)))
     dC8 = hip.DeviceArray.zeros((8000, 8000))
-    sim = E.SimGrid(24, 8000, 8000, 8000, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=[4, 2, 3])
-    Ap = [dA8.at(r[2] * 8000) for r in w["src_A"]]
-    Bp = [dB8.at(r[2] * 8000) for r in w["src_B"]]
-    Cp = [dC8.at(r[2] * 8000) for r in w["dst_C"]]
+    sim = E.SimGrid(24, 8000, 8000, 8000, 0, 0, rects, rects, rects, proc_grid=[4, 2, 3])
+    Ap = [dA8.at(r[2] * 8000) for r in rects]
+    Bp = [dB8.at(r[2] * 8000) for r in rects]
+    Cp = [dC8.at(r[2] * 8000) for r in rects]
     sim.exec(Ap, [8000] * 24, Bp, [8000] * 24, Cp, [8000] * 24)
     sim.set_mode(grouped=True, graph=False)
     sim.set_tuning(8, 2, 0)
@@ -462,6 +504,28 @@ def pmc_calib(args):
           "pack_elements": sum(d[4] * d[5] for d in pack), "ozaki": f"{n}^3 s=7",
           "ozaki_crt": f"{n}^3 moduli=15", "grid": "8000^3 on 4x2x3: per-rank exec, then grouped exec",
           "gram": "512 x 512 from 2e6 x 512"})
+
+
+def dgemm_group(args):
+    """hip_dgemm at n^3 for each tile-row group size (CA3DMM_DGEMM_GROUP =
+    4, 8, 16, 32, 64): one untimed and three timed launches per trans case
+    (the counter form of the sweep: one timed launch, --steps 1)."""
+    n = args.n
+    rnd = np.random.default_rng(3)
+    dA = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+    dB = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+    dC = hip.DeviceArray.zeros((n, n))
+    hip.set_sync(False)
+    reps = max(1, min(args.steps, 3))
+    for G in (4, 8, 16, 32, 64):
+        os.environ["CA3DMM_DGEMM_GROUP"] = str(G)
+        out = {}
+        for ta, tb in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            ms = timed(lambda: hip.dgemm(ta, tb, n, n, n, 1.0, dA, dB, 0.0, dC), reps=reps)
+            out[f"{'NT'[ta]}{'NT'[tb]}"] = round(2.0 * n ** 3 / ms / 1e9, 2)
+        emit({"suite": "dgemm-group", "n": n, "group": G, "tflops": out})
+    del os.environ["CA3DMM_DGEMM_GROUP"]
+    hip.set_sync(True)
 
 
 def yardstick(args):
@@ -513,4 +577,4 @@ def yardstick(args):
 
 def run(name, args):
     {"yardstick": yardstick, "rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "ozaki-forms": ozaki_forms, "grouped": grouped, "cholqr": cholqr,
-     "pmc-calib": pmc_calib}[name](args)
+     "pmc-calib": pmc_calib, "dgemm-group": dgemm_group}[name](args)
