@@ -182,3 +182,35 @@ def test_dgemm_odd_leading_dimensions_and_offsets(hip, monkeypatch, ta, tb):
             if kind == "int":
                 want = (A.T if ta else A) @ (B.T if tb else B)
                 assert np.array_equal(got.numpy(), want), (m, n, k, lda, ldb)
+
+
+def test_dgemm_random_shapes_layouts_match_register_staged(hip, monkeypatch):
+    """Seeded random shapes, leading dimensions, operand offsets and trans
+    cases (16-byte, 4-byte and register-staged loads; interior, edge and
+    one-tile products; k tails): the default kernel choice gives the bits of
+    the register-staged kernel, and integer products are exact."""
+    rnd = np.random.default_rng(2026)
+    for case in range(40):
+        m, n = (int(v) for v in rnd.integers(1, 700, 2))
+        k = int(rnd.integers(1, 400))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        ra, ca = (k, m) if ta else (m, k)
+        rb, cb = (n, k) if tb else (k, n)
+        lda, ldb = ra + int(rnd.integers(0, 4)), rb + int(rnd.integers(0, 4))
+        oa, ob = int(rnd.integers(0, 4)), int(rnd.integers(0, 4))
+        A = rnd.integers(-8, 9, (ra, ca)).astype(np.float64)
+        B = rnd.integers(-8, 9, (rb, cb)).astype(np.float64)
+        bufA, bufB = np.zeros(oa + lda * ca), np.zeros(ob + ldb * cb)
+        for c in range(ca):
+            bufA[oa + c * lda:oa + c * lda + ra] = A[:, c]
+        for c in range(cb):
+            bufB[ob + c * ldb:ob + c * ldb + rb] = B[:, c]
+        dA, dB = hip.DeviceArray.from_numpy(bufA), hip.DeviceArray.from_numpy(bufB)
+        got, ref = hip.DeviceArray.filled((m, n), 0xFF), hip.DeviceArray.filled((m, n), 0xFF)
+        monkeypatch.delenv("CA3DMM_DGEMM_LOADS", raising=False)
+        hip.dgemm(ta, tb, m, n, k, 1.0, _At(dA.at(oa)), _At(dB.at(ob)), 0.0, got, lda=lda, ldb=ldb, ldc=m)
+        monkeypatch.setenv("CA3DMM_DGEMM_LOADS", "reg")
+        hip.dgemm(ta, tb, m, n, k, 1.0, _At(dA.at(oa)), _At(dB.at(ob)), 0.0, ref, lda=lda, ldb=ldb, ldc=m)
+        want = (A.T if ta else A) @ (B.T if tb else B)
+        assert np.array_equal(got.numpy(), want), (case, m, n, k, ta, tb, lda, ldb, oa, ob)
+        assert np.array_equal(ref.numpy(), want), (case, m, n, k, ta, tb, lda, ldb, oa, ob)
