@@ -1153,10 +1153,16 @@ struct ca3dmm_sim {
     int grouped = 0;              // Cannon rounds + k fold as one persistent launch
     int use_graph = 0;            // replay that launch from a captured hipGraph
     int ozaki = 0;                // > 0: the local products through the CRT form with this many moduli
+    int ozaki_auto = 0;           // choose that count at every exec from the operands in place
+    double ozaki_spread = NAN;    // the spread statistic L of the last automatic choice
     ca3::grouped::Plan *plan = nullptr;
     ca3::grouped::OzPlan *ozplan = nullptr;
-    ca3::grouped::SeqGraph *seq = nullptr;  // the whole exec sequence, captured (use_graph)
-    std::vector<uintptr_t> seq_key;         // arguments and settings it was captured for
+    uint64_t plan_gen = 0;        // plans built so far: a captured sequence holds their buffers
+    // captured exec sequences (use_graph) and the arguments and settings each
+    // was captured for: [0] the whole exec, [1] and [2] its operand and
+    // product halves (automatic modulus count)
+    ca3::grouped::SeqGraph *seq[3] = {nullptr, nullptr, nullptr};
+    std::vector<uintptr_t> seq_key[3];
 };
 
 namespace {
@@ -1173,7 +1179,7 @@ int cannon_member(const Problem &P, int r, int row, int col)
 // per active rank (its q Cannon rounds as k-segments read in place), one
 // fold group per mn-position when kp > 1.
 int collect_grouped(ca3dmm_sim *S, std::vector<ca3::grouped::Problem> &probs,
-                    std::vector<ca3::grouped::Fold> &folds)
+                    std::vector<ca3::grouped::Fold> &folds, std::vector<int> *ranks = nullptr)
 {
     const Problem &P = S->P;
     namespace G = ca3::grouped;
@@ -1226,6 +1232,7 @@ int collect_grouped(ca3dmm_sim *S, std::vector<ca3::grouped::Problem> &probs,
         pr.ldc = Am;
         pr.fold = (kp > 1) ? fold_of[r % mn] : -1;
         probs.push_back(pr);
+        if (ranks) ranks->push_back(r);
     }
     return 0;
 }
@@ -1235,6 +1242,7 @@ int build_grouped(ca3dmm_sim *S)
     std::vector<ca3::grouped::Problem> probs;
     std::vector<ca3::grouped::Fold> folds;
     int rc = collect_grouped(S, probs, folds);
+    ++S->plan_gen;
     return rc ? rc : ca3::grouped::build(probs, folds, &S->plan);
 }
 
@@ -1243,13 +1251,56 @@ int build_ozaki(ca3dmm_sim *S)
     std::vector<ca3::grouped::Problem> probs;
     std::vector<ca3::grouped::Fold> folds;
     int rc = collect_grouped(S, probs, folds);
+    ++S->plan_gen;
     return rc ? rc : ca3::grouped::build_ozaki(probs, folds, S->ozaki, &S->ozplan);
 }
 
-// The exec sequence of all ranks, enqueued on the library stream without
-// synchronising (so that it can be captured whole into a hipGraph).
-int sim_sequence(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
-                 const int *ldB, void *const *dst_C, const int *ldC)
+// The automatic modulus count, on the operands in place: sample s of
+// hip_ozaki_choose's 4096 (entry (i_s, j_s) of the whole C, same hashes)
+// is taken in k layer l = s mod kp, from the local problem of the layer-l
+// rank whose C block holds it, so a and b run over layer l's k range; the
+// count rule then takes the longest layer's k.  Sets *L to the spread
+// statistic it used.  Synchronous.
+int choose_ozaki_count(ca3dmm_sim *S, int *nmod, double *L)
+{
+    const Problem &P = S->P;
+    std::vector<ca3::grouped::Problem> probs;
+    std::vector<ca3::grouped::Fold> folds;
+    std::vector<int> ranks;
+    int rc = collect_grouped(S, probs, folds, &ranks);
+    if (rc) return rc;
+    const int mC = P.btb ? P.n : P.m, nC = P.n, kp = P.grid.kp;
+    int kmax = 0;
+    for (int l = 0; l < kp; ++l) {
+        int s0 = 0, len = 0;
+        calc_block_spos_size(P.k, kp, l, &s0, &len);
+        kmax = std::max(kmax, len);
+    }
+    std::vector<int> samples;
+    if (mC > 0 && nC > 0) {
+        for (uint64_t s = 0; s < 4096; ++s) {
+            const int i = (int) (((s * 2654435761ull + 2135587861ull) & 0xffffffffull) % (uint64_t) mC);
+            const int j = (int) (((s * 2246822519ull + 3266489917ull) & 0xffffffffull) % (uint64_t) nC);
+            const int layer = (int) (s % (uint64_t) kp);
+            for (size_t t = 0; t < probs.size(); ++t) {
+                const ca3dmm_engine_p e = S->eng[ranks[t]];
+                if (P.plans[ranks[t]].rank_k != layer) continue;
+                const int li = i - e->C_2dmm_srow, lj = j - e->C_2dmm_scol;
+                if (li < 0 || li >= probs[t].M || lj < 0 || lj >= probs[t].N) continue;
+                samples.push_back((int) t);
+                samples.push_back(li);
+                samples.push_back(lj);
+                break;
+            }
+        }
+    }
+    return ca3::grouped::choose_moduli(probs, samples, kmax, nmod, L);
+}
+
+// Steps 1 and 2 of the exec: every rank's operand blocks in place for its
+// Cannon rounds.
+int sim_operands(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
+                 const int *ldB)
 {
     const Problem &P = S->P;
     const int p = P.p;
@@ -1339,6 +1390,16 @@ int sim_sequence(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const 
         }
         if (rc) return rc;
     }
+    return 0;
+}
+
+// Steps 3 to 5: the local products, the k-layer fold and the redistribution
+// of C.
+int sim_products(ca3dmm_sim *S, void *const *dst_C, const int *ldC)
+{
+    const Problem &P = S->P;
+    const int p = P.p;
+    int rc = 0;
 
     // 3.+4. through the CRT form of the Ozaki scheme: every rank's product
     //    over all its rounds, the k-layer fold in layer order on the
@@ -1393,6 +1454,10 @@ int sim_sequence(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const 
 
     // 5. redistribute C
     if (P.have_dstC) {
+        std::vector<mat_redist_engine_p> re(p);
+        std::vector<const void *> sp(p);
+        std::vector<void *> dp(p);
+        std::vector<int> sl(p), dl(p);
         for (int r = 0; r < p; ++r) {
             ca3dmm_engine_p e = S->eng[r];
             re[r] = e->redist_C; sp[r] = e->C_out_d; sl[r] = std::max(1, e->C_out_nrow);
@@ -1401,6 +1466,15 @@ int sim_sequence(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const 
         if ((rc = redist_run(p, re.data(), sp.data(), sl.data(), dp.data(), dl.data()))) return rc;
     }
     return 0;
+}
+
+// The exec sequence of all ranks, enqueued on the library stream without
+// synchronising (so that it can be captured whole into a hipGraph).
+int sim_sequence(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
+                 const int *ldB, void *const *dst_C, const int *ldC)
+{
+    const int rc = sim_operands(S, src_A, ldA, src_B, ldB);
+    return rc ? rc : sim_products(S, dst_C, ldC);
 }
 
 // What a captured sequence depends on: every argument of the exec, the
@@ -1413,6 +1487,7 @@ std::vector<uintptr_t> sequence_key(ca3dmm_sim *S, const void *const *src_A, con
     const int p = S->P.p;
     k.push_back((uintptr_t) S->grouped);
     k.push_back((uintptr_t) S->ozaki);
+    k.push_back((uintptr_t) S->plan_gen);
     k.push_back((uintptr_t) redist_staged());
     if (S->plan) {
         const ca3::grouped::Tuning &t = ca3::grouped::tuning(S->plan);
@@ -1436,33 +1511,71 @@ std::vector<uintptr_t> sequence_key(ca3dmm_sim *S, const void *const *src_A, con
     return k;
 }
 
+// Run body on the library stream, or with use_graph replay it from the
+// hipGraph of slot q (captured again when its key changes).
+int run_sequence(ca3dmm_sim *S, int q, const std::vector<uintptr_t> &key, const std::function<int()> &body)
+{
+    if (!S->use_graph) return body();
+    int rc = 0;
+    if (!S->seq[q] || key != S->seq_key[q]) {
+        (void) hip_stream_sync(nullptr);
+        ca3::grouped::destroy_sequence(S->seq[q]);
+        S->seq[q] = nullptr;
+        S->seq_key[q].clear();
+        rc = ca3::grouped::capture_sequence(&S->seq[q], body);
+        if (!rc) S->seq_key[q] = key;
+    }
+    return rc ? rc : ca3::grouped::replay_sequence(S->seq[q]);
+}
+
+void drop_sequences(ca3dmm_sim *S)
+{
+    (void) hip_stream_sync(nullptr);
+    for (int q = 0; q < 3; ++q) {
+        ca3::grouped::destroy_sequence(S->seq[q]);
+        S->seq[q] = nullptr;
+        S->seq_key[q].clear();
+    }
+}
+
 // One exec of the whole grid.  With use_graph the sequence -- the
 // redistributions of A and B, transposes, allgathers, the products and the
 // fold, the redistribution of C -- is captured into one hipGraph on the first
 // call (again when an argument or setting changes) and replayed after that.
+// With the automatic modulus count the operands are put in place first, the
+// count is chosen on them (synchronous), the Ozaki plan is rebuilt when it
+// changes, then the products run; use_graph then replays each half from a
+// hipGraph of its own.
 int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void *const *src_B,
              const int *ldB, void *const *dst_C, const int *ldC)
 {
     int rc = 0;
     // the plans allocate and synchronise: build them outside any capture
-    if (S->ozaki && !S->ozplan && (rc = build_ozaki(S))) return rc;
-    if (!S->ozaki && S->grouped && !S->plan && (rc = build_grouped(S))) return rc;
+    if (!S->ozaki_auto && S->ozaki && !S->ozplan && (rc = build_ozaki(S))) return rc;
+    if (!S->ozaki && !S->ozaki_auto && S->grouped && !S->plan && (rc = build_grouped(S))) return rc;
     const bool save = ca3::grouped::sync_mode();
     hip_proxy_set_sync(0);
-    auto body = [&]() { return sim_sequence(S, src_A, ldA, src_B, ldB, dst_C, ldC); };
-    if (S->use_graph) {
-        std::vector<uintptr_t> key = sequence_key(S, src_A, ldA, src_B, ldB, dst_C, ldC);
-        if (!S->seq || key != S->seq_key) {
-            (void) hip_stream_sync(nullptr);
-            ca3::grouped::destroy_sequence(S->seq);
-            S->seq = nullptr;
-            S->seq_key.clear();
-            rc = ca3::grouped::capture_sequence(&S->seq, body);
-            if (!rc) S->seq_key = key;
-        }
-        if (!rc) rc = ca3::grouped::replay_sequence(S->seq);
+    const std::vector<uintptr_t> key = sequence_key(S, src_A, ldA, src_B, ldB, dst_C, ldC);
+    if (!S->ozaki_auto) {
+        rc = run_sequence(S, 0, key, [&]() { return sim_sequence(S, src_A, ldA, src_B, ldB, dst_C, ldC); });
     } else {
-        rc = body();
+        rc = run_sequence(S, 1, key, [&]() { return sim_operands(S, src_A, ldA, src_B, ldB); });
+        int N = 0;
+        double L = NAN;
+        if (!rc) rc = choose_ozaki_count(S, &N, &L);
+        if (!rc && N != S->ozaki && S->ozplan) {
+            (void) hip_stream_sync(nullptr);
+            ca3::grouped::destroy_ozaki(S->ozplan);
+            S->ozplan = nullptr;
+        }
+        if (!rc) {
+            S->ozaki = N;
+            S->ozaki_spread = L;
+        }
+        if (!rc && !S->ozplan) rc = build_ozaki(S);
+        if (!rc)
+            rc = run_sequence(S, 2, sequence_key(S, src_A, ldA, src_B, ldB, dst_C, ldC),
+                              [&]() { return sim_products(S, dst_C, ldC); });
     }
     hip_proxy_set_sync(save ? 1 : 0);
     if (rc) return rc;
@@ -1555,13 +1668,27 @@ int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph)
 
 int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli)
 {
-    if (!sim || nmoduli < 0 || (nmoduli > 0 && (nmoduli < 4 || nmoduli > 25))) return HIP_PROXY_ERR_ARG;
+    if (!sim || nmoduli < -1 || (nmoduli > 0 && (nmoduli < 4 || nmoduli > 25))) return HIP_PROXY_ERR_ARG;
+    const int auto_count = nmoduli == -1;
+    if (auto_count != sim->ozaki_auto) drop_sequences(sim);
+    sim->ozaki_auto = auto_count;
+    if (auto_count) return HIP_PROXY_OK;      // the count and plan of the last exec stay until the next
+    sim->ozaki_spread = NAN;
     if (nmoduli != sim->ozaki && sim->ozplan) {
-        (void) hip_stream_sync(nullptr);
+        drop_sequences(sim);
         ca3::grouped::destroy_ozaki(sim->ozplan);
         sim->ozplan = nullptr;
     }
     sim->ozaki = nmoduli;
+    return HIP_PROXY_OK;
+}
+
+int ca3dmm_sim_get_ozaki(ca3dmm_sim_p sim, int *nmoduli, int *automatic, double *spread)
+{
+    if (!sim) return HIP_PROXY_ERR_ARG;
+    if (nmoduli) *nmoduli = sim->ozaki;
+    if (automatic) *automatic = sim->ozaki_auto;
+    if (spread) *spread = sim->ozaki_spread;
     return HIP_PROXY_OK;
 }
 
@@ -1624,8 +1751,7 @@ int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu, int t
 void ca3dmm_sim_free(ca3dmm_sim_p *sim)
 {
     if (!sim || !*sim) return;
-    (void) hip_stream_sync(nullptr);
-    ca3::grouped::destroy_sequence((*sim)->seq);
+    drop_sequences(*sim);
     if ((*sim)->plan) ca3::grouped::destroy((*sim)->plan);
     if ((*sim)->ozplan) {
         (void) hip_stream_sync(nullptr);

@@ -84,6 +84,13 @@ void set_tuning_ozaki(OzPlan *p, const OzTuning &t);
 // sequence from a captured hipGraph.
 int run_ozaki(OzPlan *p, bool grouped, bool use_graph);
 void destroy_ozaki(OzPlan *p);
+// Modulus count for the CRT products of these problems: the count rule of
+// hip_ozaki_choose (ca3dmm_ext.h) with the largest problem k, on the spread
+// statistics of the sampled entries samples[3 s .. 3 s + 2] = (problem,
+// row, column), each over the whole k of its problem; *L (may be null) gets
+// the spread statistic L the rule read.  Synchronous.
+int choose_moduli(const std::vector<Problem> &probs, const std::vector<int> &samples, int k, int *nmod,
+                  double *L);
 bool lookup_tuning_ozaki(const std::string &sig, OzTuning *t);
 void store_tuning_ozaki(const std::string &sig, const OzTuning &t);
 

@@ -1114,26 +1114,22 @@ __global__ __launch_bounds__(256) void ozaki2_residue_kernel(const double *__res
 // sum a, sum b, max a, max b (one workgroup per sample; an estimate, so the
 // summation order is free).
 constexpr int kSpreadSamples = 4096;
-__global__ __launch_bounds__(256) void ozaki_spread_kernel(const double *__restrict__ A, int64_t a_sl, int64_t a_sk,
-                                                            int m, const double *__restrict__ B, int64_t b_sl,
-                                                            int64_t b_sk, int n, int k, double *__restrict__ out)
+
+__device__ __forceinline__ void spread_accumulate(double x, double y, double v[7])
 {
-    const int s = blockIdx.x;
-    const uint64_t i = (((uint64_t) s * 2654435761ull + 2135587861ull) & 0xffffffffull) % (uint64_t) m;
-    const uint64_t j = (((uint64_t) s * 2246822519ull + 3266489917ull) & 0xffffffffull) % (uint64_t) n;
-    const double *a = A + (int64_t) i * a_sl;
-    const double *b = B + (int64_t) j * b_sl;
-    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int l = threadIdx.x; l < k; l += 256) {
-        const double x = fabs(a[(int64_t) l * a_sk]), y = fabs(b[(int64_t) l * b_sk]);
-        v[0] = fma(x, y, v[0]);
-        v[1] = fma(x, x, v[1]);
-        v[2] = fma(y, y, v[2]);
-        v[3] += x;
-        v[4] += y;
-        v[5] = fmax(v[5], x);
-        v[6] = fmax(v[6], y);
-    }
+    v[0] = fma(x, y, v[0]);
+    v[1] = fma(x, x, v[1]);
+    v[2] = fma(y, y, v[2]);
+    v[3] += x;
+    v[4] += y;
+    v[5] = fmax(v[5], x);
+    v[6] = fmax(v[6], y);
+}
+
+// Workgroup tree over the 256 threads' partial statistics; thread q < 7
+// writes statistic q.
+__device__ __forceinline__ void spread_reduce_store(const double v[7], double *__restrict__ out)
+{
     __shared__ double red[7][256];
 #pragma unroll
     for (int q = 0; q < 7; ++q) red[q][threadIdx.x] = v[q];
@@ -1148,7 +1144,50 @@ __global__ __launch_bounds__(256) void ozaki_spread_kernel(const double *__restr
         }
         __syncthreads();
     }
-    if (threadIdx.x < 7) out[(int64_t) s * 7 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 7) out[threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ __launch_bounds__(256) void ozaki_spread_kernel(const double *__restrict__ A, int64_t a_sl, int64_t a_sk,
+                                                            int m, const double *__restrict__ B, int64_t b_sl,
+                                                            int64_t b_sk, int n, int k, double *__restrict__ out)
+{
+    const int s = blockIdx.x;
+    const uint64_t i = (((uint64_t) s * 2654435761ull + 2135587861ull) & 0xffffffffull) % (uint64_t) m;
+    const uint64_t j = (((uint64_t) s * 2246822519ull + 3266489917ull) & 0xffffffffull) % (uint64_t) n;
+    const double *a = A + (int64_t) i * a_sl;
+    const double *b = B + (int64_t) j * b_sl;
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int l = threadIdx.x; l < k; l += 256)
+        spread_accumulate(fabs(a[(int64_t) l * a_sk]), fabs(b[(int64_t) l * b_sk]), v);
+    spread_reduce_store(v, out + (int64_t) s * 7);
+}
+
+// The statistics of sampled entries of the grouped launch's problems (the
+// local products of a simulated grid): sample s is entry (i, j) of problem
+// prob, and a, b run over row i of every k-segment's A and column j of its B,
+// so the sums cover the problem's whole k (segments in order).
+struct SpreadProb {
+    const double *A[ca3::grouped::kMaxSeg];
+    const double *B[ca3::grouped::kMaxSeg];
+    int64_t lda[ca3::grouped::kMaxSeg], ldb[ca3::grouped::kMaxSeg];
+    int k[ca3::grouped::kMaxSeg];
+    int nseg;
+};
+
+__global__ __launch_bounds__(256) void ozaki_grid_spread_kernel(const SpreadProb *__restrict__ probs,
+                                                                 const int *__restrict__ samples,
+                                                                 double *__restrict__ out)
+{
+    const int s = blockIdx.x;
+    const SpreadProb &P = probs[samples[3 * s]];
+    const int i = samples[3 * s + 1], j = samples[3 * s + 2];
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < P.nseg; ++g) {
+        const double *a = P.A[g] + i, *b = P.B[g] + (int64_t) j * P.ldb[g];
+        const int64_t lda = P.lda[g];
+        for (int l = threadIdx.x; l < P.k[g]; l += 256)
+            spread_accumulate(fabs(a[(int64_t) l * lda]), fabs(b[l]), v);
+    }
+    spread_reduce_store(v, out + (int64_t) s * 7);
 }
 
 // Exact FP64 pieces of a' = rint(x 2^e), |a'| <= 2^T (the residue kernel's
@@ -1762,6 +1801,25 @@ void *workspace(size_t bytes, int *rc)
     }
     *rc = 0;
     return w.p;
+}
+
+// L = max log2(||a||_2 ||b||_2 / a.b) and L_s = max log2((max(a) ||b||_1 +
+// max(b) ||a||_1) / a.b) over S samples of the seven spread statistics
+// (sum ab, sum a^2, sum b^2, sum a, sum b, max a, max b); samples with an
+// empty line are skipped, and both are 0 when every sample is.
+void spread_exponents(const double *h, int S, double *L, double *Ls)
+{
+    double l = -INFINITY, ls = -INFINITY;
+    for (int q = 0; q < S; ++q) {
+        const double *v = h + (size_t) q * 7;
+        if (!(v[1] > 0.0 && v[2] > 0.0)) continue;         // an empty line: nothing to lose
+        const double lg = std::log2(v[0]);                  // -inf for disjoint supports
+        l = std::max(l, 0.5 * std::log2(v[1]) + 0.5 * std::log2(v[2]) - lg);
+        ls = std::max(ls, std::log2(v[5] * v[4] + v[6] * v[3]) - lg);
+    }
+    if (l == -INFINITY) l = ls = 0.0;
+    *L = l;
+    *Ls = ls;
 }
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
@@ -2431,6 +2489,56 @@ void destroy_ozaki(OzPlan *p)
     delete p;
 }
 
+int choose_moduli(const std::vector<Problem> &probs, const std::vector<int> &samples, int k, int *nmod,
+                  double *Lout)
+{
+    const int ns = (int) samples.size() / 3;
+    for (int q = 0; q < ns; ++q) {
+        const int pi = samples[3 * q], i = samples[3 * q + 1], j = samples[3 * q + 2];
+        if (pi < 0 || pi >= (int) probs.size() || i < 0 || i >= probs[pi].M || j < 0 || j >= probs[pi].N ||
+            probs[pi].nseg < 0 || probs[pi].nseg > kMaxSeg)
+            return ca3::fail_msg(HIP_PROXY_ERR_ARG, "grouped ozaki: spread sample %d outside its problem", q);
+    }
+    double L = 0.0, Ls = 0.0;
+    if (ns > 0) {
+        std::vector<SpreadProb> tab(probs.size());
+        for (size_t t = 0; t < probs.size(); ++t) {
+            SpreadProb &d = tab[t];
+            memset(&d, 0, sizeof(d));
+            d.nseg = probs[t].nseg;
+            for (int g = 0; g < d.nseg; ++g) {
+                d.A[g] = probs[t].seg[g].A;
+                d.B[g] = probs[t].seg[g].B;
+                d.lda[g] = probs[t].seg[g].lda;
+                d.ldb[g] = probs[t].seg[g].ldb;
+                d.k[g] = probs[t].seg[g].k;
+            }
+        }
+        const size_t tb = (tab.size() * sizeof(SpreadProb) + 255) / 256 * 256;
+        const size_t sb = ((size_t) ns * 3 * sizeof(int) + 255) / 256 * 256;
+        int rc = 0;
+        char *w = (char *) workspace(tb + sb + (size_t) ns * 7 * sizeof(double), &rc);
+        if (rc) return rc;
+        hipStream_t st = ca3::stream();
+        std::vector<double> h((size_t) ns * 7);
+        hipError_t e = hipMemcpyAsync(w, tab.data(), tab.size() * sizeof(SpreadProb), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(w + tb, samples.data(), (size_t) ns * 3 * sizeof(int), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            ozaki_grid_spread_kernel<<<ns, 256, 0, st>>>((const SpreadProb *) w, (const int *) (w + tb),
+                                                         (double *) (w + tb + sb));
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h.data(), w + tb + sb, h.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return ca3::fail(e, "grouped ozaki: spread");
+        spread_exponents(h.data(), ns, &L, &Ls);
+    }
+    if (Lout) *Lout = L;
+    return hip_ozaki_counts(L, Ls, k, nullptr, nmod);
+}
+
 }  // namespace grouped
 }  // namespace ca3
 
@@ -2502,15 +2610,8 @@ int hip_ozaki_choose(hip_proxy_enum_t transA, hip_proxy_enum_t transB, const int
     hipError_t e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return ca3::fail(e, "ozaki_choose");
-    double L = -INFINITY, Ls = -INFINITY;
-    for (int q = 0; q < kSpreadSamples; ++q) {
-        const double *v = &h[(size_t) q * 7];
-        if (!(v[1] > 0.0 && v[2] > 0.0)) continue;         // an empty line: nothing to lose
-        const double lg = std::log2(v[0]);                  // -inf for disjoint supports
-        L = std::max(L, 0.5 * std::log2(v[1]) + 0.5 * std::log2(v[2]) - lg);
-        Ls = std::max(Ls, std::log2(v[5] * v[4] + v[6] * v[3]) - lg);
-    }
-    if (L == -INFINITY) L = Ls = 0.0;
+    double L = 0.0, Ls = 0.0;
+    spread_exponents(h.data(), kSpreadSamples, &L, &Ls);
     return hip_ozaki_counts(L, Ls, k, nslices, nmoduli);
 }
 

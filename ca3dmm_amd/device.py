@@ -52,6 +52,15 @@ class DeviceArray:
             check(lib.hip_memset_dev(out.ptr, byte, out.nbytes), "hip_memset_dev")
         return out
 
+    def upload(self, a):
+        """Overwrite the contents with host array a of the same shape."""
+        host = np.asfortranarray(np.asarray(a, dtype=self.dtype))
+        if host.shape != self.shape:
+            raise ValueError(f"upload: shape {host.shape} into {self.shape}")
+        if self.nbytes:
+            check(lib.hip_memcpy_h2d(host.ctypes.data, self.ptr, self.nbytes), "hip_memcpy_h2d")
+        return self
+
     def numpy(self):
         host = np.empty(self.shape, dtype=self.dtype, order="F")
         if self.nbytes:

@@ -117,6 +117,7 @@ _SIGS = {
     "ca3dmm_sim_autotune": (I, [P, I, IP, IP, IP]),
     "ca3dmm_sim_set_tuning": (I, [P, I, I, I]),
     "ca3dmm_sim_set_ozaki": (I, [P, I]),
+    "ca3dmm_sim_get_ozaki": (I, [P, IP, IP, C.POINTER(D)]),
     "ca3dmm_sim_autotune_ozaki": (I, [P, I, IP, IP]),
     "ca3dmm_sim_set_tuning_ozaki": (I, [P, I, I]),
 }
@@ -175,8 +176,17 @@ class SimGrid:
         return g.value, w.value, t.value
 
     def set_ozaki(self, nmoduli):
-        """nmoduli > 0: the local products through the CRT form (0: FP64 MFMA)."""
+        """nmoduli > 0: the local products through the CRT form (0: FP64
+        MFMA); -1: the CRT form with the count chosen at every exec."""
         return check(lib.ca3dmm_sim_set_ozaki(self.h, nmoduli), "ca3dmm_sim_set_ozaki")
+
+    def get_ozaki(self):
+        """(nmoduli, automatic, spread): the count the last exec used (or
+        the one set), whether it is chosen per exec, and the spread
+        statistic L of the last automatic choice (NaN before one)."""
+        n, a, L = C.c_int(), C.c_int(), C.c_double()
+        check(lib.ca3dmm_sim_get_ozaki(self.h, C.byref(n), C.byref(a), C.byref(L)), "ca3dmm_sim_get_ozaki")
+        return n.value, bool(a.value), L.value
 
     def autotune_ozaki(self, reps=3):
         """Tune the Ozaki form's grouped launch; returns (tile_n, xcd_queue)."""

@@ -81,15 +81,30 @@ int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu, int t
  * the products of all ranks run as one persistent launch fed by a device
  * queue (else one launch per rank); use_graph replays the whole sequence
  * (line scales, residues, products, reconstruction and fold) from a
- * captured hipGraph.  0 returns to the FP64 MFMA form.  No reference
- * counterpart (the reference's local GEMM is src/cannon.c:405-427). */
+ * captured hipGraph.  0 returns to the FP64 MFMA form.
+ * nmoduli = -1: the CRT form with the count chosen at every exec, after the
+ * operands are in place, by hip_ozaki_choose's rule on the local products:
+ * its 4096 sampled entries (i_s, j_s) of C (same hashes over the whole
+ * C), sample s taken in k layer s mod kp from the rank whose C block holds
+ * it (a and b over that layer's k range), k = the longest layer.  The exec
+ * then synchronises once before the products (the statistics are read
+ * back), rebuilds the plan when the count changes, and with use_graph
+ * replays the operand and product halves from one hipGraph each.
+ * No reference counterpart (the reference's local GEMM is
+ * src/cannon.c:405-427). */
 int ca3dmm_sim_set_ozaki(ca3dmm_sim_p sim, int nmoduli);
+
+/* The modulus count of the last exec (or the one set; 0: FP64 MFMA form),
+ * whether it is chosen per exec (set_ozaki(-1)), and the spread statistic
+ * L = max log2(||a||_2 ||b||_2 / a.b) of the last automatic choice (NaN
+ * before one).  Any pointer may be NULL.  No reference counterpart. */
+int ca3dmm_sim_get_ozaki(ca3dmm_sim_p sim, int *nmoduli, int *automatic, double *spread);
 
 /* The Ozaki form's grouped launch: time its tunings (workgroup tile 256 x
  * 256 with 8 waves or 256 x 128 with 4 waves; queue items in 8 per-XCD
  * ranges or from one counter), keep the fastest for this grid's shape
- * signature and report it.  Needs set_ozaki(nmoduli > 0) first.  No
- * reference counterpart. */
+ * signature and report it.  Needs set_ozaki(nmoduli > 0) first, or
+ * set_ozaki(-1) and one exec.  No reference counterpart. */
 int ca3dmm_sim_autotune_ozaki(ca3dmm_sim_p sim, int reps, int *tile_n, int *xcd_queue);
 
 /* Fix the Ozaki form's tuning by hand (tile_n 256 or 128). */

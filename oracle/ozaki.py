@@ -466,6 +466,33 @@ def spread(opA, opB, S=SPREAD_SAMPLES):
     return float(L), float(Ls)
 
 
+def grid_spread(opA, opB, layers, S=SPREAD_SAMPLES):
+    """The automatic modulus count of the simulated grid's Ozaki form
+    (engine.cpp, choose_ozaki_count): the sampled entries of spread_samples
+    over the whole m x n C, sample s taken in k layer l = s mod kp, where
+    layers[l] = (k0, kn) is that layer's k range, with a and b over that
+    range only.  Returns (L, k_max): the statistic L as in spread (0 when
+    every sample is empty) and the longest layer, whose count rule is
+    counts_from_spread(L, ., k_max)[1]."""
+    m, n = opA.shape[0], opB.shape[1]
+    kp = len(layers)
+    i, j = spread_samples(m, n, S)
+    lay = np.arange(S) % kp
+    L = -np.inf
+    for l, (k0, kn) in enumerate(layers):
+        sel = lay == l
+        a = np.abs(opA[i[sel], k0:k0 + kn])
+        b = np.abs(opB[k0:k0 + kn][:, j[sel]]).T
+        dot = np.sum(a * b, axis=1)
+        na, nb = np.sqrt(np.sum(a * a, axis=1)), np.sqrt(np.sum(b * b, axis=1))
+        live = (na > 0) & (nb > 0)
+        if np.any(live):
+            with np.errstate(divide="ignore"):
+                L = max(L, float(np.max(np.log2(na[live] * nb[live]) - np.log2(dot[live]))))
+    kmax = max((kn for _, kn in layers), default=0)
+    return (0.0 if L == -np.inf else L), kmax
+
+
 def counts_from_spread(L, Ls, k):
     """(slices, moduli) for the spread statistics, as hip_ozaki_counts."""
     N = CRT_MAX

@@ -89,3 +89,54 @@ def test_rule_reproduces_the_calibration(k, phi, L, Ls, N_meas, s_meas):
     s, N = Z.counts_from_spread(L, Ls, k)
     assert N_meas <= N <= N_meas + 2          # never fewer moduli than measured
     assert s_meas <= s <= s_meas + 1
+
+
+def test_grid_spread_one_layer_is_spread():
+    rnd = np.random.default_rng(21)
+    A, B = Z.phi_matrix(rnd, (90, 300), 2.0), Z.phi_matrix(rnd, (300, 70), 2.0)
+    L, kmax = Z.grid_spread(A, B, [(0, 300)])
+    assert L == pytest.approx(Z.spread(A, B)[0], abs=1e-12) and kmax == 300
+
+
+def test_grid_spread_brute_force():
+    """Per-sample loops in plain Python on a tiny grid: sample s in layer
+    s mod 3, statistics over that layer's k range only."""
+    import math
+    rnd = np.random.default_rng(22)
+    m, n, k, S = 5, 4, 10, 12
+    A, B = Z.phi_matrix(rnd, (m, k), 4.0), Z.phi_matrix(rnd, (k, n), 4.0)
+    A[3, 4:7] = 0.0                          # row 3 empty in layer 1: its samples there are skipped
+    layers = [(0, 4), (4, 3), (7, 3)]
+    best = -math.inf
+    for s in range(S):
+        i = ((s * 2654435761 + 2135587861) % 2 ** 32) % m
+        j = ((s * 2246822519 + 3266489917) % 2 ** 32) % n
+        k0, kn = layers[s % 3]
+        a = [abs(float(A[i, c])) for c in range(k0, k0 + kn)]
+        b = [abs(float(B[c, j])) for c in range(k0, k0 + kn)]
+        na, nb = math.sqrt(sum(x * x for x in a)), math.sqrt(sum(y * y for y in b))
+        if na > 0 and nb > 0:
+            best = max(best, math.log2(na * nb / sum(x * y for x, y in zip(a, b))))
+    L, kmax = Z.grid_spread(A, B, layers, S=S)
+    assert L == pytest.approx(best, abs=1e-12) and kmax == 4
+
+
+def test_grid_spread_keeps_the_layers_apart():
+    """Scaling one layer's k range changes the whole-k spread but not the
+    per-layer one; operands whose supports meet only across layers carry no
+    spread per layer while the whole-k statistic is infinite."""
+    rnd = np.random.default_rng(23)
+    m, n, k = 64, 48, 600
+    layers = [(0, 200), (200, 200), (400, 200)]
+    A, B = Z.phi_matrix(rnd, (m, k), 0.5), Z.phi_matrix(rnd, (k, n), 0.5)
+    A2, B2 = A.copy(), B.copy()
+    A2[:, 200:400] *= 2.0 ** 30
+    B2[200:400, :] *= 2.0 ** -30
+    L, _ = Z.grid_spread(A, B, layers)
+    assert Z.grid_spread(A2, B2, layers)[0] == pytest.approx(L, abs=1e-9)
+    assert Z.spread(A2, B2)[0] > Z.spread(A, B)[0] + 20
+    A3, B3 = A.copy(), B.copy()
+    A3[:, 200:] = 0.0                        # A lives in layer 0 only, B outside it
+    B3[:200, :] = 0.0
+    assert Z.spread(A3, B3)[0] == np.inf
+    assert Z.grid_spread(A3, B3, layers) == (0.0, 200)
