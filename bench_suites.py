@@ -528,6 +528,79 @@ def dgemm_group(args):
     hip.set_sync(True)
 
 
+class _At:
+    """A device address inside a DeviceArray, as an operand of hip.dgemm / hip.gram."""
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+def dgemm_loads(args):
+    """The direct-to-LDS load forms (CA3DMM_DGEMM_LOADS unset = 16-byte
+    loads from any 8-byte aligned operand, "aligned" = 16-byte loads only
+    for 16-byte aligned operands with even leading dimensions, "reg" =
+    register staging) on operands with odd leading dimensions or odd
+    offsets: rank 0's two Cannon rounds of the 8000^3 4x2x3 grid
+    (2000 x 4000 x (1334 + 1333), the second with ldb = 1333), n^3 in the
+    four trans cases with lda = ldb = n + 1, n^3 NN with both operands one
+    double past a 16-byte boundary, and the 512 x 512 Gram of a 1000001 x
+    512 operand.  One untimed and three timed calls each (the rank-share
+    rounds: 20 timed calls, three interleaved passes, the best kept)."""
+    n = args.n
+    rnd = np.random.default_rng(4)
+    hip.set_sync(False)
+    m, nn, k0, k1 = 2000, 4000, 1334, 1333
+    A0 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (m, k0)))
+    A1 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (m, k1)))
+    B0 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (k0, nn)))
+    B1 = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (k1, nn)))
+    Cr = hip.DeviceArray.zeros((m, nn))
+    Ao = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n + 1, n)))
+    Bo = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n + 1, n)))
+    Cn = hip.DeviceArray.zeros((n, n))
+    kg = 1000001
+    G = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (kg, 512)))
+    S = hip.DeviceArray.zeros((512, 512))
+    def use(mode):
+        if mode is None:
+            os.environ.pop("CA3DMM_DGEMM_LOADS", None)
+        else:
+            os.environ["CA3DMM_DGEMM_LOADS"] = mode
+
+    def rounds():
+        hip.dgemm(0, 0, m, nn, k0, 1.0, A0, B0, 0.0, Cr)
+        hip.dgemm(0, 0, m, nn, k1, 1.0, A1, B1, 1.0, Cr)
+    # the rank-share rounds, short launches: the modes interleaved, three
+    # passes of 20 timed calls each, best pass reported
+    best = {}
+    for _ in range(3):
+        for mode in (None, "aligned", "reg"):
+            use(mode)
+            r0 = timed(lambda: hip.dgemm(0, 0, m, nn, k0, 1.0, A0, B0, 0.0, Cr), reps=20)
+            r1 = timed(lambda: hip.dgemm(0, 0, m, nn, k1, 1.0, A1, B1, 1.0, Cr), reps=20)
+            both = timed(rounds, reps=20)
+            b = best.setdefault(mode, [r0, r1, both])
+            best[mode] = [min(x, y) for x, y in zip(b, (r0, r1, both))]
+    for mode in (None, "aligned", "reg"):
+        r0, r1, both = best[mode]
+        emit({"suite": "dgemm-loads", "loads": mode or "16-byte", "rank0_round0_ldb1334_tflops":
+              round(2.0 * m * nn * k0 / r0 / 1e9, 2), "rank0_round1_ldb1333_tflops": round(2.0 * m * nn * k1 / r1 / 1e9, 2),
+              "rank0_both_rounds_tflops": round(2.0 * m * nn * (k0 + k1) / both / 1e9, 2)})
+    for mode in (None, "aligned", "reg"):
+        use(mode)
+        out = {}
+        for ta, tb in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            ms = timed(lambda: hip.dgemm(ta, tb, n, n, n, 1.0, Ao, Bo, 0.0, Cn, lda=n + 1, ldb=n + 1), reps=3)
+            out[f"ld_odd_{'NT'[ta]}{'NT'[tb]}"] = round(2.0 * n ** 3 / ms / 1e9, 2)
+        ms = timed(lambda: hip.dgemm(0, 0, n, n, n, 1.0, _At(Ao.at(1)), _At(Bo.at(1)), 0.0, Cn, lda=n + 1,
+                                     ldb=n + 1), reps=3)
+        out["offset_1_ld_odd_NN"] = round(2.0 * n ** 3 / ms / 1e9, 2)
+        ms = timed(lambda: hip.gram(512, kg, G, S), reps=3)
+        out["gram_512_from_1000001"] = round(513.0 * 512 * kg / ms / 1e9, 2)
+        emit({"suite": "dgemm-loads", "n": n, "loads": mode or "16-byte", "tflops": out})
+    os.environ.pop("CA3DMM_DGEMM_LOADS", None)
+    hip.set_sync(True)
+
+
 def yardstick(args):
     """rocBLAS dgemm and INT8 gemm_ex at n^3, timed as a yardstick for the
     hand-written kernels (the product path never calls rocBLAS)."""
@@ -577,4 +650,4 @@ def yardstick(args):
 
 def run(name, args):
     {"yardstick": yardstick, "rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "ozaki-forms": ozaki_forms, "grouped": grouped, "cholqr": cholqr,
-     "pmc-calib": pmc_calib, "dgemm-group": dgemm_group}[name](args)
+     "pmc-calib": pmc_calib, "dgemm-group": dgemm_group, "dgemm-loads": dgemm_loads}[name](args)

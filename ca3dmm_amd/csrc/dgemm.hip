@@ -132,15 +132,16 @@ int launch_tile(const GemmArgs &g)
 // Workgroup tile: 128 x 128 (4 waves, two workgroups per CU) unless
 // CA3DMM_DGEMM_TILE selects 256x128 or 128x256 (8 waves, one per CU).
 // The 128 x 128 tile loads its whole k-stages direct-to-LDS (6-16 % faster
-// than register staging, same bits) when each operand allows it: 16-byte
-// loads for an operand that is 16-byte aligned with an even leading
-// dimension; 4-byte loads only for an operand contiguous along m or n (XC =
-// 1).  A k-contiguous operand with 4-byte loads (two lines per load) made
-// the 8192^3 products 14-31 % slower than 16-byte loads, and slower than
-// register staging, so such a product stays register-staged.
+// than register staging, same bits) with 16-byte loads, whatever the
+// operand's leading dimension and its address modulo 16 bytes: gfx950
+// serves 16-byte global loads from 8-byte aligned addresses (the compiler
+// itself emits them for such pointers), so a leading dimension of 1333
+// only splits some loads over two cache lines.
 // CA3DMM_DGEMM_LOADS=reg keeps the register-staged stages throughout;
 // CA3DMM_DGEMM_LOADS=dword takes 4-byte loads for both operands (a test
-// form).
+// form); CA3DMM_DGEMM_LOADS=aligned takes 16-byte loads only for 16-byte
+// aligned operands with even leading dimensions, 4-byte loads for other
+// operands contiguous along m or n, and register staging otherwise.
 int tile_choice()
 {
     const char *v = getenv("CA3DMM_DGEMM_TILE");
@@ -150,22 +151,21 @@ int tile_choice()
 // The direct-to-LDS load width of an operand (4 or 1 words), 0 for none
 // (the product is then register-staged).  xc: the operand is contiguous
 // along m or n.
-int dtl_words(const double *P, int64_t ld, bool xc, bool force_dword)
+int dtl_words(const double *P, int64_t ld, bool xc, LoadMode mode)
 {
-    if (!dtl_ld_ok(ld)) return 0;
-    if (force_dword) return 1;
-    if (((uintptr_t) P % 16) == 0 && (ld % 2) == 0) return 4;
-    return xc ? 1 : 0;
+    if (!dtl_ld_ok(ld) || ((uintptr_t) P % 8) != 0 || mode == kLoadsReg) return 0;
+    if (mode == kLoadsDword) return 1;
+    if (mode == kLoadsAligned && (((uintptr_t) P % 16) != 0 || (ld % 2) != 0)) return xc ? 1 : 0;
+    return 4;
 }
 
 template <int TA, int TB>
 int launch(const GemmArgs &g)
 {
     const int tc = tile_choice();
-    const char *v = getenv("CA3DMM_DGEMM_LOADS");
-    const bool reg = v && !strcmp(v, "reg"), dword = v && !strcmp(v, "dword");
-    if (tc == 0 && !reg) {
-        const int wa = dtl_words(g.A, g.lda, !TA, dword), wb = dtl_words(g.B, g.ldb, TB, dword);
+    const LoadMode mode = load_mode();
+    if (tc == 0 && mode != kLoadsReg) {
+        const int wa = dtl_words(g.A, g.lda, !TA, mode), wb = dtl_words(g.B, g.ldb, TB, mode);
         if (wa == 4 && wb == 4) return launch_dtl<TA, TB, 4, 4>(g);
         if (wa == 4 && wb == 1) return launch_dtl<TA, TB, 4, 1>(g);
         if (wa == 1 && wb == 4) return launch_dtl<TA, TB, 1, 4>(g);
@@ -217,6 +217,16 @@ int run(hip_proxy_enum_t transA, hip_proxy_enum_t transB, int m, int n, int k, d
 }
 
 }  // namespace
+
+ca3::gemm::LoadMode ca3::gemm::load_mode()
+{
+    const char *v = getenv("CA3DMM_DGEMM_LOADS");
+    if (!v) return kLoads16;
+    if (!strcmp(v, "reg")) return kLoadsReg;
+    if (!strcmp(v, "dword")) return kLoadsDword;
+    if (!strcmp(v, "aligned")) return kLoadsAligned;
+    return kLoads16;
+}
 
 extern "C" {
 

@@ -294,13 +294,22 @@ __device__ __forceinline__ int dtl_index(int k, int x)
 // block (scalar registers) plus a 32-bit lane offset, which requires
 // 7 * ld * 8 + 128 < 2^32 bytes for XC = 0 (dtl_ld_ok).
 //
-// WORDS = 4: one 16-byte load per lane and block (global_load_lds_dwordx4);
-// needs 16-byte aligned chunks (operand 16-byte aligned, even ld) and, for
-// XC = 1, an even xn (a chunk holds an x pair).  WORDS = 1: four 4-byte
-// loads per lane and block (global_load_lds_dword, lane l of load q lands
-// at M0 + 256 q + 4 l), for operands with odd ld or 8-byte alignment and
-// any xn; the LDS image is the same.
+// WORDS = 4: one 16-byte load per lane and block (global_load_lds_dwordx4)
+// of a double pair, from an 8-byte aligned address (an odd ld or an
+// operand 8 bytes past a 16-byte boundary splits some loads over two
+// cache lines); for XC = 1 it needs an even xn (a chunk holds an x pair).
+// WORDS = 1: four 4-byte loads per lane and block (global_load_lds_dword,
+// lane l of load q lands at M0 + 256 q + 4 l), for any xn; the LDS image
+// is the same.
 __host__ __device__ constexpr bool dtl_ld_ok(int64_t ld) { return ld >= 0 && ld <= (int64_t) 1 << 26; }
+
+// Load forms of the direct-to-LDS stages (CA3DMM_DGEMM_LOADS): 16-byte
+// loads from any 8-byte aligned operand (default); "aligned": 16-byte
+// loads only for 16-byte aligned operands with even ld, 4-byte loads for
+// other operands contiguous along m or n, register staging otherwise;
+// "dword": 4-byte loads; "reg": register staging only.
+enum LoadMode { kLoads16, kLoadsAligned, kLoadsDword, kLoadsReg };
+LoadMode load_mode();
 
 template <int XC, int WORDS = 4>
 struct DtlLoader;

@@ -158,7 +158,12 @@ int launch(const GramArgs &g, double alpha, double beta, double *C, int ldC, int
 {
     hipStream_t st = ca3::stream();
     const int grid = (g.npairs * g.ksplit + 7) / 8 * 8;
-    if (dtl_ld_ok(g.lda) && ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0)
+    // 16-byte direct-to-LDS loads from any 8-byte aligned A (as the DGEMM;
+    // CA3DMM_DGEMM_LOADS=aligned keeps them to 16-byte aligned A with even
+    // lda, "reg" and "dword" stage through registers)
+    const LoadMode mode = load_mode();
+    const bool aligned16 = ((uintptr_t) g.A % 16) == 0 && (g.lda % 2) == 0;
+    if (dtl_ld_ok(g.lda) && ((uintptr_t) g.A % 8) == 0 && (mode == kLoads16 || (mode == kLoadsAligned && aligned16)))
         gram_partial_kernel<4><<<grid, kThreads, 0, st>>>(g);
     else
         gram_partial_kernel<0><<<grid, kThreads, 0, st>>>(g);

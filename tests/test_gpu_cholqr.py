@@ -22,6 +22,35 @@ def test_gram_integer_exact_and_symmetric(hip, n, k):
     assert np.array_equal(C, C.T)
 
 
+class _At:
+    """A device address inside a DeviceArray, for hip.gram's A."""
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+@pytest.mark.parametrize("n,k,pad,off", [(130, 1001, 0, 0), (257, 3000, 1, 0), (96, 2048, 0, 1), (300, 777, 2, 3)])
+def test_gram_odd_leading_dimension_and_offset(hip, monkeypatch, n, k, pad, off):
+    """A with an odd leading dimension or starting an odd number of doubles
+    into its buffer: 16-byte direct-to-LDS loads from 8-byte aligned
+    addresses give the register-staged kernel's bits, exact on integers."""
+    rnd = np.random.default_rng(n + k + pad + off)
+    lda = k + pad
+    for kind in ("f", "int"):
+        A = rnd.standard_normal((k, n)) if kind == "f" else rnd.integers(-3, 4, (k, n)).astype(np.float64)
+        buf = np.zeros(off + lda * n)
+        for c in range(n):
+            buf[off + c * lda:off + c * lda + k] = A[:, c]
+        dA = hip.DeviceArray.from_numpy(buf)
+        got, ref = hip.DeviceArray.filled((n, n), 0xFF), hip.DeviceArray.filled((n, n), 0xFF)
+        monkeypatch.setenv("CA3DMM_DGEMM_LOADS", "reg")
+        hip.gram(n, k, _At(dA.at(off)), ref, lda=lda)
+        monkeypatch.delenv("CA3DMM_DGEMM_LOADS", raising=False)
+        hip.gram(n, k, _At(dA.at(off)), got, lda=lda)
+        assert np.array_equal(got.numpy(), ref.numpy()), (n, k, lda, off, kind)
+        if kind == "int":
+            assert np.array_equal(got.numpy(), A.T @ A)
+
+
 def test_gram_upper_only_and_beta(hip):
     rnd = np.random.default_rng(3)
     n, k = 200, 3000

@@ -116,13 +116,14 @@ def test_dgemm_rejects_bad_arguments(hip):
 
 
 @pytest.mark.parametrize("variant", [("CA3DMM_DGEMM_LOADS", "reg"), ("CA3DMM_DGEMM_LOADS", "dword"),
+                                     ("CA3DMM_DGEMM_LOADS", "aligned"),
                                      ("CA3DMM_DGEMM_TILE", "256x128"), ("CA3DMM_DGEMM_TILE", "128x256")])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_dgemm_variants_same_bits(hip, monkeypatch, variant, ta, tb):
     """The register-staged 128 x 128 form (the default loads its stages
-    direct-to-LDS, 16 bytes per lane where aligned), the 4-byte direct-to-LDS
-    loads and the 8-wave tile shapes add the same products in the
-    same order as the default kernel: the same bits on random inputs
+    direct-to-LDS, 16 bytes per lane), the 4-byte direct-to-LDS loads, the
+    16-byte-aligned-only policy and the 8-wave tile shapes add the same
+    products in the same order as the default kernel: the same bits on random inputs
     (interior and edge tiles, odd and even stage counts, a k tail), and
     exact integers."""
     rnd = np.random.default_rng(90 + 2 * ta + tb)
@@ -152,9 +153,11 @@ class _At:
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_dgemm_odd_leading_dimensions_and_offsets(hip, monkeypatch, ta, tb):
     """Operands with odd leading dimensions, or starting one double past a
-    16-byte boundary, take the 4-byte direct-to-LDS loads (16-byte ones for
-    the other operand where it is aligned): the same bits as the
-    register-staged kernel on random inputs, exact on integers."""
+    16-byte boundary, take 16-byte direct-to-LDS loads from 8-byte aligned
+    addresses (with CA3DMM_DGEMM_LOADS=aligned: 4-byte loads where the
+    operand is contiguous along m or n, register staging otherwise): the
+    same bits as the register-staged kernel on random inputs, exact on
+    integers."""
     rnd = np.random.default_rng(150 + 2 * ta + tb)
     for m, n, k, pa, pb in [(300, 260, 333, 1, 0), (257, 131, 160, 0, 1), (640, 384, 211, 1, 1), (129, 255, 48, 3, 5)]:
         ra, ca = (k, m) if ta else (m, k)
@@ -174,11 +177,15 @@ def test_dgemm_odd_leading_dimensions_and_offsets(hip, monkeypatch, ta, tb):
                 bufB[1 + c * ldb:1 + c * ldb + rb] = B[:, c]
             dA, dB = hip.DeviceArray.from_numpy(bufA), hip.DeviceArray.from_numpy(bufB)
             got, ref = hip.DeviceArray.filled((m, n), 0xFF), hip.DeviceArray.filled((m, n), 0xFF)
-            monkeypatch.delenv("CA3DMM_DGEMM_LOADS", raising=False)
-            hip.dgemm(ta, tb, m, n, k, 1.0, _At(dA.at(1)), _At(dB.at(1)), 0.0, got, lda=lda, ldb=ldb, ldc=m)
             monkeypatch.setenv("CA3DMM_DGEMM_LOADS", "reg")
             hip.dgemm(ta, tb, m, n, k, 1.0, _At(dA.at(1)), _At(dB.at(1)), 0.0, ref, lda=lda, ldb=ldb, ldc=m)
-            assert np.array_equal(got.numpy(), ref.numpy()), (m, n, k, lda, ldb, kind)
+            for mode in (None, "aligned"):
+                if mode is None:
+                    monkeypatch.delenv("CA3DMM_DGEMM_LOADS", raising=False)
+                else:
+                    monkeypatch.setenv("CA3DMM_DGEMM_LOADS", mode)
+                hip.dgemm(ta, tb, m, n, k, 1.0, _At(dA.at(1)), _At(dB.at(1)), 0.0, got, lda=lda, ldb=ldb, ldc=m)
+                assert np.array_equal(got.numpy(), ref.numpy()), (m, n, k, lda, ldb, kind, mode)
             if kind == "int":
                 want = (A.T if ta else A) @ (B.T if tb else B)
                 assert np.array_equal(got.numpy(), want), (m, n, k, lda, ldb)
