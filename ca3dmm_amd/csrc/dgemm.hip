@@ -25,11 +25,13 @@
 //    in groups of 16 tile-rows, so the tiles resident on one XCD share A and
 //    B panels in that XCD's L2.
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 
 #include "ca3dmm_ext.h"
 #include "dgemm_tile.h"
+#include "grouped.h"
 #include "hip_common.h"
 
 namespace {
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void dgemm_kernel(Gemm
 
 // The 128 x 128 tile with direct-to-LDS stages for its whole k-stages (the
 // k tail through the register-staged checked path).  WA / WB: 4 for 16-byte
-// loads of an operand (16-byte aligned, even ld), 1 for 4-byte loads.  Edge
+// loads of an operand, 1 for 4-byte loads.  Edge
 // tiles load copies of their last row / column in place of the missing
 // ones; with 16-byte loads that needs an even count of existing rows
 // (columns) where the operand is contiguous along m (n), otherwise the tile
@@ -107,6 +109,96 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_dtl_kernel(GemmArgs g)
         mainloop_checked(la, lb, kt0, g.K, LdsPair{A0, A1}, LdsPair{B0, B1}, acc);
     }
     epilogue(g, m0, n0, acc);
+}
+
+// One problem of a batched launch (rank r's product C := sum over its
+// Cannon rounds of A_s B_s, every operand as stored, NN): its tiles are
+// linear tile indices [base, base + tiles_m tiles_n).
+struct BatchProb {
+    int M, N, nseg;
+    int dtl;                                   // bit s: segment s may take the direct-to-LDS stages
+    int64_t base;
+    double *C;
+    int64_t ldc;
+    const double *A[ca3::grouped::kMaxSeg];
+    const double *B[ca3::grouped::kMaxSeg];
+    int64_t lda[ca3::grouped::kMaxSeg], ldb[ca3::grouped::kMaxSeg];
+    int k[ca3::grouped::kMaxSeg];
+};
+
+// Every problem's 128 x 128 tiles in one launch, one workgroup per tile
+// (gridDim.x = total tiles rounded up to a multiple of 8).  Linear tile
+// indices are laid out XCD-contiguously over the whole batch, so each XCD
+// works through its own problems' panels; inside a problem tiles are
+// walked in groups of `group` tile-rows.  A tile's segments run in order,
+// each as its own product added by the epilogue (beta 0 on the first, 1
+// after): the bits of one dgemm_dtl_kernel<0, 0> launch per segment.
+__global__ __launch_bounds__(kThreads, 2) void dgemm_batch_kernel(const BatchProb *__restrict__ probs, int nprob,
+                                                                    int64_t total, int group)
+{
+    __shared__ __attribute__((aligned(1024))) double A0[kBK * kBM], A1[kBK * kBM];
+    __shared__ __attribute__((aligned(1024))) double B0[kBK * kBN], B1[kBK * kBN];
+
+    const int64_t nb = gridDim.x;
+    int64_t t = blockIdx.x;
+    t = (t & 7) * (nb >> 3) + (t >> 3);
+    if (t >= total) return;
+    int lo = 0, hi = nprob - 1;                // last problem with base <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].base <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchProb &P = probs[lo];
+    int tm, tn;
+    tile_coords<Tile128>((int) (t - P.base), 1, P.M, P.N, &tm, &tn, group);   // nb = 1: no XCD remap here
+    const int m0 = tm * kBM, n0 = tn * kBN;
+    const int mn = min(kBM, P.M - m0), nn = min(kBN, P.N - n0);
+
+    GemmArgs g;
+    g.M = P.M; g.N = P.N;
+    g.alpha = 1.0;
+    g.C = P.C; g.ldc = P.ldc;
+    g.nsrc = 0;
+    for (int s = 0; s < P.nseg; ++s) {
+        f64x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        const int K = P.k[s];
+        int kt0 = 0;
+        int sm0 = m0, sn0 = n0, smn = mn, snn = nn;
+        asm volatile("" : "+s"(sm0), "+s"(sn0), "+s"(smn), "+s"(snn));
+        if (((P.dtl >> s) & 1) && (smn & 1) == 0 && K >= kBK) {
+            DtlLoader<1, 4> da;
+            DtlLoader<0, 4> db;
+            da.init(P.A[s], P.lda[s], sm0, smn);
+            db.init(P.B[s], P.ldb[s], sn0, snn);
+            const int nfull = K / kBK;
+            mainloop_dtl<1, 0>(da, db, P.lda[s], P.ldb[s], nfull, A0, A1, B0, B1, acc);
+            kt0 = nfull * kBK;
+        }
+        if (kt0 < K) {
+            OperandLoader<1> la;
+            OperandLoader<0> lb;
+            la.init(P.A[s], P.lda[s], sm0, P.M);
+            lb.init(P.B[s], P.ldb[s], sn0, P.N);
+            mainloop_checked(la, lb, kt0, K, LdsPair{A0, A1}, LdsPair{B0, B1}, acc);
+        }
+        // the epilogue's addresses are the same on every segment: keep the
+        // compiler from computing them once and holding them (in vector
+        // registers) across the main loops
+        double *C = P.C;
+        int64_t ldc = P.ldc;
+        int em0 = m0, en0 = n0;
+        asm volatile("" : "+s"(C), "+s"(ldc), "+s"(em0), "+s"(en0));
+        g.C = C;
+        g.ldc = ldc;
+        g.K = K;
+        g.beta = s ? 1.0 : 0.0;
+        epilogue(g, em0, en0, acc);
+    }
 }
 
 template <int TA, int TB, int WA, int WB>
@@ -217,6 +309,87 @@ int run(hip_proxy_enum_t transA, hip_proxy_enum_t transB, int m, int n, int k, d
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------
+// Batched launch of the simulated grid's per-rank products (grouped.h)
+// ---------------------------------------------------------------------
+namespace ca3 {
+namespace grouped {
+
+struct BatchPlan {
+    BatchProb *d = nullptr;
+    int nprob = 0;
+    int64_t total = 0;
+    int group = 16;
+};
+
+int build_batch(const std::vector<Problem> &probs, BatchPlan **out)
+{
+    *out = nullptr;
+    std::vector<BatchProb> h;
+    int64_t base = 0;
+    const gemm::LoadMode mode = gemm::load_mode();
+    for (const Problem &q : probs) {
+        if (q.M <= 0 || q.N <= 0) continue;
+        if (q.nseg < 0 || q.nseg > kMaxSeg || !q.C || q.ldc < q.M)
+            return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm batch: bad problem");
+        BatchProb b;
+        memset(&b, 0, sizeof(b));
+        b.M = q.M; b.N = q.N; b.nseg = q.nseg;
+        b.base = base;
+        b.C = q.C; b.ldc = q.ldc;
+        for (int s = 0; s < q.nseg; ++s) {
+            const Seg &g = q.seg[s];
+            if (g.k < 0 || (g.k > 0 && (!g.A || !g.B || g.lda < q.M || g.ldb < g.k)))
+                return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm batch: bad segment %d", s);
+            b.A[s] = g.A; b.B[s] = g.B;
+            b.lda[s] = g.lda; b.ldb[s] = g.ldb;
+            b.k[s] = g.k;
+            if (dtl_words(g.A, g.lda, true, mode) == 4 && dtl_words(g.B, g.ldb, false, mode) == 4)
+                b.dtl |= 1 << s;
+        }
+        h.push_back(b);
+        base += (int64_t) ((q.M + gemm::kBM - 1) / gemm::kBM) * ((q.N + gemm::kBN - 1) / gemm::kBN);
+    }
+    if ((base + 7) / 8 * 8 > INT_MAX) return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "dgemm batch: %lld tiles", (long long) base);
+    auto *p = new BatchPlan;
+    p->nprob = (int) h.size();
+    p->total = base;
+    const char *v = getenv("CA3DMM_DGEMM_GROUP");
+    const int G = v ? atoi(v) : 16;
+    p->group = (G >= 1 && G <= 256) ? G : 16;
+    if (!h.empty()) {
+        hipError_t e = hipMalloc(&p->d, h.size() * sizeof(BatchProb));
+        if (e == hipSuccess) e = hipMemcpy(p->d, h.data(), h.size() * sizeof(BatchProb), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            destroy_batch(p);
+            return ca3::fail(e, "dgemm batch plan");
+        }
+    }
+    *out = p;
+    return HIP_PROXY_OK;
+}
+
+int run_batch(BatchPlan *p)
+{
+    if (!p) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm batch: no plan");
+    if (p->total == 0) return HIP_PROXY_OK;
+    const int grid = (int) ((p->total + 7) / 8 * 8);
+    dgemm_batch_kernel<<<grid, gemm::kThreads, 0, ca3::stream()>>>(p->d, p->nprob, p->total, p->group);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ca3::fail(e, "dgemm_batch_kernel");
+    return ca3::finish();
+}
+
+void destroy_batch(BatchPlan *p)
+{
+    if (!p) return;
+    if (p->d) (void) hipFree(p->d);
+    delete p;
+}
+
+}  // namespace grouped
+}  // namespace ca3
 
 ca3::gemm::LoadMode ca3::gemm::load_mode()
 {

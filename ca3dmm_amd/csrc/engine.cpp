@@ -1157,6 +1157,7 @@ struct ca3dmm_sim {
     double ozaki_spread = NAN;    // the spread statistic L of the last automatic choice
     ca3::grouped::Plan *plan = nullptr;
     ca3::grouped::OzPlan *ozplan = nullptr;
+    ca3::grouped::BatchPlan *batch = nullptr;  // the per-rank rounds as one launch
     uint64_t plan_gen = 0;        // plans built so far: a captured sequence holds their buffers
     // captured exec sequences (use_graph) and the arguments and settings each
     // was captured for: [0] the whole exec, [1] and [2] its operand and
@@ -1253,6 +1254,52 @@ int build_ozaki(ca3dmm_sim *S)
     int rc = collect_grouped(S, probs, folds);
     ++S->plan_gen;
     return rc ? rc : ca3::grouped::build_ozaki(probs, folds, S->ozaki, &S->ozplan);
+}
+
+// The per-rank Cannon rounds (step 3 below) as one batched launch unless
+// CA3DMM_SIM_BATCH=0 asks for one hip_dgemm launch per rank and round.
+bool batch_on()
+{
+    const char *v = getenv("CA3DMM_SIM_BATCH");
+    return !(v && !strcmp(v, "0"));
+}
+
+// The problems of the batched launch: exactly the hip_dgemm calls of the
+// per-rank rounds, one problem per rank with its q rounds as segments in
+// round order (rounds with k = 0 included).
+int build_batch(ca3dmm_sim *S)
+{
+    const Problem &P = S->P;
+    namespace G = ca3::grouped;
+    std::vector<G::Problem> probs;
+    for (int r = 0; r < P.p; ++r) {
+        ca3dmm_engine_p e = S->eng[r];
+        if (!e->is_active) continue;
+        const cannon_engine_p ce = e->cannon_engine;
+        if (ce->m == 0 || ce->n == 0 || ce->k == 0) continue;
+        const int q = ce->np_dim, row = ce->rank_row, col = ce->rank_col;
+        const int Am = ce->m_displs[row + 1] - ce->m_displs[row];
+        const int Bn = ce->n_displs[col + 1] - ce->n_displs[col];
+        if (!Am || !Bn) continue;
+        if (q > G::kMaxSeg) return HIP_PROXY_ERR_UNSUPP;
+        G::Problem pr;
+        memset(&pr, 0, sizeof(pr));
+        pr.M = Am; pr.N = Bn;
+        pr.C = (double *) (e->task_k_num == 1 ? e->C_out_d : e->C_2dmm_d);
+        pr.ldc = std::max(1, Am);
+        pr.fold = -1;
+        for (int s2 = 0; s2 < q; ++s2) {
+            const int off = (row + col + s2) % q;
+            const int lk = ce->k_displs[off + 1] - ce->k_displs[off];
+            G::Seg &sg = pr.seg[pr.nseg++];
+            sg.A = (const double *) S->eng[cannon_member(P, r, row, off)]->A_2dmm_d;
+            sg.B = (const double *) S->eng[cannon_member(P, r, off, col)]->B_2dmm_d;
+            sg.lda = std::max(1, Am); sg.ldb = std::max(1, lk); sg.k = lk;
+        }
+        probs.push_back(pr);
+    }
+    ++S->plan_gen;
+    return G::build_batch(probs, &S->batch);
 }
 
 // The automatic modulus count, on the operands in place: sample s of
@@ -1415,8 +1462,11 @@ int sim_products(ca3dmm_sim *S, void *const *dst_C, const int *ldC)
 
     // 3. Cannon rounds: round s of rank (row, col) multiplies the A block of
     //    (row, (row+col+s) % q) by the B block of ((row+col+s) % q, col),
-    //    beta = 0 in round 0 and 1 afterwards (cannon.c:319-433).
-    for (int r = 0; r < p && !rc && !S->grouped && fp64_rounds; ++r) {
+    //    beta = 0 in round 0 and 1 afterwards (cannon.c:319-433); all ranks'
+    //    rounds in one batched launch (the same bits as the launches below)
+    const bool batched = fp64_rounds && !S->grouped && S->batch && batch_on();
+    if (batched && (rc = ca3::grouped::run_batch(S->batch))) return rc;
+    for (int r = 0; r < p && !rc && !S->grouped && fp64_rounds && !batched; ++r) {
         ca3dmm_engine_p e = S->eng[r];
         if (!e->is_active) continue;
         const cannon_engine_p ce = e->cannon_engine;
@@ -1489,6 +1539,7 @@ std::vector<uintptr_t> sequence_key(ca3dmm_sim *S, const void *const *src_A, con
     k.push_back((uintptr_t) S->ozaki);
     k.push_back((uintptr_t) S->plan_gen);
     k.push_back((uintptr_t) redist_staged());
+    k.push_back((uintptr_t) batch_on());
     if (S->plan) {
         const ca3::grouped::Tuning &t = ca3::grouped::tuning(S->plan);
         k.push_back((uintptr_t) t.tile);
@@ -1553,6 +1604,7 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     // the plans allocate and synchronise: build them outside any capture
     if (!S->ozaki_auto && S->ozaki && !S->ozplan && (rc = build_ozaki(S))) return rc;
     if (!S->ozaki && !S->ozaki_auto && S->grouped && !S->plan && (rc = build_grouped(S))) return rc;
+    if (!S->ozaki && !S->ozaki_auto && !S->grouped && !S->batch && batch_on() && (rc = build_batch(S))) return rc;
     const bool save = ca3::grouped::sync_mode();
     hip_proxy_set_sync(0);
     const std::vector<uintptr_t> key = sequence_key(S, src_A, ldA, src_B, ldB, dst_C, ldC);
@@ -1752,6 +1804,7 @@ void ca3dmm_sim_free(ca3dmm_sim_p *sim)
 {
     if (!sim || !*sim) return;
     drop_sequences(*sim);
+    ca3::grouped::destroy_batch((*sim)->batch);
     if ((*sim)->plan) ca3::grouped::destroy((*sim)->plan);
     if ((*sim)->ozplan) {
         (void) hip_stream_sync(nullptr);

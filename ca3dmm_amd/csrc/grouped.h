@@ -94,6 +94,18 @@ int choose_moduli(const std::vector<Problem> &probs, const std::vector<int> &sam
 bool lookup_tuning_ozaki(const std::string &sig, OzTuning *t);
 void store_tuning_ozaki(const std::string &sig, const OzTuning &t);
 
+// The same problems (every operand NN, as stored) as one non-persistent
+// launch of the direct-to-LDS DGEMM (dgemm.hip): one workgroup per
+// 128 x 128 tile of every problem, its k-segments in order, each added to
+// C as its own product (beta 0 on the first, 1 after), so C gets the bits
+// of one hip_dgemm call per segment.  Segments may have k = 0; folds are
+// not part of it.  The load form (CA3DMM_DGEMM_LOADS) and tile-row group
+// (CA3DMM_DGEMM_GROUP) are read when the plan is built.
+struct BatchPlan;
+int build_batch(const std::vector<Problem> &probs, BatchPlan **out);
+int run_batch(BatchPlan *p);
+void destroy_batch(BatchPlan *p);
+
 // A whole enqueued sequence (every kernel `body` launches on the library
 // stream) captured once into a hipGraph and replayed.  body must not
 // synchronise, allocate or copy through the host while it is captured.

@@ -162,6 +162,45 @@ def tile_coords(bid, nb, M, N, bm=BM, bn=BN, G=16):
     return first + r % gsz, r // gsz
 
 
+def batch_tiles(problems, G=16):
+    """dgemm_batch_kernel (dgemm.hip): workgroup b of the padded grid ->
+    (problem, tile row, tile col), or None for a padding workgroup; problems
+    are (M, N) with their tiles at consecutive linear indices."""
+    bases, total = [], 0
+    for M, N in problems:
+        bases.append(total)
+        total += -(-M // BM) * -(-N // BN)
+    nb = (total + 7) // 8 * 8
+    out = []
+    for b in range(nb):
+        t = (b & 7) * (nb >> 3) + (b >> 3)
+        if t >= total:
+            out.append(None)
+            continue
+        lo, hi = 0, len(problems) - 1
+        while lo < hi:
+            mid = (lo + hi + 1) >> 1
+            if bases[mid] <= t:
+                lo = mid
+            else:
+                hi = mid - 1
+        M, N = problems[lo]
+        out.append((lo,) + tile_coords(t - bases[lo], 1, M, N, G=G))
+    return out
+
+
+@pytest.mark.parametrize("problems", [[(2000, 4000)] * 8 + [(2000, 4000)] * 16,
+                                      [(2502, 5004), (2502, 5003), (2501, 5004), (2501, 5003)] * 6,
+                                      [(1, 1)], [(130, 77), (65, 77), (130, 1), (1, 300)],
+                                      [(128, 128)] * 7, [(8192, 8192)]])
+def test_batch_tiles_cover_every_tile_once(problems):
+    for G in (1, 16):
+        got = [x for x in batch_tiles(problems, G) if x is not None]
+        want = [(p, i, j) for p, (M, N) in enumerate(problems)
+                for i in range(-(-M // BM)) for j in range(-(-N // BN))]
+        assert sorted(got) == sorted(want) and len(got) == len(set(got))
+
+
 @pytest.mark.parametrize("bm,bn", TILES)
 @pytest.mark.parametrize("M,N", [(128, 128), (1000, 1000), (2000, 4000), (8192, 8192), (2501, 5003),
                                  (130, 9000), (9000, 130), (17 * 128, 3 * 128), (100, 100)])

@@ -131,6 +131,31 @@ def test_sim_random_fp_within_bound(hip):
     assert np.all(np.abs(got - A @ B) <= bound + 2.0 ** -53 * np.abs(A @ B))
 
 
+@pytest.mark.parametrize("p,grid,ta,tb,mnk", [(24, [4, 2, 3], 0, 0, (1001, 777, 1999)), (24, [4, 2, 3], 1, 1, (1001, 777, 1999)),
+                                              (12, [2, 3, 2], 0, 1, (640, 900, 1333)), (8, [2, 2, 2], 1, 0, (513, 515, 2050)),
+                                              (6, [2, 3, 1], 0, 0, (700, 530, 900)), (1, [1, 1, 1], 1, 0, (300, 260, 333)),
+                                              (24, [4, 2, 3], 0, 0, (300, 200, 20)), (7, [2, 1, 3], 0, 0, (130, 77, 210))])
+def test_sim_batched_rounds_equal_per_round_launches(hip, monkeypatch, p, grid, ta, tb, mnk):
+    """The per-rank Cannon rounds of every rank as one batched launch give
+    the bits of one hip_dgemm launch per rank and round (CA3DMM_SIM_BATCH=0),
+    eagerly and replayed from a hipGraph, on random FP64 inputs: uneven and
+    odd block sizes, k blocks shorter than a stage, idle ranks."""
+    m, n, k = mnk
+    rnd = np.random.default_rng(m + n + k + p)
+    A = rnd.uniform(-1, 1, (k, m) if ta else (m, k))
+    B = rnd.uniform(-1, 1, (n, k) if tb else (k, n))
+    print(f"[batch] p={p} grid={grid} ta={ta} tb={tb} mnk={mnk}", flush=True)
+    monkeypatch.setenv("CA3DMM_SIM_BATCH", "0")
+    want = run_sim(p, m, n, k, ta, tb, A, B, proc_grid=grid)
+    monkeypatch.delenv("CA3DMM_SIM_BATCH")
+    got = run_sim(p, m, n, k, ta, tb, A, B, proc_grid=grid)
+    graph = run_sim(p, m, n, k, ta, tb, A, B, proc_grid=grid, graph=True, repeat=3)
+    assert np.array_equal(got, want) and np.array_equal(graph, want)
+    opA, opB = (A.T if ta else A), (B.T if tb else B)
+    bound = k * 2.0 ** -53 * (np.abs(opA) @ np.abs(opB))
+    assert np.all(np.abs(got - opA @ opB) <= bound + 2.0 ** -53 * np.abs(opA @ opB))
+
+
 def test_sim_idle_ranks(hip):
     """p = 7 with grid 2x1x3: rank 6 idles but still takes part in the
     redistribution of its input columns."""
