@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--no-grid", action="store_true", help="skip the 8000^3 simulated-grid line")
     ap.add_argument("--suite", default=None,
                     choices=["rank-share", "ozaki-sweep", "ozaki-forms", "grouped", "cholqr", "pmc-calib", "dgemm-group",
-                             "dgemm-loads", "yardstick"],
+                             "dgemm-loads", "grid-batch", "yardstick"],
                     help="run one of the secondary measurements instead (JSON lines)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

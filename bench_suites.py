@@ -528,6 +528,31 @@ def dgemm_group(args):
     hip.set_sync(True)
 
 
+def grid_batch(args):
+    """The simulated 4x2x3 grid (example_AB's 1D column layouts, NN) on
+    8000^3 and 10007^3 in its per-rank mode, whose Cannon rounds run as one
+    dgemm_batch_kernel launch: one untimed and --steps (at most 3) timed
+    execs each (the counter form: --steps 1)."""
+    rnd = np.random.default_rng(5)
+    hip.set_sync(False)
+    for n in (8000, 10007):
+        p = 24
+        lay = col1d(n, n, p)
+        dA = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+        dB = hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (n, n)))
+        dC = hip.DeviceArray.zeros((n, n))
+        Ap = [dA.at(r[2] * n) for r in lay]
+        Bp = [dB.at(r[2] * n) for r in lay]
+        Cp = [dC.at(r[2] * n) for r in lay]
+        sim = E.SimGrid(p, n, n, n, 0, 0, lay, lay, lay, proc_grid=[4, 2, 3])
+        ms = timed(lambda: sim.exec(Ap, [n] * p, Bp, [n] * p, Cp, [n] * p), reps=max(1, min(args.steps, 3)))
+        emit({"suite": "grid-batch", "n": n, "grid": "4x2x3", "ms": round(ms, 3),
+              "tflops": round(2.0 * n ** 3 / ms / 1e9, 2)})
+        sim.free()
+        del dA, dB, dC
+    hip.set_sync(True)
+
+
 class _At:
     """A device address inside a DeviceArray, as an operand of hip.dgemm / hip.gram."""
     def __init__(self, ptr):
@@ -650,4 +675,4 @@ def yardstick(args):
 
 def run(name, args):
     {"yardstick": yardstick, "rank-share": rank_share, "ozaki-sweep": ozaki_sweep, "ozaki-forms": ozaki_forms, "grouped": grouped, "cholqr": cholqr,
-     "pmc-calib": pmc_calib, "dgemm-group": dgemm_group, "dgemm-loads": dgemm_loads}[name](args)
+     "pmc-calib": pmc_calib, "dgemm-group": dgemm_group, "dgemm-loads": dgemm_loads, "grid-batch": grid_batch}[name](args)
