@@ -219,7 +219,7 @@ def rank_share(args):
     sim.set_mode(grouped=True, graph=False)
     g, wg, tile = sim.autotune(3)
     ms = timed(run, reps=5)
-    tname = {0: "128x128", 1: "256x128", 2: "128x256"}[tile]
+    tname = {0: "128x128", 1: "256x128", 2: "128x256", 3: "batched 128x128 direct-to-LDS"}[tile]
     emit({"suite": "rank-share", "what": f"8000^3 on 4x2x3, grouped (tile {tname}, rows={g}, {wg} WG/CU), whole exec",
           "ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)})
     sim.set_mode(grouped=True, graph=True)
@@ -348,7 +348,7 @@ def grouped(args):
           "tflops": round(flops / ms / 1e9, 2), "max_abs_diff_vs_per_launch": d})
     g, wg, tile = sim.autotune(2)
     ms = timed(run, reps=3)
-    tname = {0: "128x128", 1: "256x128", 2: "128x256"}[tile]
+    tname = {0: "128x128", 1: "256x128", 2: "128x256", 3: "batched 128x128 direct-to-LDS"}[tile]
     emit({"suite": "grouped", "mode": f"grouped, tuned (tile {tname}, rows={g}, {wg} WG/CU)", "ms": round(ms, 3),
           "tflops": round(flops / ms / 1e9, 2)})
     sim.set_mode(grouped=True, graph=True)

@@ -130,9 +130,12 @@ struct BatchProb {
 // (gridDim.x = total tiles rounded up to a multiple of 8).  Linear tile
 // indices are laid out XCD-contiguously over the whole batch, so each XCD
 // works through its own problems' panels; inside a problem tiles are
-// walked in groups of `group` tile-rows.  A tile's segments run in order,
-// each as its own product added by the epilogue (beta 0 on the first, 1
-// after): the bits of one dgemm_dtl_kernel<0, 0> launch per segment.
+// walked in groups of `group` tile-rows.  A tile's segments run in order.
+// PER_SEG: each segment is its own product added by the epilogue (beta 0
+// on the first, 1 after), the bits of one dgemm_dtl_kernel<0, 0> launch
+// per segment.  Otherwise all segments accumulate in registers and the tile
+// is stored once (C = the sum), the bits of the persistent grouped kernel.
+template <bool PER_SEG>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_batch_kernel(const BatchProb *__restrict__ probs, int nprob,
                                                                     int64_t total, int group)
 {
@@ -157,15 +160,19 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_batch_kernel(const BatchPro
 
     GemmArgs g;
     g.M = P.M; g.N = P.N;
+    g.K = 0;                                  // not read by the epilogue
     g.alpha = 1.0;
+    g.beta = 0.0;
     g.C = P.C; g.ldc = P.ldc;
     g.nsrc = 0;
+    f64x4 acc[4][4];
     for (int s = 0; s < P.nseg; ++s) {
-        f64x4 acc[4][4];
+        if (PER_SEG || s == 0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+                for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        }
         const int K = P.k[s];
         int kt0 = 0;
         int sm0 = m0, sn0 = n0, smn = mn, snn = nn;
@@ -186,18 +193,28 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_batch_kernel(const BatchPro
             lb.init(P.B[s], P.ldb[s], sn0, P.N);
             mainloop_checked(la, lb, kt0, K, LdsPair{A0, A1}, LdsPair{B0, B1}, acc);
         }
-        // the epilogue's addresses are the same on every segment: keep the
-        // compiler from computing them once and holding them (in vector
-        // registers) across the main loops
-        double *C = P.C;
-        int64_t ldc = P.ldc;
-        int em0 = m0, en0 = n0;
-        asm volatile("" : "+s"(C), "+s"(ldc), "+s"(em0), "+s"(en0));
-        g.C = C;
-        g.ldc = ldc;
-        g.K = K;
-        g.beta = s ? 1.0 : 0.0;
-        epilogue(g, em0, en0, acc);
+        if (PER_SEG) {
+            // the epilogue's addresses are the same on every segment: keep
+            // the compiler from computing them once and holding them (in
+            // vector registers) across the main loops
+            double *C = P.C;
+            int64_t ldc = P.ldc;
+            int em0 = m0, en0 = n0;
+            asm volatile("" : "+s"(C), "+s"(ldc), "+s"(em0), "+s"(en0));
+            g.C = C;
+            g.ldc = ldc;
+            g.beta = s ? 1.0 : 0.0;
+            epilogue(g, em0, en0, acc);
+        }
+    }
+    if (!PER_SEG) {
+        if (P.nseg == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        }
+        epilogue(g, m0, n0, acc);             // alpha = 1, beta = 0: C = the sum as accumulated
     }
 }
 
@@ -321,9 +338,10 @@ struct BatchPlan {
     int nprob = 0;
     int64_t total = 0;
     int group = 16;
+    bool per_segment = true;
 };
 
-int build_batch(const std::vector<Problem> &probs, BatchPlan **out)
+int build_batch(const std::vector<Problem> &probs, bool per_segment, BatchPlan **out)
 {
     *out = nullptr;
     std::vector<BatchProb> h;
@@ -353,6 +371,7 @@ int build_batch(const std::vector<Problem> &probs, BatchPlan **out)
     }
     if ((base + 7) / 8 * 8 > INT_MAX) return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "dgemm batch: %lld tiles", (long long) base);
     auto *p = new BatchPlan;
+    p->per_segment = per_segment;
     p->nprob = (int) h.size();
     p->total = base;
     const char *v = getenv("CA3DMM_DGEMM_GROUP");
@@ -370,12 +389,16 @@ int build_batch(const std::vector<Problem> &probs, BatchPlan **out)
     return HIP_PROXY_OK;
 }
 
-int run_batch(BatchPlan *p)
+int run_batch(BatchPlan *p, int group)
 {
     if (!p) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "dgemm batch: no plan");
     if (p->total == 0) return HIP_PROXY_OK;
     const int grid = (int) ((p->total + 7) / 8 * 8);
-    dgemm_batch_kernel<<<grid, gemm::kThreads, 0, ca3::stream()>>>(p->d, p->nprob, p->total, p->group);
+    const int G = (group >= 1 && group <= 256) ? group : p->group;
+    if (p->per_segment)
+        dgemm_batch_kernel<true><<<grid, gemm::kThreads, 0, ca3::stream()>>>(p->d, p->nprob, p->total, G);
+    else
+        dgemm_batch_kernel<false><<<grid, gemm::kThreads, 0, ca3::stream()>>>(p->d, p->nprob, p->total, G);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return ca3::fail(e, "dgemm_batch_kernel");
     return ca3::finish();

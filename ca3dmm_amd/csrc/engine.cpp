@@ -1299,7 +1299,7 @@ int build_batch(ca3dmm_sim *S)
         probs.push_back(pr);
     }
     ++S->plan_gen;
-    return G::build_batch(probs, &S->batch);
+    return G::build_batch(probs, true, &S->batch);
 }
 
 // The automatic modulus count, on the operands in place: sample s of
@@ -1787,8 +1787,8 @@ int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per
 
 int ca3dmm_sim_set_tuning(ca3dmm_sim_p sim, int group_rows, int wg_per_cu, int tile)
 {
-    if (!sim || group_rows < 1 || wg_per_cu < 1 || wg_per_cu > 2 || tile < 0 || tile > 2 ||
-        (tile != 0 && wg_per_cu != 1))
+    if (!sim || group_rows < 1 || wg_per_cu < 1 || wg_per_cu > 2 || tile < 0 || tile > 3 ||
+        ((tile == 1 || tile == 2) && wg_per_cu != 1))
         return HIP_PROXY_ERR_ARG;
     int rc = 0;
     if (!sim->plan && (rc = build_grouped(sim))) return rc;

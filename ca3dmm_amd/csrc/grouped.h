@@ -39,7 +39,9 @@ struct Fold {                 // k-layer reduction of the C blocks of kp problem
 };
 
 struct Tuning {
-    int tile = 0;             // workgroup tile: 0 = 128 x 128, 1 = 256 x 128, 2 = 128 x 256
+    int tile = 0;             // workgroup tile: 0 = 128 x 128, 1 = 256 x 128, 2 = 128 x 256 (persistent,
+                              // register-staged); 3 = one batched direct-to-LDS launch of 128 x 128
+                              // tiles (rounds in registers), then the folds as separate launches
     int group_rows = 8;       // tile rows walked together in the queue order
     int wg_per_cu = 2;        // persistent workgroups per CU (1 for the 8-wave tiles)
 };
@@ -96,14 +98,17 @@ void store_tuning_ozaki(const std::string &sig, const OzTuning &t);
 
 // The same problems (every operand NN, as stored) as one non-persistent
 // launch of the direct-to-LDS DGEMM (dgemm.hip): one workgroup per
-// 128 x 128 tile of every problem, its k-segments in order, each added to
-// C as its own product (beta 0 on the first, 1 after), so C gets the bits
-// of one hip_dgemm call per segment.  Segments may have k = 0; folds are
-// not part of it.  The load form (CA3DMM_DGEMM_LOADS) and tile-row group
-// (CA3DMM_DGEMM_GROUP) are read when the plan is built.
+// 128 x 128 tile of every problem, its k-segments in order.  per_segment:
+// each segment is added to C as its own product (beta 0 on the first, 1
+// after), so C gets the bits of one hip_dgemm call per segment; otherwise
+// the segments accumulate in registers and C is stored once, the bits of
+// the persistent grouped kernel.  Segments may have k = 0; folds are not
+// part of it.  The load form (CA3DMM_DGEMM_LOADS) and the tile-row group
+// (CA3DMM_DGEMM_GROUP; group > 0 overrides it per run) are read when the
+// plan is built.
 struct BatchPlan;
-int build_batch(const std::vector<Problem> &probs, BatchPlan **out);
-int run_batch(BatchPlan *p);
+int build_batch(const std::vector<Problem> &probs, bool per_segment, BatchPlan **out);
+int run_batch(BatchPlan *p, int group = 0);
 void destroy_batch(BatchPlan *p);
 
 // A whole enqueued sequence (every kernel `body` launches on the library

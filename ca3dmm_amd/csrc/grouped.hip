@@ -20,12 +20,18 @@
 //    hipGraph and replayed.
 //  * The workgroup tile is 128 x 128 (4 waves, one or two workgroups per
 //    CU), 256 x 128 or 128 x 256 (8 waves, one per CU).
+//  * One more candidate is not persistent: every rank's 128 x 128 tiles
+//    in one batched launch of the direct-to-LDS DGEMM (dgemm.hip), the
+//    rounds accumulated in registers as here, then each fold group's
+//    layers added in layer order by the fold kernel, column slice by column
+//    slice; the same bits as the persistent kernel.
 //  * The tuner times the tile shape, the queue order (tile rows walked
 //    together) and the persistent grid size, and keeps the fastest per
 //    problem signature, in memory and in a tuning table on disk
 //    (CA3DMM_TUNE_CACHE, default ~/.cache/ca3dmm_amd/tuning.txt) that later
 //    processes read back.
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -35,6 +41,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include "ca3dmm_ext.h"
 #include "dgemm_tile.h"
 #include "grouped.h"
 #include "hip_common.h"
@@ -45,6 +52,7 @@ namespace grouped {
 using namespace ca3::gemm;
 
 constexpr int kTileShapes = 3;     // 128 x 128, 256 x 128, 128 x 256
+constexpr int kTileBatched = 3;    // Tuning::tile of the batched direct-to-LDS launch
 
 struct Plan {
     int nprob = 0, nfold = 0;
@@ -55,6 +63,8 @@ struct Plan {
     int *d_tile0[kTileShapes] = {nullptr, nullptr, nullptr};     // [nprob + 1] first item of each problem
     int *d_ftile0[kTileShapes] = {nullptr, nullptr, nullptr};    // [nfold] first counter of each fold group
     int *d_counters = nullptr;    // [max ncounters + 1]; slot ncounters[tile] is the queue head
+    BatchPlan *batch = nullptr;   // the batched candidate's tables
+    std::vector<Fold> h_folds;    // its fold launches
     Tuning tune;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -154,8 +164,31 @@ __global__ __launch_bounds__(TL::THREADS, TL::MIN_BLOCKS) void grouped_kernel(
 int tile_bm(int tile) { return tile == 1 ? 256 : 128; }
 int tile_bn(int tile) { return tile == 2 ? 256 : 128; }
 
+// The batched candidate: the products, then every fold group's column
+// slices through the fold kernel (layers in layer order).
+int launch_batched(Plan *p, const Tuning &t)
+{
+    int rc = run_batch(p->batch, t.group_rows);
+    for (const Fold &F : p->h_folds) {
+        for (int sl = 0; sl < F.kp && !rc; ++sl) {
+            const int ncol = F.cs[sl + 1] - F.cs[sl];
+            if (F.M <= 0 || ncol <= 0) continue;
+            const double *parts[kMaxKp];
+            int lds[kMaxKp];
+            for (int l = 0; l < F.kp; ++l) {
+                parts[l] = F.Cl[l] + (int64_t) F.cs[sl] * F.ldl;
+                lds[l] = (int) F.ldl;
+            }
+            rc = hip_fold_sum(F.kp, parts, lds, F.M, ncol, F.Cout[sl], (int) F.ldo);
+        }
+        if (rc) break;
+    }
+    return rc;
+}
+
 int launch(Plan *p, const Tuning &t, hipStream_t st)
 {
+    if (t.tile == kTileBatched) return launch_batched(p, t);
     const int ti = (t.tile >= 0 && t.tile < kTileShapes) ? t.tile : 0;
     const int nc = p->ncounters[ti], total = p->total[ti];
     zero_ints<<<std::max(1, std::min(1024, (nc + 1 + 255) / 256)), 256, 0, st>>>(p->d_counters, nc + 1);
@@ -255,7 +288,7 @@ bool lookup_tuning(const char *form, const std::string &sig, Tuning *t)
     if (it == g_table.end()) return false;
     Tuning r;
     if (sscanf(it->second.c_str(), "tile=%d rows=%d wg=%d", &r.tile, &r.group_rows, &r.wg_per_cu) != 3) return false;
-    if (r.tile < 0 || r.tile >= kTileShapes || r.group_rows < 1 || r.wg_per_cu < 1 || r.wg_per_cu > 2) return false;
+    if (r.tile < 0 || r.tile > kTileBatched || r.group_rows < 1 || r.wg_per_cu < 1 || r.wg_per_cu > 2) return false;
     *t = r;
     return true;
 }
@@ -355,6 +388,17 @@ int build(const std::vector<Problem> &probs, const std::vector<Fold> &folds, Pla
         destroy(p);
         return ca3::fail(e, "grouped plan");
     }
+    int rc = build_batch(probs, false, &p->batch);
+    if (rc) {
+        destroy(p);
+        return rc;
+    }
+    for (const Fold &F : folds)
+        if (F.ldl > INT_MAX || F.ldo > INT_MAX) {
+            destroy(p);
+            return ca3::fail_msg(HIP_PROXY_ERR_UNSUPP, "grouped plan: fold leading dimension above INT_MAX");
+        }
+    p->h_folds = folds;
     Tuning t;
     if (lookup_tuning("fp64", p->signature, &t)) p->tune = t;
     *out = p;
@@ -370,7 +414,10 @@ int run(Plan *p, bool use_graph)
         drop_graph(p);
     if (!p->exec) {
         CA3_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const bool sync_saved = ca3::sync_mode();
+        hip_proxy_set_sync(0);                    // no stream synchronisation inside the capture
         int rc = launch(p, p->tune, st);
+        hip_proxy_set_sync(sync_saved ? 1 : 0);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
         if (rc) { if (g) (void) hipGraphDestroy(g); return rc; }
@@ -387,15 +434,18 @@ int autotune(Plan *p, int reps)
 {
     static const int rows[] = {1, 4, 8, 16};
     hipStream_t st = ca3::stream();
+    // the candidates are timed back to back: no synchronisation per launch
+    const bool sync_saved = ca3::sync_mode();
+    hip_proxy_set_sync(0);
     hipEvent_t e0, e1;
     CA3_CHECK(hipEventCreate(&e0));
     CA3_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     Tuning bt = p->tune;
     int rc = 0;
-    for (int ti = 0; ti < kTileShapes && !rc; ++ti) {
+    for (int ti = 0; ti <= kTileBatched && !rc; ++ti) {
         for (int gr : rows) {
-            for (int w = 1; w <= (ti == 0 ? 2 : 1); ++w) {
+            for (int w = (ti == kTileBatched ? 2 : 1); w <= (ti == 0 || ti == kTileBatched ? 2 : 1); ++w) {
                 Tuning t;
                 t.tile = ti;
                 t.group_rows = gr;
@@ -414,6 +464,7 @@ int autotune(Plan *p, int reps)
     }
     (void) hipEventDestroy(e0);
     (void) hipEventDestroy(e1);
+    hip_proxy_set_sync(sync_saved ? 1 : 0);
     if (rc) return rc;
     p->tune = bt;
     store_tuning("fp64", p->signature, bt);
@@ -435,6 +486,7 @@ void destroy(Plan *p)
         (void) hipFree(p->d_ftile0[ti]);
     }
     (void) hipFree(p->d_counters);
+    destroy_batch(p->batch);
     delete p;
 }
 

@@ -170,7 +170,8 @@ class SimGrid:
 
     def autotune(self, reps=3):
         """Tune the FP64 grouped launch; returns (group_rows, wg_per_cu, tile)
-        with tile 0 = 128x128, 1 = 256x128, 2 = 128x256."""
+        with tile 0 = 128x128, 1 = 256x128, 2 = 128x256 (persistent), 3 =
+        the batched direct-to-LDS launch plus fold launches."""
         g, w, t = C.c_int(), C.c_int(), C.c_int()
         check(lib.ca3dmm_sim_autotune(self.h, reps, C.byref(g), C.byref(w), C.byref(t)), "ca3dmm_sim_autotune")
         return g.value, w.value, t.value

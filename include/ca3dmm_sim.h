@@ -59,16 +59,21 @@ void *ca3dmm_sim_c_out(ca3dmm_sim_p sim, int rank);
  * adds the kp layer tiles in layer order).  use_graph != 0: capture the whole
  * exec -- redistributions, transposes, allgathers, products, fold -- into one
  * hipGraph on the first call (again when an argument or setting changes)
- * and replay it.  Default: one DGEMM launch per rank and round, eager. */
+ * and replay it.  Default: every rank's rounds in one batched DGEMM launch
+ * (the bits of one launch per rank and round; CA3DMM_SIM_BATCH=0 makes
+ * those launches), eager. */
 int ca3dmm_sim_set_mode(ca3dmm_sim_p sim, int grouped, int use_graph);
 
-/* Time the grouped launch over its tunings -- workgroup tile 128 x 128 (4
- * waves), 256 x 128 or 128 x 256 (8 waves); tile rows walked together in the
- * queue order: 1, 4, 8, 16; persistent workgroups per CU: 1, 2 (128 x 128
- * only) -- keep the fastest for this grid's shape signature, in memory and
- * in the tuning table on disk (CA3DMM_TUNE_CACHE, default
- * ~/.cache/ca3dmm_amd/tuning.txt; a later plan of the same signature starts
- * from it), and report it.  tile (may be NULL): 0, 1, 2 as above. */
+/* Time the grouped launch over its tunings -- tile 0, 1, 2: the persistent
+ * register-staged kernel with workgroup tile 128 x 128 (4 waves), 256 x 128
+ * or 128 x 256 (8 waves); tile 3: one batched direct-to-LDS launch of
+ * 128 x 128 tiles with the rounds accumulated in registers, then the fold
+ * launches (the same bits); tile rows walked together: 1, 4, 8, 16;
+ * persistent workgroups per CU: 1, 2 (tile 0 only) -- keep the fastest for
+ * this grid's shape signature, in memory and in the tuning table on disk
+ * (CA3DMM_TUNE_CACHE, default ~/.cache/ca3dmm_amd/tuning.txt; a later plan
+ * of the same signature starts from it), and report it.  tile (may be
+ * NULL): 0 to 3 as above. */
 int ca3dmm_sim_autotune(ca3dmm_sim_p sim, int reps, int *group_rows, int *wg_per_cu, int *tile);
 
 /* Fix the grouped launch's tuning by hand (e.g. the untuned baseline). */

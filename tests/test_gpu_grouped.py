@@ -200,11 +200,12 @@ def test_sequence_graph_recaptured_for_new_arguments(hip):
     sim.free()
 
 
-@pytest.mark.parametrize("tile,wg", [(0, 1), (0, 2), (1, 1), (2, 1)])
+@pytest.mark.parametrize("tile,wg", [(0, 1), (0, 2), (1, 1), (2, 1), (3, 2)])
 def test_grouped_every_tile_shape_exact(hip, tile, wg):
     """The grouped launch under each workgroup tile (128x128 with one or two
-    workgroups per CU, 256x128, 128x256): bit-exact on integers over a grid
-    with kp = 3 folds and uneven cuboids."""
+    workgroups per CU, 256x128, 128x256, the batched direct-to-LDS launch
+    with fold launches): bit-exact on integers over a grid with kp = 3
+    folds and uneven cuboids."""
     from ca3dmm_amd import device as dev
     from ca3dmm_amd import engine as E
     from oracle import oracle as O
@@ -240,3 +241,38 @@ def test_autotune_table_persists(hip, tmp_path, monkeypatch):
     lines = (tmp_path / "tuning.txt").read_text().splitlines()
     fp64 = [l for l in lines if l.startswith("fp64 ") and "640x" not in l[:5]]
     assert any(l.endswith(f"tile={tile} rows={g} wg={wg}") for l in fp64), lines
+
+
+@pytest.mark.parametrize("grid,p,mnk", [([4, 2, 3], 24, (1001, 777, 1999)), ([2, 3, 2], 12, (640, 900, 1333)),
+                                        ([2, 3, 1], 6, (700, 530, 900)), ([2, 1, 3], 7, (130, 77, 210))])
+def test_grouped_batched_candidate_same_bits_as_persistent(hip, grid, p, mnk):
+    """Random FP64 inputs: the batched direct-to-LDS candidate (tile 3,
+    rounds in registers, fold launches) stores the bits of the persistent
+    register-staged kernel (tile 0), eagerly and replayed from the exec's
+    hipGraph, with and without k-layer folds, idle ranks included."""
+    from ca3dmm_amd import device as dev
+    from ca3dmm_amd import engine as E
+    from oracle import oracle as O
+    m, n, k = mnk
+    rnd = np.random.default_rng(90 + p)
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=grid)
+    A, B = rnd.uniform(-1, 1, (m, k)), rnd.uniform(-1, 1, (k, n))
+    dA, dB = dev.DeviceArray.from_numpy(A), dev.DeviceArray.from_numpy(B)
+    Ap = [dA.at(r[2] * m) for r in w["src_A"]]
+    Bp = [dB.at(r[2] * k) for r in w["src_B"]]
+    out = {}
+    print(f"[grouped] batched candidate p={p} grid={grid} mnk={mnk}", flush=True)
+    for tile, wg, graph in ((0, 2, False), (3, 2, False), (3, 2, True)):
+        sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=grid, allocate=True)
+        dC = dev.DeviceArray.filled((m, n), 0xFF)
+        Cp = [dC.at(r[2] * m) for r in w["dst_C"]]
+        sim.set_mode(grouped=True, graph=graph)
+        sim.set_tuning(8, wg, tile)
+        for _ in range(2 if graph else 1):
+            sim.exec(Ap, [m] * p, Bp, [k] * p, Cp, [m] * p)
+        out[(tile, graph)] = dC.numpy()
+        sim.free()
+    ref = out[(0, False)]
+    assert np.array_equal(out[(3, False)], ref) and np.array_equal(out[(3, True)], ref)
+    bound = k * 2.0 ** -53 * (np.abs(A) @ np.abs(B))
+    assert np.all(np.abs(ref - A @ B) <= bound + 2.0 ** -53 * np.abs(A @ B))
