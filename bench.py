@@ -70,10 +70,17 @@ def grid_8000(hip, dA, dB, dC):
     out = {"workload": "8000^3 NN on the simulated 4x2x3 grid (24 ranks), whole ca3dmm_sim exec"}
     hip.set_sync(False)
     try:
-        for mode, grouped, graph in (("per_rank", False, False), ("grouped_graph", True, True)):
+        # per_rank: all ranks' rounds as one batched launch, per-round
+        # epilogues (the bits of one DGEMM per rank and round);
+        # grouped_graph: the grouped mode's batched candidate (rounds in
+        # registers, then the folds), the whole exec replayed from a hipGraph;
+        # grouped_persistent_graph: the same with the persistent
+        # register-staged kernel
+        for mode, grouped, graph, tile in (("per_rank", False, False, None), ("grouped_graph", True, True, 3),
+                                           ("grouped_persistent_graph", True, True, 0)):
             sim.set_mode(grouped=grouped, graph=graph)
             if grouped:
-                sim.set_tuning(16, 2, 0)
+                sim.set_tuning(16, 2, tile)
             run()
             hip.sync()
             t = hip.Timer()
@@ -89,10 +96,9 @@ def grid_8000(hip, dA, dB, dC):
 
 
 def dgemm_kernel_name():
-    """The kernel hip_dgemm launches for aligned operands with even leading
-    dimensions (every bench operand): the 128 x 128 tile with direct-to-LDS
-    stages unless CA3DMM_DGEMM_TILE or CA3DMM_DGEMM_LOADS=reg select the
-    register-staged kernel."""
+    """The kernel hip_dgemm launches for the bench operands: the 128 x 128
+    tile with direct-to-LDS stages unless CA3DMM_DGEMM_TILE or
+    CA3DMM_DGEMM_LOADS=reg select the register-staged kernel."""
     if os.environ.get("CA3DMM_DGEMM_TILE") in ("256x128", "128x256"):
         return "dgemm_kernel"
     return "dgemm_kernel" if os.environ.get("CA3DMM_DGEMM_LOADS") == "reg" else "dgemm_dtl_kernel"
