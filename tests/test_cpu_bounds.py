@@ -413,7 +413,9 @@ def dtl_loads(xc, xn, ld):
                     base = xb * ld
                     off = (x - xb) * ld * 8 + 16 * ((lane & 7) ^ ((xl >> 1) & 7))
                     assert 0 <= off < 2 ** 32
-                assert off % 16 == 0
+                # 16-byte aligned for an even ld; 8-byte aligned (a double
+                # pair, possibly split over two cache lines) for an odd one
+                assert off % (16 if ld % 2 == 0 else 8) == 0
                 for h in range(2):
                     e = base + off // 8 + h
                     k, x = (b, e - b * ld) if xc else (e % ld, e // ld)
@@ -423,15 +425,17 @@ def dtl_loads(xc, xn, ld):
 
 @pytest.mark.parametrize("xc", [0, 1])
 @pytest.mark.parametrize("xn", [128, 127, 120, 104, 64, 9, 8, 2, 1])
-def test_dtl_edge_tile_loads_stay_inside_and_fill_existing_slots(xc, xn):
+@pytest.mark.parametrize("odd_ld", [False, True])
+def test_dtl_edge_tile_loads_stay_inside_and_fill_existing_slots(xc, xn, odd_ld):
     """Edge tiles through the direct-to-LDS stages: every load reads an
     element of the 16 x xn block that exists (x < xn, k < 16), and every LDS
     slot of an existing (k, x) holds exactly that element, as for an
     interior tile; the remaining slots hold copies of existing elements.
-    XC = 1 needs xn even (odd counts take the checked path)."""
+    XC = 1 needs xn even (odd counts take the checked path).  Odd leading
+    dimensions (16-byte loads from 8-byte aligned addresses) map the same."""
     if xc and xn % 2:
         pytest.skip("XC = 1 edge tiles need an even count")
-    ld = 1000 if xc else 2048                          # ld >= 128 (XC = 1) or >= 16 (XC = 0)
+    ld = (1000 if xc else 2048) + (1 if odd_ld else 0)  # ld >= 128 (XC = 1) or >= 16 (XC = 0)
     slot = {dtl_index(xc, kk, xx): (kk, xx) for kk in range(16) for xx in range(128)}
     for lds, k, x, e in dtl_loads(xc, xn, ld):
         assert 0 <= k < 16 and 0 <= x < xn, (xc, xn, lds, k, x)
