@@ -42,16 +42,16 @@ def measured_traffic(kernel, n):
     except (OSError, ValueError, KeyError):
         return None
     return None
-def grid_8000(hip, dA, dB, dC):
-    """The rank-share configuration measured whole: 8000^3 NN on the
-    simulated 4x2x3 grid (24 ranks, 1D column layouts in and out), one
-    ca3dmm_sim exec = redistributions, allgather, 48 Cannon rounds, 24
-    kp = 3 folds and the C redistribution; per-rank launches, then the
-    persistent grouped launch (tile 128 x 128, rows 16, 2 workgroups per
-    CU) replayed from one hipGraph.  The first 8000 x 8000 of the bench
-    operands are the inputs (leading dimension 8192)."""
+def grid_line(hip, n, dA, dB, dC):
+    """A CA3DMM grid configuration measured whole: n^3 NN on the simulated
+    4x2x3 grid (24 ranks, 1D column layouts in and out), one ca3dmm_sim
+    exec = redistributions, allgather, 48 Cannon rounds, 24 kp = 3 folds
+    and the C redistribution; the per-rank mode (all ranks' rounds as one
+    batched launch), then the grouped mode's batched candidate and its
+    persistent register-staged kernel, each replayed from one hipGraph.
+    The first n x n of the operands are the inputs."""
     from ca3dmm_amd import engine as E
-    n, ld, p = 8000, dA.shape[0], 24
+    ld, p = dA.shape[0], 24
 
     # example_AB's 1D column layout: rank r owns columns [scol, scol + ncol)
     # of all n rows, the first n % p ranks one column more
@@ -67,7 +67,7 @@ def grid_8000(hip, dA, dB, dC):
     Cp = [dC.at(r[2] * ld) for r in rects]
     run = lambda: sim.exec(Ap, [ld] * p, Bp, [ld] * p, Cp, [ld] * p)
     fl = 2.0 * n ** 3
-    out = {"workload": "8000^3 NN on the simulated 4x2x3 grid (24 ranks), whole ca3dmm_sim exec"}
+    out = {"workload": f"{n}^3 NN on the simulated 4x2x3 grid (24 ranks), whole ca3dmm_sim exec"}
     hip.set_sync(False)
     try:
         # per_rank: all ranks' rounds as one batched launch, per-round
@@ -342,10 +342,17 @@ def main():
     hip.set_sync(True)
     log("bench: kernel timing done")
 
-    grid = None
+    grid = grid_10007 = None
     if not args.no_grid:
         log("bench: the 8000^3 grid on 4x2x3 (24 simulated ranks)")
-        grid = grid_8000(hip, dA, dB, dC)
+        grid = grid_line(hip, 8000, dA, dB, dC)
+        # the 10007^3 grid (uneven 2502/2501 x 5004/5003 cuboids): operands
+        # of its own size, uniform[-1,1]
+        log("bench: the 10007^3 grid on 4x2x3 (24 simulated ranks)")
+        big = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (10007, 10007))) for _ in range(2)]
+        big.append(hip.DeviceArray.zeros((10007, 10007)))
+        grid_10007 = grid_line(hip, 10007, *big)
+        del big
 
     # Ozaki at equal max error: phi = 0.5 inputs (uniform[-0.5,0.5]*exp(0.5 N(0,1))),
     # native error measured on sampled entries against double-double, then
@@ -406,6 +413,7 @@ def main():
         "accuracy": err_stats,
         "ozaki": ozaki,
         "grid": grid,
+        "grid_10007": grid_10007,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
