@@ -324,24 +324,41 @@ int hip_copy_matrix_block(size_t dt_size, const int nrow, const int ncol, const 
 int hip_fold_sum(int nsrc, const double *const *P, const int *ldp, int nrow, int ncol,
                  double *dst, int ldd)
 {
-    if (nsrc < 1 || nsrc > kMaxFold || !P || !ldp || !dst || nrow < 0 || ncol < 0 || ldd < nrow)
+    if (nsrc < 1 || !P || !ldp || !dst || nrow < 0 || ncol < 0 || ldd < nrow)
         return ca3::fail_msg(HIP_PROXY_ERR_ARG, "fold_sum: bad arguments (nsrc=%d)", nsrc);
-    if (nrow == 0 || ncol == 0) return HIP_PROXY_OK;
-    FoldArgs f;
-    memset(&f, 0, sizeof(f));
-    f.nsrc = nsrc;
-    bool vec = (nrow % 2 == 0) && ((uintptr_t) dst % 16 == 0) && (ldd % 2 == 0);
-    for (int t = 0; t < nsrc; ++t) {
+    for (int t = 0; t < nsrc; ++t)
         if (!P[t] || ldp[t] < nrow) return ca3::fail_msg(HIP_PROXY_ERR_ARG, "fold_sum: source %d", t);
-        f.P[t] = P[t];
-        f.ldp[t] = ldp[t];
-        vec = vec && ((uintptr_t) P[t] % 16 == 0) && (ldp[t] % 2 == 0);
+    if (nrow == 0 || ncol == 0) return HIP_PROXY_OK;
+    // More sources than one launch takes: the first kMaxFold into dst, then
+    // dst and the next kMaxFold - 1, and so on.  Each launch adds left to
+    // right onto the running value, so the order (and the bits) are those
+    // of one left-to-right fold.
+    for (int t0 = 0; t0 < nsrc;) {
+        FoldArgs f;
+        memset(&f, 0, sizeof(f));
+        int n = 0;
+        if (t0 > 0) {
+            f.P[0] = dst;
+            f.ldp[0] = ldd;
+            n = 1;
+        }
+        const int take = std::min(nsrc - t0, kMaxFold - n);
+        for (int t = 0; t < take; ++t, ++n) {
+            f.P[n] = P[t0 + t];
+            f.ldp[n] = ldp[t0 + t];
+        }
+        f.nsrc = n;
+        t0 += take;
+        bool vec = (nrow % 2 == 0) && ((uintptr_t) dst % 16 == 0) && (ldd % 2 == 0);
+        for (int t = 0; t < n; ++t) vec = vec && ((uintptr_t) f.P[t] % 16 == 0) && (f.ldp[t] % 2 == 0);
+        const int rowsv = nrow / (vec ? 2 : 1);
+        const dim3 grid((rowsv + 256 * kFoldU - 1) / (256 * kFoldU), std::min(ncol, 65535));
+        if (vec) fold_sum_kernel<2><<<grid, 256, 0, ca3::stream()>>>(f, nrow, ncol, dst, ldd);
+        else fold_sum_kernel<1><<<grid, 256, 0, ca3::stream()>>>(f, nrow, ncol, dst, ldd);
+        const int rc = ca3::finish();
+        if (rc) return rc;
     }
-    const int rowsv = nrow / (vec ? 2 : 1);
-    const dim3 grid((rowsv + 256 * kFoldU - 1) / (256 * kFoldU), std::min(ncol, 65535));
-    if (vec) fold_sum_kernel<2><<<grid, 256, 0, ca3::stream()>>>(f, nrow, ncol, dst, ldd);
-    else fold_sum_kernel<1><<<grid, 256, 0, ca3::stream()>>>(f, nrow, ncol, dst, ldd);
-    return ca3::finish();
+    return HIP_PROXY_OK;
 }
 
 int hip_op_sum(int dtype, int len, const void *invec, void *inoutvec)

@@ -1158,6 +1158,7 @@ struct ca3dmm_sim {
     ca3::grouped::Plan *plan = nullptr;
     ca3::grouped::OzPlan *ozplan = nullptr;
     ca3::grouped::BatchPlan *batch = nullptr;  // the per-rank rounds as one launch
+    bool batch_unsupported = false;            // more rounds per rank than a batch item holds
     uint64_t plan_gen = 0;        // plans built so far: a captured sequence holds their buffers
     // captured exec sequences (use_graph) and the arguments and settings each
     // was captured for: [0] the whole exec, [1] and [2] its operand and
@@ -1604,7 +1605,14 @@ int sim_exec(ca3dmm_sim *S, const void *const *src_A, const int *ldA, const void
     // the plans allocate and synchronise: build them outside any capture
     if (!S->ozaki_auto && S->ozaki && !S->ozplan && (rc = build_ozaki(S))) return rc;
     if (!S->ozaki && !S->ozaki_auto && S->grouped && !S->plan && (rc = build_grouped(S))) return rc;
-    if (!S->ozaki && !S->ozaki_auto && !S->grouped && !S->batch && batch_on() && (rc = build_batch(S))) return rc;
+    if (!S->ozaki && !S->ozaki_auto && !S->grouped && !S->batch && !S->batch_unsupported && batch_on()) {
+        rc = build_batch(S);
+        if (rc == HIP_PROXY_ERR_UNSUPP) {      // more Cannon rounds than a batch item holds: one launch per round
+            S->batch_unsupported = true;
+            rc = 0;
+        }
+        if (rc) return rc;
+    }
     const bool save = ca3::grouped::sync_mode();
     hip_proxy_set_sync(0);
     const std::vector<uintptr_t> key = sequence_key(S, src_A, ldA, src_B, ldB, dst_C, ldC);

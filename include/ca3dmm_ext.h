@@ -36,7 +36,9 @@ int hip_op_sum(int dtype, int len, const void *invec, void *inoutvec);
 /* k-layer partial-C reduction (ca3dmm.c:1311-1324 with mpi_op_omp_sum) in
  * one pass and a fixed order: dst = ((P[0] + P[1]) + P[2]) + ...
  * Column-major nrow x ncol blocks; P[t] leading dim ldp[t].  dst may alias
- * P[0].  Bit-exact against the same fold on the CPU. */
+ * P[0] (no other source).  Any nsrc >= 1: above 16 sources the fold runs
+ * as several launches onto the running value, in the same order.
+ * Bit-exact against the same fold on the CPU. */
 int hip_fold_sum(int nsrc, const double *const *P, const int *ldp,
                  int nrow, int ncol, double *dst, int ldd);
 

@@ -175,3 +175,63 @@ def test_sim_btb(hip, p, pg):
     B = _ints(rnd, (k, n))
     got = run_sim(p, n, n, k, 0, 0, None, B, proc_grid=pg, is_btb=True)
     assert np.array_equal(got, B.T @ B)
+
+
+def test_sim_random_grids_batched_paths_same_bits(hip, monkeypatch):
+    """Seeded random rank counts, shapes and trans cases on the grid the
+    library picks (idle ranks, k blocks shorter than a stage, uneven
+    cuboids): the batched per-rank rounds give the bits of one launch per
+    rank and round, and the grouped mode's batched candidate (tile 3) those
+    of its persistent kernel (tile 0)."""
+    from oracle import oracle as O
+    rnd = np.random.default_rng(4242)
+    for case in range(12):
+        p = int(rnd.integers(1, 25))
+        m, n, k = (int(v) for v in rnd.integers(1, 600, 3))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        A = rnd.uniform(-1, 1, (k, m) if ta else (m, k))
+        B = rnd.uniform(-1, 1, (n, k) if tb else (k, n))
+        print(f"[batch] random case {case}: p={p} m={m} n={n} k={k} ta={ta} tb={tb}", flush=True)
+        monkeypatch.setenv("CA3DMM_SIM_BATCH", "0")
+        per_round = run_sim(p, m, n, k, ta, tb, A, B)
+        monkeypatch.delenv("CA3DMM_SIM_BATCH")
+        batched = run_sim(p, m, n, k, ta, tb, A, B)
+        assert np.array_equal(batched, per_round), case
+        w = O.ca3dmm_world(m, n, k, ta, tb, p)
+        if w["plans"][0].task_k_num > 8:
+            continue                          # the grouped plans hold at most 8 k layers
+        out = []
+        for tile in (0, 3):
+            dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
+            dC = hip.DeviceArray.filled((m, n), 0xFF)
+            sim = E.SimGrid(p, m, n, k, ta, tb, w["src_A"], w["src_B"], w["dst_C"], allocate=True)
+            sim.set_mode(grouped=True, graph=False)
+            sim.set_tuning(4, 2, tile)
+            sim.exec([dA.at(r[2] * A.shape[0]) for r in w["src_A"]], [A.shape[0]] * p,
+                     [dB.at(r[2] * B.shape[0]) for r in w["src_B"]], [B.shape[0]] * p,
+                     [dC.at(r[2] * m) for r in w["dst_C"]], [m] * p)
+            out.append(dC.numpy())
+            sim.free()
+        assert np.array_equal(out[0], out[1]), case
+        opA, opB = (A.T if ta else A), (B.T if tb else B)
+        bound = k * 2.0 ** -53 * (np.abs(opA) @ np.abs(opB))
+        assert np.all(np.abs(out[0] - opA @ opB) <= bound + 2.0 ** -53 * np.abs(opA @ opB)), case
+
+
+def test_sim_more_rounds_than_a_batch_item_holds(hip, monkeypatch):
+    """A 9 x 9 Cannon group (nine rounds per rank, more than one batched
+    item holds): the per-rank mode falls back to one launch per rank and
+    round, with the same bits as asked for explicitly."""
+    from oracle import oracle as O
+    rnd = np.random.default_rng(99)
+    m, n, k, p = 300, 270, 333, 81
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=[9, 9, 1])
+    assert max(pl.cannon.np_dim for pl in w["plans"] if pl.is_active) == 9
+    A, B = rnd.uniform(-1, 1, (m, k)), rnd.uniform(-1, 1, (k, n))
+    print("[batch] 81 ranks on 9 x 9 x 1", flush=True)
+    got = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=[9, 9, 1])
+    monkeypatch.setenv("CA3DMM_SIM_BATCH", "0")
+    want = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=[9, 9, 1])
+    assert np.array_equal(got, want)
+    bound = k * 2.0 ** -53 * (np.abs(A) @ np.abs(B))
+    assert np.all(np.abs(got - A @ B) <= bound + 2.0 ** -53 * np.abs(A @ B))

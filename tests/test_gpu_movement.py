@@ -78,7 +78,8 @@ def test_redistribution_pack_unpack_matches_oracle(hip):
     assert np.array_equal(ddst.numpy(), want_dst)
 
 
-@pytest.mark.parametrize("nsrc,nrow,ncol", [(1, 5, 3), (2, 64, 64), (3, 2000, 37), (3, 1001, 13), (8, 17, 129)])
+@pytest.mark.parametrize("nsrc,nrow,ncol", [(1, 5, 3), (2, 64, 64), (3, 2000, 37), (3, 1001, 13), (8, 17, 129),
+                                            (16, 40, 7), (17, 33, 9), (21, 64, 5), (40, 31, 11)])
 def test_fold_sum_bit_exact(hip, nsrc, nrow, ncol):
     rnd = np.random.default_rng(nsrc * 100 + nrow)
     parts = [np.asfortranarray(rnd.standard_normal((nrow, ncol)) * 10.0 ** rnd.integers(-8, 8))
