@@ -235,3 +235,34 @@ def test_sim_more_rounds_than_a_batch_item_holds(hip, monkeypatch):
     assert np.array_equal(got, want)
     bound = k * 2.0 ** -53 * (np.abs(A) @ np.abs(B))
     assert np.all(np.abs(got - A @ B) <= bound + 2.0 ** -53 * np.abs(A @ B))
+
+
+def test_single_rank_engine_random_layouts(hip, monkeypatch):
+    """The drop-in boundary on one rank with seeded random shapes, trans
+    cases, padded leading dimensions, output windows (dst_C a sub-block or
+    the whole C) and redistribution forced or not: exact on integers."""
+    rnd = np.random.default_rng(777)
+    for case in range(16):
+        m, n, k = (int(v) for v in rnd.integers(1, 700, 3))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        force = int(rnd.integers(0, 2))
+        if rnd.integers(0, 2):
+            sr, sc = int(rnd.integers(0, m)), int(rnd.integers(0, n))
+            dst = (sr, int(rnd.integers(1, m - sr + 1)), sc, int(rnd.integers(1, n - sc + 1)))
+        else:
+            dst = (0, m, 0, n)
+        ra, ca = (k, m) if ta else (m, k)
+        rb, cb = (n, k) if tb else (k, n)
+        lda, ldb, ldc = ra + int(rnd.integers(0, 5)), rb + int(rnd.integers(0, 5)), dst[1] + int(rnd.integers(0, 5))
+        A, B = _ints(rnd, (ra, ca)), _ints(rnd, (rb, cb))
+        bufA, bufB = np.zeros((lda, ca)), np.zeros((ldb, cb))
+        bufA[:ra], bufB[:rb] = A, B
+        print(f"[engine] random case {case}: m={m} n={n} k={k} ta={ta} tb={tb} dst={dst} force={force}", flush=True)
+        monkeypatch.setenv("CA3DMM_FORCE_REDIST", str(force))
+        eng = single_engine(m, n, k, ta, tb, dst=dst)
+        dA, dB = hip.DeviceArray.from_numpy(bufA), hip.DeviceArray.from_numpy(bufB)
+        dC = hip.DeviceArray.filled((ldc, dst[3]), 0xFF)
+        E.lib.ca3dmm_engine_exec(eng, dA.ptr, lda, dB.ptr, ldb, dC.ptr, ldc)
+        want = ((A.T if ta else A) @ (B.T if tb else B))[dst[0]:dst[0] + dst[1], dst[2]:dst[2] + dst[3]]
+        assert np.array_equal(dC.numpy()[:dst[1]], want), case
+        E.lib.ca3dmm_engine_free(C.byref(eng))

@@ -108,3 +108,40 @@ def test_ozaki2_accuracy_no_worse_than_native(hip):
     hip.ozaki2_dgemm(0, 0, n, n, n, 1.0, dA, dB, 0.0, dC, 16)
     got = errs()
     assert got[0] <= nat[0] and got[1] <= nat[1], (got, nat)
+
+
+class _At:
+    """A device address inside a DeviceArray, for the Ozaki entries' A / B / C."""
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+def test_both_forms_padded_layouts_and_offsets(hip):
+    """Seeded random shapes and trans cases with padded leading dimensions
+    and operands starting an odd number of doubles into their buffers (C
+    too): both Ozaki forms reproduce the specification bit for bit."""
+    rnd = np.random.default_rng(5150)
+    for case in range(8):
+        m, n = (int(v) for v in rnd.integers(1, 260, 2))
+        k = int(rnd.integers(1, 600))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        N, s = int(rnd.integers(4, 26)), int(rnd.integers(2, 9))
+        opA, opB = Z.phi_matrix(rnd, (m, k), 2.0), Z.phi_matrix(rnd, (k, n), 2.0)
+        A, B = (opA.T if ta else opA), (opB.T if tb else opB)
+        lda, ldb, ldc = A.shape[0] + int(rnd.integers(0, 4)), B.shape[0] + int(rnd.integers(0, 4)), m + 3
+        oa, ob, oc = (int(v) for v in rnd.integers(0, 4, 3))
+
+        def packed(X, ld, off):
+            buf = np.zeros(off + ld * X.shape[1])
+            for c in range(X.shape[1]):
+                buf[off + c * ld:off + c * ld + X.shape[0]] = X[:, c]
+            return hip.DeviceArray.from_numpy(buf)
+        dA, dB = packed(A, lda, oa), packed(B, ldb, ob)
+        print(f"[ozaki] padded case {case}: m={m} n={n} k={k} ta={ta} tb={tb} N={N} s={s}", flush=True)
+        for form, cnt, spec in (("crt", N, Z.ozaki2_dgemm), ("slice", s, Z.ozaki_dgemm8)):
+            dC = hip.DeviceArray.filled((oc + ldc * n,), 0xFF)
+            f = hip.ozaki2_dgemm if form == "crt" else hip.ozaki_dgemm
+            f(ta, tb, m, n, k, 1.0, _At(dA.at(oa)), _At(dB.at(ob)), 0.0, _At(dC.at(oc)), cnt,
+              lda=lda, ldb=ldb, ldc=ldc)
+            got = dC.numpy()[oc:].reshape(n, ldc).T[:m]
+            assert np.array_equal(got, spec(opA, opB, cnt)), (case, form)
