@@ -266,3 +266,57 @@ def test_single_rank_engine_random_layouts(hip, monkeypatch):
         want = ((A.T if ta else A) @ (B.T if tb else B))[dst[0]:dst[0] + dst[1], dst[2]:dst[2] + dst[3]]
         assert np.array_equal(dC.numpy()[:dst[1]], want), case
         E.lib.ca3dmm_engine_free(C.byref(eng))
+
+
+def _cut(rnd, length, parts):
+    """parts consecutive ranges covering [0, length) at random cut points
+    (empty ranges possible)."""
+    c = np.sort(rnd.integers(0, length + 1, parts - 1))
+    return [0] + [int(v) for v in c] + [length]
+
+
+def _random_rects(rnd, nrow, ncol, p):
+    """A random pr x pc block partition of an nrow x ncol matrix (pr pc = p),
+    blocks dealt to the ranks in random order: (srow, nrow, scol, ncol)."""
+    pr = int(rnd.choice([d for d in range(1, p + 1) if p % d == 0]))
+    rc, cc = _cut(rnd, nrow, pr), _cut(rnd, ncol, p // pr)
+    rects = [(rc[i], rc[i + 1] - rc[i], cc[j], cc[j + 1] - cc[j]) for i in range(pr) for j in range(p // pr)]
+    return [rects[o] for o in rnd.permutation(p)]
+
+
+def test_sim_random_block_layouts(hip):
+    """The grid's redistributions with seeded random 2-D block layouts of A,
+    B and C (each rank's block in its own buffer with a padded leading
+    dimension, empty blocks included), random trans cases and the grid the
+    library picks: exact on integers, per-rank and grouped modes."""
+    rnd = np.random.default_rng(31337)
+    for case in range(8):
+        p = int(rnd.choice([2, 3, 4, 6, 8, 12]))
+        m, n, k = (int(v) for v in rnd.integers(1, 400, 3))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        A = _ints(rnd, (k, m) if ta else (m, k))
+        B = _ints(rnd, (n, k) if tb else (k, n))
+        rA, rB, rC = _random_rects(rnd, *A.shape, p), _random_rects(rnd, *B.shape, p), _random_rects(rnd, m, n, p)
+        print(f"[layouts] case {case}: p={p} m={m} n={n} k={k} ta={ta} tb={tb}", flush=True)
+
+        def blocks(X, rects):
+            out = []
+            for (sr, nr, sc, nc) in rects:
+                ld = max(1, nr) + int(rnd.integers(0, 3))
+                buf = np.zeros((ld, max(1, nc)), order="F")
+                buf[:nr, :nc] = X[sr:sr + nr, sc:sc + nc]
+                out.append((hip.DeviceArray.from_numpy(buf), ld))
+            return out
+        bA, bB = blocks(A, rA), blocks(B, rB)
+        want = (A.T if ta else A) @ (B.T if tb else B)
+        for grouped in (False, True):
+            bC = [(hip.DeviceArray.filled((max(1, nr) + 1, max(1, nc)), 0xFF), max(1, nr) + 1) for (sr, nr, sc, nc) in rC]
+            sim = E.SimGrid(p, m, n, k, ta, tb, rA, rB, rC, allocate=True)
+            sim.set_mode(grouped=grouped, graph=False)
+            sim.exec([b.ptr for b, _ in bA], [ld for _, ld in bA], [b.ptr for b, _ in bB], [ld for _, ld in bB],
+                     [b.ptr for b, _ in bC], [ld for _, ld in bC])
+            got = np.full((m, n), np.nan)
+            for (sr, nr, sc, nc), (b, _) in zip(rC, bC):
+                got[sr:sr + nr, sc:sc + nc] = b.numpy()[:nr, :nc]
+            sim.free()
+            assert np.array_equal(got, want), (case, grouped)
