@@ -230,3 +230,32 @@ def test_cholesky_qr_in_place_equals_out_of_place(hip, monkeypatch):
         assert np.array_equal(dQ1.numpy(), dQ2.numpy()), variant
         assert np.array_equal(dR1.numpy(), dR2.numpy()), variant
 
+
+
+def test_cholesky_qr_random_shapes_and_padding(hip):
+    """Seeded random m x n (n not a multiple of the 64-wide Cholesky blocks
+    or of the 128-column solve blocks) with padded leading dimensions of A
+    and R, each variant: Q orthonormal and A = Q R to working precision, R
+    upper triangular, and the padding rows of both buffers untouched."""
+    rnd = np.random.default_rng(1618)
+    for case in range(9):
+        n = int(rnd.integers(1, 300))
+        m = int(rnd.integers(n, 20000))
+        variant = (hip.CHOLQR2, hip.SCHOLQR3, hip.CHOLQR_AUTO)[case % 3]
+        U, _ = np.linalg.qr(rnd.standard_normal((m, n)))
+        V, _ = np.linalg.qr(rnd.standard_normal((n, n)))
+        A = (U * np.logspace(0, -4, n)) @ V.T                 # kappa = 1e4
+        lda, ldr = m + int(rnd.integers(1, 4)), n + int(rnd.integers(1, 4))
+        bufA = np.full((lda, n), 7.0, order="F")
+        bufA[:m] = A
+        bufR = np.full((ldr, n), 7.0, order="F")
+        dA, dR = hip.DeviceArray.from_numpy(bufA), hip.DeviceArray.from_numpy(bufR)
+        print(f"[cholqr] random case {case}: m={m} n={n} variant={variant}", flush=True)
+        info, passes = hip.cholesky_qr(m, n, dA, dR, variant=variant, lda=lda, ldr=ldr)
+        assert info == 0, (case, info)
+        Qb, Rb = dA.numpy(), dR.numpy()
+        Q, R = Qb[:m], Rb[:n]
+        assert np.all(Qb[m:] == 7.0) and np.all(Rb[n:] == 7.0), case
+        assert np.array_equal(np.tril(R, -1), np.zeros((n, n))), case
+        assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13 * max(1.0, np.sqrt(n)), case
+        assert np.linalg.norm(A - Q @ R) / np.linalg.norm(A) < 1e-14 * max(1.0, np.sqrt(n)), case
