@@ -637,8 +637,10 @@ __global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_diag_kernel(O
 // stage z are done (vmcnt <= (NBUF - 2) * (WA + WB)), the barrier then makes
 // every wave's part visible.  Loads past the last stage re-read the last
 // stage (in bounds) into a buffer that is never consumed.
+// (One workgroup per CU: the 4-stage ring of a 256 x 128 tile takes 96 KB
+// of the 160 KB LDS.)
 template <bool SPLIT, int WTM, int WTN, int NBUF>
-__global__ __launch_bounds__(256, (WTN == 128 ? 1 : 2)) void ozaki_dtl_kernel(OzArgs g)
+__global__ __launch_bounds__(256, 1) void ozaki_dtl_kernel(OzArgs g)
 {
     constexpr int kBMo = 2 * WTM, kBNo = 2 * WTN;
     constexpr int FI = WTM / 16, FJ = WTN / 16;
