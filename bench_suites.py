@@ -197,6 +197,19 @@ def rank_share(args):
         by = 16 * elems
         emit({"suite": "rank-share", "what": name, "blocks": len(lst), "elements": elems, "ms": round(ms, 5),
               "bytes": by, "gbs": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / HBM_GBS, 4)})
+    # the copy ceiling at rank 0's sizes: one contiguous run of the same
+    # element count through the same copy kernel (launch and ramp included)
+    for name, lst in (("pack A, rank 0", pack0), ("one-pass redistribution of A and B, rank 0", direct0)):
+        elems = sum(d[4] * d[5] for d in lst)
+        src, dst = hip.DeviceArray((elems,)), hip.DeviceArray((elems,))
+        one = [(src.ptr, elems, dst.ptr, elems, 1, elems)]
+        ms_copy = timed(lambda: hip.copy_blocks(8, one), reps=20)
+        ms_blk = timed(lambda: hip.copy_blocks(8, lst), reps=20)
+        by = 16 * elems
+        emit({"suite": "rank-share", "what": f"{name} against a plain copy of its {elems} elements",
+              "ms_blocks": round(ms_blk, 5), "ms_plain_copy": round(ms_copy, 5),
+              "gbs_plain_copy": round(by / ms_copy / 1e6, 1), "blocks_over_plain": round(ms_copy / ms_blk, 3)})
+        del src, dst
     del bufs, direct, direct0
     # the whole 8000^3 grid in one persistent grouped launch (all 24 ranks'
     # Cannon rounds and the kp = 3 fold), autotuned, then the full exec
