@@ -95,6 +95,43 @@ def grid_line(hip, n, dA, dB, dC):
     return out
 
 
+def cholqr_line(hip, cond=1e8):
+    """CholeskyQR2 of a 2e6 x 512 matrix with condition number cond
+    (A = U diag(sigma) V^T, sigma log-spaced from 1 to 1/cond; U from
+    CholeskyQR2 of a uniform[-1,1] matrix, V from numpy's QR): one untimed
+    and three timed hip_cholesky_qr calls, with ||Q^T Q - I||_F and
+    ||A - QR||_F / ||A||_F."""
+    import numpy as np
+
+    from ca3dmm_amd import cholqr as CQ
+    m, n = 2000000, 512
+    rnd = np.random.default_rng(2022)
+    dG = hip.DeviceArray.from_numpy(np.asfortranarray(rnd.uniform(-1, 1, (m, n))))
+    dU, _ = CQ.cholesky_qr2(dG, m, n)
+    del dG
+    V, _ = np.linalg.qr(rnd.standard_normal((n, n)))
+    sigma = np.logspace(0, -np.log10(cond), n)
+    dSVt = hip.DeviceArray.from_numpy(np.asfortranarray(sigma[:, None] * V.T))
+    dA = hip.DeviceArray.zeros((m, n))
+    hip.dgemm(0, 0, m, n, n, 1.0, dU, dSVt, 0.0, dA)
+    del dU
+    dQ, dR, passes = CQ.cholesky_qr(dA, m, n, hip.CHOLQR2)
+    del dQ, dR
+    hip.sync()
+    t = hip.Timer()
+    t.start()
+    for _ in range(3):
+        dQ, dR, passes = CQ.cholesky_qr(dA, m, n, hip.CHOLQR2)
+        if _ < 2:
+            del dQ, dR
+    ms = t.stop() / 3
+    orth, res = CQ.qr_errors(dA, dQ, dR, m, n)
+    flops = 2 * (n * (n + 1.0) * m + n * n * float(m))          # two passes: Gram (upper) + solve
+    return {"workload": f"CholeskyQR2 of {m} x {n}, kappa {cond:g} (hip_cholesky_qr, FP64 MFMA Gram)",
+            "ms": round(ms, 2), "passes": passes, "tflops_gram_plus_solve": round(flops / ms / 1e9, 2),
+            "orth_QtQ_minus_I_F": orth, "residual_rel_F": res}
+
+
 def dgemm_kernel_name():
     """The kernel hip_dgemm launches for the bench operands: the 128 x 128
     tile with direct-to-LDS stages unless CA3DMM_DGEMM_TILE or
@@ -249,7 +286,8 @@ def main():
     ap.add_argument("--cpu-tests", type=int, default=5, help="timed executions of the CPU baseline")
     ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--cond", type=float, default=None, help="one condition number for the cholqr suite")
-    ap.add_argument("--no-grid", action="store_true", help="skip the 8000^3 simulated-grid line")
+    ap.add_argument("--no-grid", action="store_true", help="skip the simulated-grid lines")
+    ap.add_argument("--no-cholqr", action="store_true", help="skip the CholeskyQR2 line")
     ap.add_argument("--suite", default=None,
                     choices=["rank-share", "ozaki-sweep", "ozaki-forms", "grouped", "cholqr", "pmc-calib", "dgemm-group",
                              "dgemm-loads", "grid-batch", "yardstick"],
@@ -354,6 +392,11 @@ def main():
         grid_10007 = grid_line(hip, 10007, *big)
         del big
 
+    cholqr = None
+    if not args.no_cholqr:
+        log("bench: CholeskyQR2 of 2e6 x 512, kappa 1e8")
+        cholqr = cholqr_line(hip)
+
     # Ozaki at equal max error: phi = 0.5 inputs (uniform[-0.5,0.5]*exp(0.5 N(0,1))),
     # native error measured on sampled entries against double-double, then
     # for each Ozaki form the smallest slice / modulus count whose error is no worse
@@ -414,6 +457,7 @@ def main():
         "ozaki": ozaki,
         "grid": grid,
         "grid_10007": grid_10007,
+        "cholqr": cholqr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
