@@ -65,12 +65,13 @@ def single_engine(m, n, k, ta, tb, dst=None, btb=False):
     return eng
 
 
-def ozaki_grid_reference(m, n, k, p, proc_grid, opA, opB, nmod):
+def ozaki_grid_reference(m, n, k, p, proc_grid, opA, opB, nmod, is_btb=False):
     """The Ozaki form of the simulated grid, restated: every C block of the
     2D grid is the sum over the k layers, in layer order, of the CRT-form
-    product (oracle/ozaki.py, ozaki2_dgemm) of the layer's operand blocks."""
+    product (oracle/ozaki.py, ozaki2_dgemm) of the layer's operand blocks
+    (for B^T B: opA = B^T, opB = B on the B^T B grid)."""
     from oracle import ozaki as Z
-    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=proc_grid)
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=proc_grid, is_btb=is_btb)
     out = np.full((m, n), np.nan)
     for pl in w["plans"]:
         if not pl.is_active or pl.rank_k != 0 or not pl.C_2dmm_nrow or not pl.C_2dmm_ncol:

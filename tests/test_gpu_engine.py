@@ -320,3 +320,39 @@ def test_sim_random_block_layouts(hip):
                 got[sr:sr + nr, sc:sc + nc] = b.numpy()[:nr, :nc]
             sim.free()
             assert np.array_equal(got, want), (case, grouped)
+
+
+@pytest.mark.parametrize("mode", ["grouped-persistent", "grouped-batched", "ozaki-15", "ozaki-auto"])
+def test_sim_btb_other_forms(hip, mode):
+    """B^T B on a 12-rank grid through the grouped forms (persistent kernel,
+    batched candidate; exact on integers) and the Ozaki form (15 moduli, and
+    the automatic count; bit for bit the restated grid at that count, whose
+    FP64 reconstruction need not be exact), eager and replayed from the
+    exec's hipGraph."""
+    from oracle import oracle as O
+    from tests.simutil import ozaki_grid_reference
+    rnd = np.random.default_rng(1212)
+    p, pg, n, k = 12, [2, 2, 3], 150, 1200
+    B = _ints(rnd, (k, n))
+    w = O.ca3dmm_world(n, n, k, 0, 0, p, proc_grid=pg, is_btb=True)
+    dB = hip.DeviceArray.from_numpy(B)
+    print(f"[btb] {mode}", flush=True)
+    for graph in (False, True):
+        dC = hip.DeviceArray.filled((n, n), 0xFF)
+        sim = E.SimGrid(p, n, n, k, 0, 0, None, w["src_B"], w["dst_C"], proc_grid=pg, is_btb=True, allocate=True)
+        if mode.startswith("ozaki"):
+            sim.set_ozaki(15 if mode == "ozaki-15" else -1)
+        sim.set_mode(grouped=True, graph=graph)
+        if mode.startswith("grouped"):
+            sim.set_tuning(4, 2, 0 if mode == "grouped-persistent" else 3)
+        for _ in range(2 if graph else 1):
+            sim.exec(None, None, [dB.at(r[2] * k) for r in w["src_B"]], [k] * p,
+                     [dC.at(r[2] * n) for r in w["dst_C"]], [n] * p)
+        N = sim.get_ozaki()[0]
+        sim.free()
+        if mode.startswith("ozaki"):
+            assert (N == 15) if mode == "ozaki-15" else (4 <= N <= 25)
+            want = ozaki_grid_reference(n, n, k, p, pg, np.ascontiguousarray(B.T), B, N, is_btb=True)
+        else:
+            want = B.T @ B
+        assert np.array_equal(dC.numpy(), want), (mode, graph)
