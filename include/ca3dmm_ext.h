@@ -45,7 +45,8 @@ int hip_fold_sum(int nsrc, const double *const *P, const int *ldp,
 /* hip_dgemm whose epilogue also absorbs k-layer partials:
  * C := fold(P[0..nsrc-1]) + (alpha*op(A)*op(B) + beta*C), left to right.
  * With beta == 0 and the last layer's product this equals hip_fold_sum of
- * all kp partials bit for bit. */
+ * all kp partials bit for bit.  0 <= nsrc <= 8 (HIP_PROXY_ERR_ARG above;
+ * fold more layers with hip_fold_sum). */
 int hip_dgemm_fold(hip_proxy_enum_t transA, hip_proxy_enum_t transB,
                    const int m, const int n, const int k, const double alpha,
                    const double *A, const int ldA, const double *B, const int ldB,
