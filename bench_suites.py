@@ -227,7 +227,7 @@ def rank_share(args):
     run = lambda: sim.exec(Ap, [8000] * 24, Bp, [8000] * 24, Cp, [8000] * 24)
     fl = 2.0 * 8000 ** 3
     ms = timed(run, reps=5)
-    emit({"suite": "rank-share", "what": "8000^3 on 4x2x3, per-rank per-round launches, whole exec",
+    emit({"suite": "rank-share", "what": "8000^3 on 4x2x3, per-rank mode (all ranks' rounds in one batched launch), whole exec",
           "ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)})
     sim.set_mode(grouped=True, graph=False)
     g, wg, tile = sim.autotune(3)
@@ -351,7 +351,7 @@ def grouped(args):
     hip.set_sync(False)
     ms = timed(run, reps=3)
     ref_c = dC.numpy()
-    emit(dict({"suite": "grouped", "mode": "per-rank, per-round launches", "ms": round(ms, 3),
+    emit(dict({"suite": "grouped", "mode": "per-rank: all ranks' rounds in one batched launch", "ms": round(ms, 3),
                "tflops": round(flops / ms / 1e9, 2)}, **errs()))
     sim.set_mode(grouped=True, graph=False)
     sim.set_tuning(1, 1)
