@@ -82,6 +82,39 @@ def test_dgemm_general_alpha_beta_rounded_separately(hip):
     assert np.array_equal(dC.numpy(), want)
 
 
+def test_dgemm_fold_random_cases(hip):
+    """Seeded random shapes, trans cases, alpha, beta and 0..8 partials: the
+    fused epilogue gives fold(P) + (alpha op(A) op(B) + beta C) bit for bit
+    (as hip_fold_sum of the partials and a separate product), exact on
+    integers."""
+    rnd = np.random.default_rng(9191)
+    for case in range(10):
+        m, n = (int(v) for v in rnd.integers(1, 400, 2))
+        k = int(rnd.integers(0, 500))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        nsrc = int(rnd.integers(0, 9))
+        alpha, beta = float(rnd.integers(-2, 3)), float(rnd.integers(-2, 3))
+        A = rnd.integers(-4, 5, (k, m) if ta else (m, k)).astype(np.float64)
+        B = rnd.integers(-4, 5, (n, k) if tb else (k, n)).astype(np.float64)
+        C0 = rnd.integers(-50, 51, (m, n)).astype(np.float64)
+        P = [rnd.integers(-1000, 1001, (m, n)).astype(np.float64) for _ in range(nsrc)]
+        dA = hip.DeviceArray.from_numpy(A if A.size else np.zeros((max(1, A.shape[0]), 1)))
+        dB = hip.DeviceArray.from_numpy(B if B.size else np.zeros((max(1, B.shape[0]), 1)))
+        dC = hip.DeviceArray.from_numpy(C0)
+        dP = [hip.DeviceArray.from_numpy(x) for x in P]
+        print(f"[fold] case {case}: m={m} n={n} k={k} ta={ta} tb={tb} nsrc={nsrc} alpha={alpha} beta={beta}",
+              flush=True)
+        hip.dgemm_fold(ta, tb, m, n, k, alpha, dA, dB, beta, dC, dP)
+        prod = (A.T if ta else A) @ (B.T if tb else B) if k else np.zeros((m, n))
+        want = alpha * prod + beta * C0
+        if nsrc:
+            s = P[0].copy()
+            for t in range(1, nsrc):
+                s = s + P[t]
+            want = s + want
+        assert np.array_equal(dC.numpy(), want), case
+
+
 def test_dgemm_k_zero_scales_c(hip):
     Cm = np.arange(12.0).reshape(3, 4)
     dC = hip.DeviceArray.from_numpy(Cm)
