@@ -259,3 +259,30 @@ def test_cholesky_qr_random_shapes_and_padding(hip):
         assert np.array_equal(np.tril(R, -1), np.zeros((n, n))), case
         assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13 * max(1.0, np.sqrt(n)), case
         assert np.linalg.norm(A - Q @ R) / np.linalg.norm(A) < 1e-14 * max(1.0, np.sqrt(n)), case
+
+
+def test_gram_random_cases(hip):
+    """Seeded random n, k, alpha, beta, mirror and padded leading dimensions
+    of A and C: the upper triangle is alpha A^T A + beta C, the lower
+    triangle its mirror or untouched, the padding untouched; exact on
+    integers."""
+    rnd = np.random.default_rng(2718)
+    for case in range(10):
+        n, k = int(rnd.integers(1, 300)), int(rnd.integers(1, 5000))
+        alpha, beta = float(rnd.integers(-2, 3)), float(rnd.integers(-2, 3))
+        mirror = bool(rnd.integers(0, 2))
+        lda, ldc = k + int(rnd.integers(0, 4)), n + int(rnd.integers(0, 4))
+        A = rnd.integers(-3, 4, (k, n)).astype(np.float64)
+        C0 = rnd.integers(-9, 10, (ldc, n)).astype(np.float64)
+        bufA = np.full((lda, n), 5.0, order="F")
+        bufA[:k] = A
+        dA, dC = hip.DeviceArray.from_numpy(bufA), hip.DeviceArray.from_numpy(np.asfortranarray(C0))
+        print(f"[gram] case {case}: n={n} k={k} alpha={alpha} beta={beta} mirror={mirror}", flush=True)
+        hip.gram(n, k, dA, dC, alpha=alpha, beta=beta, mirror=mirror, lda=lda, ldc=ldc)
+        got = dC.numpy()
+        full = alpha * (A.T @ A) + beta * C0[:n]
+        iu, il = np.triu_indices(n), np.tril_indices(n, -1)
+        assert np.array_equal(got[:n][iu], full[iu]), case
+        lower_want = full.T[il] if mirror else C0[:n][il]
+        assert np.array_equal(got[:n][il], lower_want), case
+        assert np.array_equal(got[n:], C0[n:]), case

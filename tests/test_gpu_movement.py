@@ -177,3 +177,29 @@ def test_copy_long_rows_and_mixed_alignment(hip):
         want[300001 + r * 150000:300001 + r * 150000 + 149999] = src[300001 + r * 150000:300001 + r * 150000 + 149999]
     want[600002:600002 + 3 * 33333] = src[600002:600002 + 3 * 33333]
     assert np.array_equal(got, want)
+
+
+def test_dgeam_random_cases(hip):
+    """hip_dgeam (the reference's cuda_cublas_dgeam, used as a transpose):
+    C := alpha op(A) + beta op(B) over seeded random shapes, trans cases,
+    scalars (beta = 0 ignores B) and padded leading dimensions; exact on
+    integers, the padding of C untouched."""
+    from ca3dmm_amd._lib import lib
+    rnd = np.random.default_rng(1414)
+    for case in range(12):
+        m, n = (int(v) for v in rnd.integers(1, 700, 2))
+        ta, tb = (int(v) for v in rnd.integers(0, 2, 2))
+        alpha, beta = float(rnd.integers(-3, 4)), float(rnd.integers(-3, 4))
+        A = rnd.integers(-50, 51, (n, m) if ta else (m, n)).astype(np.float64)
+        B = rnd.integers(-50, 51, (n, m) if tb else (m, n)).astype(np.float64)
+        lda, ldb, ldc = A.shape[0] + int(rnd.integers(0, 3)), B.shape[0] + int(rnd.integers(0, 3)), m + 2
+        bA = np.zeros((lda, A.shape[1]), order="F"); bA[:A.shape[0]] = A
+        bB = np.zeros((ldb, B.shape[1]), order="F"); bB[:B.shape[0]] = B
+        C0 = np.full((ldc, n), 7.0, order="F")
+        dA, dB, dC = (hip.DeviceArray.from_numpy(x) for x in (bA, bB, C0))
+        rc = lib.hip_dgeam(ta, tb, m, n, alpha, dA.ptr, lda, beta, dB.ptr, ldb, dC.ptr, ldc)
+        assert rc == 0, case
+        got = dC.numpy()
+        want = alpha * (A.T if ta else A) + beta * (B.T if tb else B)
+        assert np.array_equal(got[:m], want), case
+        assert np.all(got[m:] == 7.0), case
