@@ -286,3 +286,42 @@ def test_gram_random_cases(hip):
         lower_want = full.T[il] if mirror else C0[:n][il]
         assert np.array_equal(got[:n][il], lower_want), case
         assert np.array_equal(got[n:], C0[n:]), case
+
+
+def test_potrf_trtri_trsm_random_padded(hip):
+    """Seeded random sizes with padded leading dimensions: the upper
+    Cholesky factor satisfies R^T R = S (lower triangle and padding
+    untouched), R X = I for X = R^{-1}, and the right solve gives X R =
+    alpha B, each to working precision."""
+    rnd = np.random.default_rng(3141)
+    for case in range(10):
+        n, m = int(rnd.integers(1, 700)), int(rnd.integers(1, 3000))
+        X = rnd.standard_normal((2 * n + 3, n))
+        S = X.T @ X + n * np.eye(n)
+        ld = n + int(rnd.integers(1, 4))
+        bufS = np.full((ld, n), 3.0, order="F")
+        bufS[:n] = S
+        dS = hip.DeviceArray.from_numpy(bufS)
+        print(f"[solve] case {case}: n={n} m={m}", flush=True)
+        assert hip.potrf_upper(n, dS) == 0, case
+        got = dS.numpy()
+        R = np.triu(got[:n])
+        assert np.array_equal(np.tril(got[:n], -1), np.tril(S, -1)), case
+        assert np.all(got[n:] == 3.0), case
+        assert np.linalg.norm(R.T @ R - S) <= 1e-14 * n * np.linalg.norm(S), case
+        dR = hip.DeviceArray.from_numpy(np.asfortranarray(np.vstack([R, np.zeros((ld - n, n))])))
+        dX = hip.DeviceArray.from_numpy(np.full((ld, n), 3.0, order="F"))
+        hip.trtri_upper(n, dR, dX)
+        Xi = dX.numpy()
+        assert np.all(Xi[n:] == 3.0), case
+        assert np.linalg.norm(R @ np.triu(Xi[:n]) - np.eye(n)) <= 1e-13 * n, case
+        alpha = float(rnd.choice([1.0, -0.5, 2.0]))
+        B = rnd.standard_normal((m, n))
+        ldb = m + int(rnd.integers(1, 4))
+        bufB = np.full((ldb, n), 3.0, order="F")
+        bufB[:m] = B
+        dB = hip.DeviceArray.from_numpy(bufB)
+        hip.trsm_right_upper(m, n, alpha, dR, dB)
+        Y = dB.numpy()
+        assert np.all(Y[m:] == 3.0), case
+        assert np.linalg.norm(Y[:m] @ R - alpha * B) <= 1e-13 * n * np.linalg.norm(alpha * B), case
