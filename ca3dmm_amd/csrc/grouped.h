@@ -10,8 +10,8 @@
 namespace ca3 {
 namespace grouped {
 
-constexpr int kMaxSeg = 8;    // Cannon rounds per problem
-constexpr int kMaxKp = 8;     // k layers per fold group
+constexpr int kMaxSeg = 16;   // Cannon rounds per problem
+constexpr int kMaxKp = 16;    // k layers per fold group
 
 struct Seg {                  // one k-segment (Cannon round) of a problem
     const double *A;          // m x k, column-major, leading dimension lda

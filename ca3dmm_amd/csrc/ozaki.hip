@@ -1339,7 +1339,7 @@ __global__ __launch_bounds__(256) void ozaki2_crt_kernel(const int8_t *__restric
 // layer products of the same C block, added in layer order, each column
 // written to its owner's C_out slice) or per problem (kp = 1).  A job's
 // workgroups take 4 blocks each, as ozaki2_crt_kernel.
-constexpr int kGrpMaxKp = 8;
+constexpr int kGrpMaxKp = 16;
 struct CrtJob {
     int kp, M, N, Mp, nbm, nwg;           // nbm: ceil(M / 16); nwg: workgroups of the job
     const int8_t *R[kGrpMaxKp];           // the layers' residue planes (rplane apart)

@@ -58,9 +58,9 @@ void *ca3dmm_sim_c_out(ca3dmm_sim_p sim, int rank);
  * queue (rounds accumulated in registers; the last layer to finish a tile
  * adds the kp layer tiles in layer order), or as the tuner's batched
  * candidate (see ca3dmm_sim_autotune).  The grouped mode and the Ozaki form
- * hold at most 8 k layers and 8 Cannon rounds per rank; exec returns
+ * hold at most 16 k layers and 16 Cannon rounds per rank; exec returns
  * HIP_PROXY_ERR_UNSUPP on larger grids (the default mode has no such
- * limit: beyond 8 rounds it launches per round).  use_graph != 0: capture the whole
+ * limit: beyond 16 rounds it launches per round).  use_graph != 0: capture the whole
  * exec -- redistributions, transposes, allgathers, products, fold -- into one
  * hipGraph on the first call (again when an argument or setting changes)
  * and replay it.  Default: every rank's rounds in one batched DGEMM launch

@@ -198,8 +198,8 @@ def test_sim_random_grids_batched_paths_same_bits(hip, monkeypatch):
         batched = run_sim(p, m, n, k, ta, tb, A, B)
         assert np.array_equal(batched, per_round), case
         w = O.ca3dmm_world(m, n, k, ta, tb, p)
-        if w["plans"][0].task_k_num > 8:
-            continue                          # the grouped plans hold at most 8 k layers
+        if w["plans"][0].task_k_num > 16:
+            continue                          # the grouped plans hold at most 16 k layers
         out = []
         for tile in (0, 3):
             dA, dB = hip.DeviceArray.from_numpy(A), hip.DeviceArray.from_numpy(B)
@@ -218,23 +218,53 @@ def test_sim_random_grids_batched_paths_same_bits(hip, monkeypatch):
         assert np.all(np.abs(out[0] - opA @ opB) <= bound + 2.0 ** -53 * np.abs(opA @ opB)), case
 
 
-def test_sim_more_rounds_than_a_batch_item_holds(hip, monkeypatch):
-    """A 9 x 9 Cannon group (nine rounds per rank, more than one batched
-    item holds): the per-rank mode falls back to one launch per rank and
-    round, with the same bits as asked for explicitly."""
+@pytest.mark.parametrize("q", [9, 17])
+def test_sim_more_rounds_than_a_batch_item_holds(hip, monkeypatch, q):
+    """A q x q Cannon group: with 9 rounds per rank the per-rank mode's
+    batched launch holds them; with 17, more than a batched item holds, it
+    falls back to one launch per rank and round.  Either way the bits of
+    the launches per round asked for explicitly."""
     from oracle import oracle as O
-    rnd = np.random.default_rng(99)
-    m, n, k, p = 300, 270, 333, 81
-    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=[9, 9, 1])
-    assert max(pl.cannon.np_dim for pl in w["plans"] if pl.is_active) == 9
+    rnd = np.random.default_rng(99 + q)
+    m, n, k, p = 300, 270, 333, q * q
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=[q, q, 1])
+    assert max(pl.cannon.np_dim for pl in w["plans"] if pl.is_active) == q
     A, B = rnd.uniform(-1, 1, (m, k)), rnd.uniform(-1, 1, (k, n))
-    print("[batch] 81 ranks on 9 x 9 x 1", flush=True)
-    got = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=[9, 9, 1])
+    print(f"[batch] {p} ranks on {q} x {q} x 1", flush=True)
+    got = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=[q, q, 1])
     monkeypatch.setenv("CA3DMM_SIM_BATCH", "0")
-    want = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=[9, 9, 1])
+    want = run_sim(p, m, n, k, 0, 0, A, B, proc_grid=[q, q, 1])
     assert np.array_equal(got, want)
     bound = k * 2.0 ** -53 * (np.abs(A) @ np.abs(B))
     assert np.all(np.abs(got - A @ B) <= bound + 2.0 ** -53 * np.abs(A @ B))
+
+
+@pytest.mark.parametrize("grid", [[9, 9, 1], [1, 1, 12], [2, 1, 16]])
+def test_sim_grouped_and_ozaki_many_rounds_or_layers(hip, grid):
+    """Grids with 9 Cannon rounds per rank or 12 and 16 k layers through the
+    grouped forms (persistent kernel and batched candidate; exact on
+    integers) and the Ozaki form (bit for bit the restated grid)."""
+    from oracle import oracle as O
+    from tests.simutil import ozaki_grid_reference
+    p = grid[0] * grid[1] * grid[2]
+    m, n, k = 300, 270, 900
+    rnd = np.random.default_rng(p + grid[2])
+    Ai, Bi = _ints(rnd, (m, k)), _ints(rnd, (k, n))
+    Af, Bf = np.asfortranarray(rnd.uniform(-1, 1, (m, k))), np.asfortranarray(rnd.uniform(-1, 1, (k, n)))
+    w = O.ca3dmm_world(m, n, k, 0, 0, p, proc_grid=grid)
+    print(f"[grouped] grid {grid}", flush=True)
+    for tile in (0, 3):
+        dA, dB = hip.DeviceArray.from_numpy(Ai), hip.DeviceArray.from_numpy(Bi)
+        dC = hip.DeviceArray.filled((m, n), 0xFF)
+        sim = E.SimGrid(p, m, n, k, 0, 0, w["src_A"], w["src_B"], w["dst_C"], proc_grid=grid, allocate=True)
+        sim.set_mode(grouped=True, graph=False)
+        sim.set_tuning(4, 2, tile)
+        sim.exec([dA.at(r[2] * m) for r in w["src_A"]], [m] * p, [dB.at(r[2] * k) for r in w["src_B"]], [k] * p,
+                 [dC.at(r[2] * m) for r in w["dst_C"]], [m] * p)
+        sim.free()
+        assert np.array_equal(dC.numpy(), Ai @ Bi), tile
+    got = run_sim(p, m, n, k, 0, 0, Af, Bf, proc_grid=grid, grouped=True, ozaki=14)
+    assert np.array_equal(got, ozaki_grid_reference(m, n, k, p, grid, Af, Bf, 14))
 
 
 def test_single_rank_engine_random_layouts(hip, monkeypatch):
