@@ -176,3 +176,40 @@ def test_ozaki_rejects_bad_slices(hip):
         hip.ozaki_dgemm(0, 0, 4, 4, 4, 1.0, d, d, 0.0, d, 0)
     with pytest.raises(HipError):
         hip.ozaki_dgemm(0, 0, 4, 4, 4, 1.0, d, d, 0.0, d, 17)
+
+
+class _At:
+    """A device address inside a DeviceArray, for hip_ozaki_gram's A / C."""
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+def test_ozaki_gram_padded_layouts(hip):
+    """Seeded random n, k, s, scalars and mirror with A and C at padded
+    leading dimensions and odd offsets into their buffers: the upper
+    triangle is the specification's, the lower its mirror or untouched, the
+    padding untouched."""
+    rnd = np.random.default_rng(4711)
+    for case in range(6):
+        n, k, s = int(rnd.integers(1, 260)), int(rnd.integers(1, 3000)), int(rnd.integers(2, 10))
+        mirror = bool(rnd.integers(0, 2))
+        lda, ldc, oa, oc = k + int(rnd.integers(0, 4)), n + int(rnd.integers(1, 4)), int(rnd.integers(0, 4)), 1
+        A = Z.phi_matrix(rnd, (k, n), 2.0)
+        Cin = rnd.standard_normal((n, n))
+        bufA = np.full(oa + lda * n, 9.0)
+        for c in range(n):
+            bufA[oa + c * lda:oa + c * lda + k] = A[:, c]
+        bufC = np.full(oc + ldc * n, 9.0)
+        for c in range(n):
+            bufC[oc + c * ldc:oc + c * ldc + n] = Cin[:, c]
+        dA, dC = hip.DeviceArray.from_numpy(bufA), hip.DeviceArray.from_numpy(bufC)
+        print(f"[ozaki gram] padded case {case}: n={n} k={k} s={s} mirror={mirror}", flush=True)
+        hip.ozaki_gram(n, k, _At(dA.at(oa)), _At(dC.at(oc)), s, alpha=-1.5, beta=0.5, mirror=mirror,
+                       lda=lda, ldc=ldc)
+        out = dC.numpy()
+        got = out[oc:].reshape(n, ldc).T
+        want = Z.ozaki_dgemm8(np.ascontiguousarray(A.T), A, s, alpha=-1.5, beta=0.5, Cin=Cin)
+        iu, il = np.triu_indices(n), np.tril_indices(n, -1)
+        assert np.array_equal(got[:n][iu], want[iu]), case
+        assert np.array_equal(got[:n][il], (want.T if mirror else Cin)[il]), case
+        assert np.all(got[n:] == 9.0) and out[0] == 9.0, case
