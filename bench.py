@@ -42,14 +42,17 @@ def measured_traffic(kernel, n):
     except (OSError, ValueError, KeyError):
         return None
     return None
-def grid_line(hip, n, dA, dB, dC):
+def grid_line(hip, n, dA, dB, dC, host=None):
     """A CA3DMM grid configuration measured whole: n^3 NN on the simulated
     4x2x3 grid (24 ranks, 1D column layouts in and out), one ca3dmm_sim
     exec = redistributions, allgather, 48 Cannon rounds, 24 kp = 3 folds
     and the C redistribution; the per-rank mode (all ranks' rounds as one
     batched launch), then the grouped mode's batched candidate and its
     persistent register-staged kernel, each replayed from one hipGraph.
-    The first n x n of the operands are the inputs."""
+    The first n x n of the operands are the inputs.  host = (A, B): the
+    operands on the host; then the Ozaki form with the automatic modulus
+    count (grouped, replayed from a hipGraph) is timed too, and both forms'
+    errors are sampled on 512 entries against double-double."""
     from ca3dmm_amd import engine as E
     ld, p = dA.shape[0], 24
 
@@ -89,6 +92,29 @@ def grid_line(hip, n, dA, dB, dC):
                 run()
             ms = t.stop() / 5
             out[mode] = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 2)}
+        if host is not None:
+            import numpy as np
+            rs = np.random.default_rng(7)
+            ii, jj = rs.integers(0, n, 512), rs.integers(0, n, 512)
+            ref, ab = dd_sample(0, 0, host[0], host[1], ii, jj)
+
+            def errs():
+                hip.sync()
+                e = np.abs(dC.sample(ii, jj) - ref)
+                return {"max_err_over_absAabsB": float(np.max(e / ab)), "max_rel_err": float(np.max(e / np.abs(ref)))}
+            out["grouped_graph"].update(errs())          # the last exec ran the persistent kernel: same bits
+            sim.set_ozaki(-1)
+            sim.set_mode(grouped=True, graph=True)
+            run()
+            hip.sync()
+            t = hip.Timer()
+            t.start()
+            for _ in range(5):
+                run()
+            ms = t.stop() / 5
+            out["ozaki_auto_grouped_graph"] = dict({"ms": round(ms, 3), "effective_tflops": round(fl / ms / 1e9, 2),
+                                                     "nmoduli": sim.get_ozaki()[0]}, **errs())
+            sim.set_ozaki(0)
     finally:
         hip.set_sync(True)
         sim.free()
@@ -387,10 +413,11 @@ def main():
         # the 10007^3 grid (uneven 2502/2501 x 5004/5003 cuboids): operands
         # of its own size, uniform[-1,1]
         log("bench: the 10007^3 grid on 4x2x3 (24 simulated ranks)")
-        big = [hip.DeviceArray.from_numpy(rnd.uniform(-1, 1, (10007, 10007))) for _ in range(2)]
+        hostAB = [np.asfortranarray(rnd.uniform(-1, 1, (10007, 10007))) for _ in range(2)]
+        big = [hip.DeviceArray.from_numpy(x) for x in hostAB]
         big.append(hip.DeviceArray.zeros((10007, 10007)))
-        grid_10007 = grid_line(hip, 10007, *big)
-        del big
+        grid_10007 = grid_line(hip, 10007, *big, host=hostAB)
+        del big, hostAB
 
     cholqr = None
     if not args.no_cholqr:
