@@ -40,8 +40,9 @@ CASES = [  # p, m, n, k, transA, transB, btb, proc_grid
 ]
 
 
-def _world(p, m, n, k, ta, tb, btb, pg):
-    w = O.ca3dmm_world(m, n, k, ta, tb, p, proc_grid=pg, is_btb=btb)
+def _world(p, m, n, k, ta, tb, btb, pg, layouts=None):
+    kw = dict(zip(("src_A", "src_B", "dst_C"), layouts)) if layouts else {}
+    w = O.ca3dmm_world(m, n, k, ta, tb, p, proc_grid=pg, is_btb=btb, **kw)
     srcA = w["src_A"] if not btb else None
     sim = E.SimGrid(p, m, n, k, ta, tb, srcA, w["src_B"], w["dst_C"], proc_grid=pg, is_btb=btb,
                     allocate=False)
@@ -50,8 +51,38 @@ def _world(p, m, n, k, ta, tb, btb, pg):
 
 @pytest.mark.parametrize("case", CASES, ids=[str(c[:7]) for c in CASES])
 def test_every_rank_matches_oracle(case):
-    p, m, n, k, ta, tb, btb, pg = case
-    w, sim = _world(p, m, n, k, ta, tb, btb, pg)
+    _check_against_oracle(*case)
+
+
+def _rects(rnd, nrow, ncol, p):
+    """A random pr x pc block partition (pr pc = p), blocks dealt to the
+    ranks in random order, empty blocks possible: (srow, nrow, scol, ncol)."""
+    pr = int(rnd.choice([d for d in range(1, p + 1) if p % d == 0]))
+    cut = lambda length, parts: [0] + sorted(int(v) for v in rnd.integers(0, length + 1, parts - 1)) + [length]
+    rc, cc = cut(nrow, pr), cut(ncol, p // pr)
+    rects = [(rc[i], rc[i + 1] - rc[i], cc[j], cc[j + 1] - cc[j]) for i in range(pr) for j in range(p // pr)]
+    return [rects[o] for o in rnd.permutation(p)]
+
+
+def test_random_worlds_match_oracle():
+    """Seeded random rank counts, shapes, trans cases, B^T B and random 2-D
+    block layouts of the inputs and of C: every rank's plan, Cannon blocks,
+    redistribution lists and workspace equal the oracle's."""
+    rnd = np.random.default_rng(8128)
+    for case in range(30):
+        p = int(rnd.integers(1, 33))
+        m, n, k = (int(v) for v in rnd.integers(1, 5000, 3))
+        btb = bool(rnd.integers(0, 5) == 0)
+        ta, tb = (0, 0) if btb else tuple(int(v) for v in rnd.integers(0, 2, 2))
+        layouts = None
+        if not btb and rnd.integers(0, 2):
+            layouts = (_rects(rnd, *((k, m) if ta else (m, k)), p), _rects(rnd, *((n, k) if tb else (k, n)), p),
+                       _rects(rnd, m, n, p))
+        _check_against_oracle(p, 0 if btb else m, n, k, ta, tb, btb, None, layouts)
+
+
+def _check_against_oracle(p, m, n, k, ta, tb, btb, pg, layouts=None):
+    w, sim = _world(p, m, n, k, ta, tb, btb, pg, layouts)
     for r in range(p):
         e, pl = sim.engine(r), w["plans"][r]
         assert e.my_rank == r and e.n_proc == p
