@@ -378,8 +378,16 @@ def test_sim_btb_other_forms(hip, mode):
         for _ in range(2 if graph else 1):
             sim.exec(None, None, [dB.at(r[2] * k) for r in w["src_B"]], [k] * p,
                      [dC.at(r[2] * n) for r in w["dst_C"]], [n] * p)
-        N = sim.get_ozaki()[0]
+        N, _, L = sim.get_ozaki()
         sim.free()
+        if mode == "ozaki-auto":
+            # the count is the oracle's rule on B^T and B over the plan's k layers
+            from oracle import ozaki as Z
+            kp = w["plans"][0].task_k_num
+            layers = [O.block_spos_size(k, kp, l) for l in range(kp)]
+            Lo, kmax = Z.grid_spread(np.ascontiguousarray(B.T), B, layers)
+            assert L == pytest.approx(Lo, rel=1e-9, abs=1e-9), (L, Lo)
+            assert N == Z.counts_from_spread(Lo, 0.0, kmax)[1], (N, Lo)
         if mode.startswith("ozaki"):
             assert (N == 15) if mode == "ozaki-15" else (4 <= N <= 25)
             want = ozaki_grid_reference(n, n, k, p, pg, np.ascontiguousarray(B.T), B, N, is_btb=True)
