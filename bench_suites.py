@@ -420,18 +420,15 @@ def cholqr(args):
             rec = {"suite": "cholqr", "m": m, "n": n, "cond": cond, "variant": name,
                    "gram": f"ozaki-int8 s={gs}" if gs else "fp64-mfma", "gram_ms": round(ms_gram, 3),
                    "gram_tflops": round(n * (n + 1) * m / ms_gram / 1e9, 2)}
-            try:
-                hip.sync()
-                t = hip.Timer()
-                t.start()
-                dQ, dR, passes = CQ.cholesky_qr(dA, m, n, variant, gram_slices=gs)
-                rec["ms"] = round(t.stop(), 2)
+            ms, dQ, dR, passes, info = _bench().time_cholesky_qr(hip, dA, m, n, variant, gram_slices=gs or 0, reps=1)
+            if info:
+                rec.update({"breakdown": f"Cholesky breakdown in pass {passes + 1}", "info": info, "pass": passes + 1})
+            else:
+                rec["ms"] = round(ms, 2)
                 rec["passes"] = passes
                 orth, res = CQ.qr_errors(dA, dQ, dR, m, n)
                 rec.update({"orth_QtQ_minus_I_F": orth, "residual_rel_F": res})
-                del dQ, dR
-            except CQ.CholeskyBreakdown as ex:
-                rec.update({"breakdown": str(ex), "info": ex.info, "pass": ex.step})
+            del dQ, dR
             emit(rec)
         del dA, dG
 
